@@ -1,0 +1,203 @@
+/*
+ * itg.h -- C ABI of libitg.so, the MI355X (gfx950) backend for NDTensors'
+ * tensor contraction (dense and QN block-sparse).
+ *
+ * Conventions shared by every entry point
+ *  - Tensors are column-major (first mode fastest), exactly as NDTensors'
+ *    `Dense` / `BlockSparse` storages hold them. Complex elements are
+ *    interleaved (re, im) pairs of doubles, i.e. Julia's ComplexF64.
+ *  - All data pointers are DEVICE pointers owned by the caller. The library
+ *    owns only handles, plans, device-side plan tables and its workspace;
+ *    each is released by the matching destroy call.
+ *  - Labels follow the ncon convention that
+ *    ITensors.compute_contraction_labels produces (src/indexset.jl:672-707):
+ *    a label shared by T1 and T2 is contracted (ITensors makes it negative),
+ *    every other label appears exactly once in R. Label values are Julia Int.
+ *  - Scalars (alpha, beta) are passed as two doubles {re, im}; the imaginary
+ *    part is ignored for real dtypes.
+ *  - beta == 0 never reads R: R may hold NaN/Inf on entry
+ *    (NDTensors/test/test_dense.jl:237-259).
+ *  - Every function returns an itg_status; no C++ exception crosses the ABI.
+ *    itg_last_error() gives the message of the last failure on that handle.
+ *  - A handle is not thread-safe: use one handle per host thread, or
+ *    serialise calls. Work is enqueued on the handle's stream (default:
+ *    the null stream); calls return once the work is enqueued unless noted.
+ */
+#ifndef ITG_H
+#define ITG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int itg_status;
+enum {
+  ITG_OK = 0,
+  ITG_ERR_INVALID = 1,     /* bad argument (shapes, labels, pointers)       */
+  ITG_ERR_UNSUPPORTED = 2, /* valid for NDTensors but not for this backend: */
+                           /* the caller falls back to its own path, as the */
+                           /* cuTENSOR extension does on CUTENSORError      */
+                           /* (NDTensors/ext/NDTensorscuTENSORExt/contract.jl:45-53) */
+  ITG_ERR_HIP = 3,         /* a HIP runtime call failed                     */
+  ITG_ERR_NOMEM = 4        /* device allocation of workspace failed         */
+};
+
+/* Element types (Julia eltypes). */
+typedef enum {
+  ITG_F64 = 1,  /* Float64    */
+  ITG_C128 = 3  /* ComplexF64 */
+} itg_dtype;
+
+/* Arithmetic form of the GEMM-shaped work. */
+typedef enum {
+  ITG_PREC_FP64 = 0,       /* FP64 MFMA (v_mfma_f64_16x16x4_f64), FP64 accumulate */
+  ITG_PREC_SPLIT_BF16 = 1, /* FP64 operands split into BF16 slices (per-row /   */
+                           /* per-column power-of-two scaling), slice products on */
+                           /* BF16 MFMA with exact FP32 accumulation, slices      */
+                           /* recombined and summed in FP64                       */
+  ITG_PREC_SPLIT_INT8 = 2  /* FP64 operands scaled to integers and reduced modulo */
+                           /* pairwise-coprime moduli; residue products on INT8   */
+                           /* MFMA (exact INT32 accumulation), CRT recombination  */
+                           /* in FP64                                             */
+} itg_prec;
+
+/* Execution flags for itg_bs_execute / itg_contract_dense_ex. */
+enum {
+  ITG_FLAG_CONJ_A = 1u, /* use conj(T1) (dag of complex data, src/itensor.jl:1806-1816) */
+  ITG_FLAG_CONJ_B = 2u  /* use conj(T2)                                                  */
+};
+
+typedef struct itg_handle itg_handle;
+typedef struct itg_bs_plan itg_bs_plan;
+
+/* ---------------------------------------------------------------- handle */
+
+/* Create a handle bound to HIP device `device`. No reference counterpart:
+ * it holds what the Julia extension would keep per task (stream, workspace). */
+itg_status itg_init(int device, itg_handle** out);
+itg_status itg_destroy(itg_handle* h);
+const char* itg_last_error(const itg_handle* h);
+/* Enqueue subsequent work on `stream` (a hipStream_t; NULL = null stream). */
+itg_status itg_set_stream(itg_handle* h, void* stream);
+itg_status itg_synchronize(itg_handle* h);
+/* Library version, "major.minor.patch". */
+const char* itg_version(void);
+
+/* --------------------------------------------------------------- permute */
+
+/* dst = alpha * permutedims(src, perm) + beta * dst.
+ * `dims` are the dimensions of src; perm is 1-based (Julia) and
+ * dst's mode i is src's mode perm[i].
+ * Replaces NDTensors.permutedims!(::Exposed{<:Array}, ...) with @strided
+ * (NDTensors/src/array/permutedims.jl:5-24) and the alpha/beta variants used
+ * by the scalar contraction paths
+ * (NDTensors/src/abstractarray/tensoralgebra/contract.jl:88-113). */
+itg_status itg_permute(itg_handle* h, itg_dtype dtype, int ndim,
+                       const int64_t* dims, const int32_t* perm,
+                       const void* src, void* dst,
+                       const double* alpha, const double* beta);
+
+/* ------------------------------------------------------ dense contraction */
+
+/* R = alpha * contract(T1, labelsT1, T2, labelsT2) + beta * R, with R's mode
+ * order given by labelsR.
+ * Replaces NDTensors.contract!(R::DenseTensor, labelsR, T1::DenseTensor,
+ * labelsT1, T2::DenseTensor, labelsT2, alpha, beta)
+ * (NDTensors/src/dense/tensoralgebra/contract.jl:160-216), i.e. the method
+ * the cuTENSOR extension overrides (NDTensors/ext/NDTensorscuTENSORExt/
+ * contract.jl:15-24). Covers the outer-product (N1+N2==NR) and
+ * scalar (nnz==1) cases of that method too. All three tensors must have the
+ * same dtype (the Julia side promotes first, as
+ * dense/tensoralgebra/contract.jl:196-211 does). */
+itg_status itg_contract_dense(itg_handle* h, itg_dtype dtype,
+                              int NR, const int64_t* dimsR, const int64_t* labelsR, void* R,
+                              int N1, const int64_t* dims1, const int64_t* labels1, const void* T1,
+                              int N2, const int64_t* dims2, const int64_t* labels2, const void* T2,
+                              const double* alpha, const double* beta, itg_prec prec);
+
+/* Same as itg_contract_dense with ITG_FLAG_CONJ_* flags (used by the adjoint,
+ * ext/ITensorsChainRulesCoreExt/itensor.jl:94-103, where dag(x) conjugates). */
+itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype,
+                                 int NR, const int64_t* dimsR, const int64_t* labelsR, void* R,
+                                 int N1, const int64_t* dims1, const int64_t* labels1, const void* T1,
+                                 int N2, const int64_t* dims2, const int64_t* labels2, const void* T2,
+                                 const double* alpha, const double* beta, itg_prec prec,
+                                 unsigned flags);
+
+/* -------------------------------------------------- block-sparse contract */
+
+/* Build the contraction plan of two BlockSparse tensors.
+ * Replaces contraction_output + contract_blockoffsets(Algorithm"sequential")
+ * (NDTensors/src/blocksparse/contract.jl:20-55,
+ *  NDTensors/src/blocksparse/contract_sequential.jl:1-41).
+ *
+ *  blocks1  : N1 x nblk1 block coordinates, 1-based, block b at
+ *             blocks1[b*N1 .. b*N1+N1-1] (the UInt tuples of Block{N1}),
+ *             in the insertion order of T1's BlockOffsets.
+ *  offs1    : nblk1 0-based element offsets of those blocks in T1's data.
+ *  nsec1    : N1 numbers of sectors per mode; secdims1 concatenates the
+ *             sector sizes of every mode (BlockDims, blockdims.jl:8-19).
+ *  (same for T2)
+ *  labels1/labels2/labelsR as for itg_contract_dense.
+ *
+ * The output blocks come out in the reference's order: first appearance while
+ * iterating block1 over T1's blocks (outer) and block2 over T2's blocks
+ * (inner); offsets accumulate in that order. Sector sizes of a contracted
+ * mode must agree between T1 and T2. */
+itg_status itg_bs_plan_create(itg_handle* h,
+                              int N1, int64_t nblk1, const int64_t* blocks1, const int64_t* offs1,
+                              const int64_t* nsec1, const int64_t* secdims1, const int64_t* labels1,
+                              int N2, int64_t nblk2, const int64_t* blocks2, const int64_t* offs2,
+                              const int64_t* nsec2, const int64_t* secdims2, const int64_t* labels2,
+                              int NR, const int64_t* labelsR,
+                              itg_bs_plan** out);
+
+/* Output structure: number of blocks, total nnz, number of block pairs. */
+itg_status itg_bs_plan_sizes(const itg_bs_plan* p, int64_t* nblkR, int64_t* nnzR,
+                             int64_t* npairs);
+/* blocksR: NR x nblkR 1-based coordinates; offsR: nblkR 0-based offsets. */
+itg_status itg_bs_plan_blocks(const itg_bs_plan* p, int64_t* blocksR, int64_t* offsR);
+/* pairs: 3 x npairs 0-based (block index in T1, in T2, in R), plan order. */
+itg_status itg_bs_plan_pairs(const itg_bs_plan* p, int64_t* pairs);
+
+/* R = alpha * (T1 * T2) on every output block of the plan (each output
+ * block is overwritten by its first pair and accumulated by the rest, as
+ * contract_generic.jl:78-129 does with beta = 0 then 1). R must have
+ * nnzR elements. `pair_alpha` is NULL or npairs {re,im} per-pair factors
+ * multiplying alpha (the slot compute_alpha fills for fermions,
+ * contract_utilities.jl:2-15). An empty plan returns without work.
+ * Replaces contract!(R, labelsR, T1, labelsT1, T2, labelsT2, plan)
+ * (NDTensors/src/blocksparse/contract.jl:57-76) and its per-pair loop. */
+itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype,
+                          void* R, const void* T1, const void* T2,
+                          const double* alpha, const double* pair_alpha,
+                          itg_prec prec, unsigned flags);
+
+itg_status itg_bs_plan_destroy(itg_bs_plan* p);
+
+/* Block-sparse permutedims: dst block b (same order and offsets as the caller
+ * passes in offsR) = permutedims(src block b, perm). dims of each block are
+ * N x nblk (block b at dims[b*N..]). Replaces permutedims!(R::BlockSparseTensor,
+ * T::BlockSparseTensor, perm) (NDTensors/src/blocksparse/blocksparsetensor.jl:834-881)
+ * for the common case of matching block lists. */
+itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
+                          const int64_t* blockdims, const int64_t* offs_src,
+                          const int64_t* offs_dst, const int32_t* perm,
+                          const void* src, void* dst);
+
+/* ------------------------------------------------------------- autotuner */
+
+/* Time the GEMM tile shapes on this plan's pairs and keep the fastest per
+ * (shape bucket, dtype, prec); later executes of any plan use the cached
+ * choice. Synchronises the stream. */
+itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype,
+                           void* R, const void* T1, const void* T2, itg_prec prec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITG_H */

@@ -1,0 +1,626 @@
+// C ABI of libitg.so (declared in include/itg.h).
+//
+// Dense and block-sparse contractions both lower to an ExecPlan (itg_plan.h):
+// a dense call is a one-pair plan, cached per (dims, labels) so repeated calls
+// with the same shapes (the steps of a network applied many times) skip
+// planning and table uploads. A block-sparse plan is built once by
+// itg_bs_plan_create and executed any number of times.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "../../include/itg.h"
+#include "itg_plan.h"
+
+using namespace itg;
+
+struct itg_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  // dense plan cache (LRU)
+  std::list<std::pair<std::string, std::unique_ptr<ExecPlan>>> dense_lru;
+  std::unordered_map<std::string, decltype(dense_lru)::iterator> dense_map;
+  // autotuned tile cfg per (bucket, cplx, prec)
+  std::map<std::string, int> tuned;
+};
+
+struct itg_bs_plan {
+  int device = 0;
+  int NR = 0;
+  Structure st;
+  BlockList b1, b2, bR;
+  std::vector<int64_t> blocksR;  // NR x nblkR, 1-based
+  std::vector<PairSpec> pairs;
+  std::unique_ptr<ExecPlan> ep;      // pair_alpha = NULL
+  std::unique_ptr<ExecPlan> ep_alpha;  // last per-pair alpha variant
+  std::vector<double> ep_alpha_key;
+};
+
+namespace {
+
+itg_status fail(itg_handle* h, itg_status st, const std::string& msg) {
+  if (h) h->err = msg;
+  return st;
+}
+
+itg_status hip_fail(itg_handle* h, hipError_t e, const char* what) {
+  return fail(h, e == hipErrorOutOfMemory ? ITG_ERR_NOMEM : ITG_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool dtype_ok(itg_dtype dt) { return dt == ITG_F64 || dt == ITG_C128; }
+
+hipError_t ensure_ws(itg_handle* h, size_t bytes) {
+  if (bytes <= h->ws_bytes) return hipSuccess;
+  if (h->ws) {
+    // the old workspace may still be read by queued kernels
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return e;
+    e = hipFree(h->ws);
+    if (e != hipSuccess) return e;
+    h->ws = nullptr;
+    h->ws_bytes = 0;
+  }
+  size_t b = std::max(bytes, (size_t)1 << 20);
+  hipError_t e = hipMalloc(&h->ws, b);
+  if (e != hipSuccess) return e;
+  h->ws_bytes = b;
+  return hipSuccess;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+std::string tune_key(const ExecPlan& ep, int cplx, int prec) {
+  // bucket by the log2 of the largest block dim and the transpose form
+  int b = 0;
+  while ((int64_t(1) << b) < ep.max_dim && b < 40) ++b;
+  char buf[96];
+  std::snprintf(buf, sizeof buf, "b%d_c%d_p%d_t%d%d", b, cplx, prec, ep.ta, ep.tb);
+  return buf;
+}
+
+int pick_cfg(itg_handle* h, const ExecPlan& ep, int cplx, int prec) {
+  auto it = h->tuned.find(tune_key(ep, cplx, prec));
+  if (it != h->tuned.end()) return it->second;
+  return default_gemm_cfg(ep, cplx);
+}
+
+itg_status run_plan(itg_handle* h, ExecPlan& ep, int cplx, int prec, const void* T1,
+                    const void* T2, void* R, unsigned flags, const double* alpha,
+                    const double* beta) {
+  const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
+  const double b_re = beta ? beta[0] : 0.0, b_im = (beta && cplx) ? beta[1] : 0.0;
+  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx));
+  if (e != hipSuccess) return hip_fail(h, e, "workspace allocation");
+  std::string err;
+  const int cfg = pick_cfg(h, ep, cplx, prec);
+  e = exec_run(ep, cplx, prec, cfg, h->stream, h->ws, T1, T2, R, flags, a_re, a_im, b_re, b_im,
+               &err);
+  if (e == hipErrorNotSupported) return fail(h, ITG_ERR_UNSUPPORTED, err);
+  if (e != hipSuccess) return fail(h, ITG_ERR_HIP, err);
+  return ITG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* itg_version(void) { return "0.1.0"; }
+
+itg_status itg_init(int device, itg_handle** out) {
+  if (!out) return ITG_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return ITG_ERR_HIP;
+  if (device < 0 || device >= n) return ITG_ERR_INVALID;
+  itg_handle* h = new itg_handle;
+  h->device = device;
+  *out = h;
+  return ITG_OK;
+}
+
+itg_status itg_destroy(itg_handle* h) {
+  if (!h) return ITG_OK;
+  {
+    DeviceGuard g(h->device);
+    if (h->ws) {
+      (void)hipStreamSynchronize(h->stream);
+      (void)hipFree(h->ws);
+    }
+    h->dense_map.clear();
+    h->dense_lru.clear();
+  }
+  delete h;
+  return ITG_OK;
+}
+
+const char* itg_last_error(const itg_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+itg_status itg_set_stream(itg_handle* h, void* stream) {
+  if (!h) return ITG_ERR_INVALID;
+  h->stream = (hipStream_t)stream;
+  return ITG_OK;
+}
+
+itg_status itg_synchronize(itg_handle* h) {
+  if (!h) return ITG_ERR_INVALID;
+  DeviceGuard g(h->device);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
+  return ITG_OK;
+}
+
+itg_status itg_permute(itg_handle* h, itg_dtype dtype, int ndim, const int64_t* dims,
+                       const int32_t* perm, const void* src, void* dst, const double* alpha,
+                       const double* beta) {
+  if (!h) return ITG_ERR_INVALID;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (ndim < 0 || ndim > kMaxModes) return fail(h, ITG_ERR_UNSUPPORTED, "too many modes");
+  if (ndim > 0 && (!dims || !perm)) return fail(h, ITG_ERR_INVALID, "null dims/perm");
+  int p0[kMaxModes];
+  bool seen[kMaxModes] = {false};
+  int64_t n = 1;
+  for (int i = 0; i < ndim; ++i) {
+    if (dims[i] < 0) return fail(h, ITG_ERR_INVALID, "negative dimension");
+    n *= dims[i];
+    const int q = perm[i] - 1;
+    if (q < 0 || q >= ndim || seen[q]) return fail(h, ITG_ERR_INVALID, "perm is not a permutation");
+    seen[q] = true;
+    p0[i] = q;
+  }
+  if (n > INT32_MAX) return fail(h, ITG_ERR_UNSUPPORTED, "tensor too large");
+  if (n == 0) return ITG_OK;
+  if (!src || !dst) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  DeviceGuard g(h->device);
+  // one-block grouped permutation
+  BlockList bl;
+  bl.N = ndim;
+  for (int i = 0; i < ndim; ++i) bl.dims.push_back((int32_t)dims[i]);
+  bl.offs.push_back(0);
+  bl.nnz = n;
+  PermuteLaunch pl;
+  {
+    hipError_t e = build_permute_launch(&pl, ndim, p0, bl, nullptr, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "permute tables");
+  }
+  const int cplx = dtype == ITG_C128;
+  const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
+  const double b_re = beta ? beta[0] : 0.0, b_im = (beta && cplx) ? beta[1] : 0.0;
+  hipError_t e = launch_permute(h->stream, cplx, pl.nm, pl.perm, (const PermBlock*)pl.blocks.p,
+                                (const int64_t*)pl.prefix.p, pl.nblk, pl.tiles, pl.kind, src, dst,
+                                a_re, a_im, b_re, b_im);
+  if (e != hipSuccess) return hip_fail(h, e, "permute launch");
+  // the tables are freed when `pl` goes out of scope: wait for the kernel
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "permute");
+  return ITG_OK;
+}
+
+itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype, int NR, const int64_t* dimsR,
+                                 const int64_t* labelsR, void* R, int N1, const int64_t* dims1,
+                                 const int64_t* labels1, const void* T1, int N2,
+                                 const int64_t* dims2, const int64_t* labels2, const void* T2,
+                                 const double* alpha, const double* beta, itg_prec prec,
+                                 unsigned flags) {
+  if (!h) return ITG_ERR_INVALID;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
+    return fail(h, ITG_ERR_INVALID, "unknown precision mode");
+  if ((NR > 0 && (!dimsR || !labelsR)) || (N1 > 0 && (!dims1 || !labels1)) ||
+      (N2 > 0 && (!dims2 || !labels2)))
+    return fail(h, ITG_ERR_INVALID, "null dims/labels");
+  Structure st;
+  bool unsup = false;
+  std::string m = analyze_labels(NR, labelsR, N1, labels1, N2, labels2, &st, &unsup);
+  if (!m.empty()) return fail(h, unsup ? ITG_ERR_UNSUPPORTED : ITG_ERR_INVALID, m);
+  // dims consistency
+  int64_t n1 = 1, n2 = 1, nR = 1;
+  for (int i = 0; i < N1; ++i) {
+    if (dims1[i] < 0) return fail(h, ITG_ERR_INVALID, "negative dimension");
+    n1 *= dims1[i];
+  }
+  for (int i = 0; i < N2; ++i) {
+    if (dims2[i] < 0) return fail(h, ITG_ERR_INVALID, "negative dimension");
+    n2 *= dims2[i];
+  }
+  for (int r = 0; r < NR; ++r) {
+    const int64_t want = st.r_src[r] == 1 ? dims1[st.r_pos[r]] : dims2[st.r_pos[r]];
+    if (dimsR[r] != want) return fail(h, ITG_ERR_INVALID, "dimension mismatch between R and inputs");
+    nR *= dimsR[r];
+  }
+  for (int i = 0; i < st.nk; ++i)
+    if (dims1[st.k1[i]] != dims2[st.k2[i]])
+      return fail(h, ITG_ERR_INVALID, "contracted dimensions do not match");
+  if (n1 > INT32_MAX || n2 > INT32_MAX || nR > INT32_MAX)
+    return fail(h, ITG_ERR_UNSUPPORTED, "tensor with more than 2^31-1 elements");
+  if (nR == 0) return ITG_OK;
+  if (!R || (n1 > 0 && !T1) || (n2 > 0 && !T2)) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  const int cplx = dtype == ITG_C128;
+  if (cplx && prec != ITG_PREC_FP64)
+    return fail(h, ITG_ERR_UNSUPPORTED, "split precision is implemented for Float64 only");
+  DeviceGuard g(h->device);
+
+  // plan cache key
+  std::string key;
+  key.reserve(256);
+  auto app = [&](const int64_t* v, int n) {
+    key.append((const char*)&n, sizeof n);
+    key.append((const char*)v, sizeof(int64_t) * (size_t)n);
+  };
+  app(dimsR, NR);
+  app(labelsR, NR);
+  app(dims1, N1);
+  app(labels1, N1);
+  app(dims2, N2);
+  app(labels2, N2);
+  ExecPlan* ep = nullptr;
+  auto it = h->dense_map.find(key);
+  if (it != h->dense_map.end()) {
+    h->dense_lru.splice(h->dense_lru.begin(), h->dense_lru, it->second);
+    ep = it->second->second.get();
+  } else {
+    BlockList b1, b2, bR;
+    b1.N = N1;
+    b2.N = N2;
+    bR.N = NR;
+    for (int i = 0; i < N1; ++i) b1.dims.push_back((int32_t)dims1[i]);
+    for (int i = 0; i < N2; ++i) b2.dims.push_back((int32_t)dims2[i]);
+    for (int i = 0; i < NR; ++i) bR.dims.push_back((int32_t)dimsR[i]);
+    if (N1 == 0) b1.dims.push_back(1);
+    if (N2 == 0) b2.dims.push_back(1);
+    if (NR == 0) bR.dims.push_back(1);
+    b1.offs = {0};
+    b2.offs = {0};
+    bR.offs = {0};
+    b1.nnz = n1;
+    b2.nnz = n2;
+    bR.nnz = nR;
+    std::vector<PairSpec> pairs{{0, 0, 0}};
+    std::unique_ptr<ExecPlan> np(new ExecPlan);
+    if (n1 == 0 || n2 == 0) {
+      // nothing to sum: R = beta*R, done by a K = 0 direct pass
+    }
+    std::string err = build_exec_plan(st, b1, b2, bR, pairs, nullptr, h->stream, np.get());
+    if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
+    h->dense_lru.emplace_front(key, std::move(np));
+    h->dense_map[key] = h->dense_lru.begin();
+    ep = h->dense_lru.front().second.get();
+    while (h->dense_lru.size() > 64) {
+      // evicted plans' tables may still be read by queued kernels
+      (void)hipStreamSynchronize(h->stream);
+      h->dense_map.erase(h->dense_lru.back().first);
+      h->dense_lru.pop_back();
+    }
+  }
+  return run_plan(h, *ep, cplx, (int)prec, T1, T2, R, flags, alpha, beta);
+}
+
+itg_status itg_contract_dense(itg_handle* h, itg_dtype dtype, int NR, const int64_t* dimsR,
+                              const int64_t* labelsR, void* R, int N1, const int64_t* dims1,
+                              const int64_t* labels1, const void* T1, int N2,
+                              const int64_t* dims2, const int64_t* labels2, const void* T2,
+                              const double* alpha, const double* beta, itg_prec prec) {
+  return itg_contract_dense_ex(h, dtype, NR, dimsR, labelsR, R, N1, dims1, labels1, T1, N2, dims2,
+                               labels2, T2, alpha, beta, prec, 0u);
+}
+
+// ----------------------------------------------------------- block sparse
+itg_status itg_bs_plan_create(itg_handle* h, int N1, int64_t nblk1, const int64_t* blocks1,
+                              const int64_t* offs1, const int64_t* nsec1,
+                              const int64_t* secdims1, const int64_t* labels1, int N2,
+                              int64_t nblk2, const int64_t* blocks2, const int64_t* offs2,
+                              const int64_t* nsec2, const int64_t* secdims2,
+                              const int64_t* labels2, int NR, const int64_t* labelsR,
+                              itg_bs_plan** out) {
+  if (!h || !out) return ITG_ERR_INVALID;
+  *out = nullptr;
+  if (nblk1 < 0 || nblk2 < 0) return fail(h, ITG_ERR_INVALID, "negative block count");
+  if ((nblk1 > 0 && (!blocks1 || !offs1)) || (nblk2 > 0 && (!blocks2 || !offs2)))
+    return fail(h, ITG_ERR_INVALID, "null block arrays");
+  if ((N1 > 0 && (!nsec1 || !secdims1 || !labels1)) || (N2 > 0 && (!nsec2 || !secdims2 || !labels2)) ||
+      (NR > 0 && !labelsR))
+    return fail(h, ITG_ERR_INVALID, "null sector/label arrays");
+  Structure st;
+  bool unsup = false;
+  std::string m = analyze_labels(NR, labelsR, N1, labels1, N2, labels2, &st, &unsup);
+  if (!m.empty()) return fail(h, unsup ? ITG_ERR_UNSUPPORTED : ITG_ERR_INVALID, m);
+  // sector tables
+  std::vector<std::vector<int64_t>> sec1(N1), sec2(N2);
+  {
+    int64_t o = 0;
+    for (int i = 0; i < N1; ++i) {
+      if (nsec1[i] < 0) return fail(h, ITG_ERR_INVALID, "negative sector count");
+      sec1[i].assign(secdims1 + o, secdims1 + o + nsec1[i]);
+      o += nsec1[i];
+    }
+    o = 0;
+    for (int i = 0; i < N2; ++i) {
+      if (nsec2[i] < 0) return fail(h, ITG_ERR_INVALID, "negative sector count");
+      sec2[i].assign(secdims2 + o, secdims2 + o + nsec2[i]);
+      o += nsec2[i];
+    }
+  }
+  for (int i = 0; i < st.nk; ++i)
+    if (sec1[st.k1[i]] != sec2[st.k2[i]])
+      return fail(h, ITG_ERR_INVALID, "contracted indices have different block structure");
+  std::unique_ptr<itg_bs_plan> p(new itg_bs_plan);
+  p->device = h->device;
+  p->NR = NR;
+  p->st = st;
+  auto fill = [&](BlockList& bl, int N, int64_t nblk, const int64_t* blocks, const int64_t* offs,
+                  const std::vector<std::vector<int64_t>>& sec) -> std::string {
+    bl.N = N;
+    bl.dims.resize((size_t)std::max<int64_t>(nblk, 0) * std::max(N, 1), 1);
+    bl.offs.assign(offs, offs + nblk);
+    int64_t end = 0;
+    for (int64_t b = 0; b < nblk; ++b) {
+      int64_t sz = 1;
+      for (int i = 0; i < N; ++i) {
+        const int64_t c = blocks[b * N + i];
+        if (c < 1 || c > (int64_t)sec[i].size()) return "block coordinate out of range";
+        bl.dims[b * std::max(N, 1) + i] = (int32_t)sec[i][c - 1];
+        sz *= sec[i][c - 1];
+      }
+      if (sz > INT32_MAX) return "block with more than 2^31-1 elements";
+      end = std::max(end, offs[b] + sz);
+    }
+    bl.nnz = end;
+    return "";
+  };
+  std::string err = fill(p->b1, N1, nblk1, blocks1, offs1, sec1);
+  if (err.empty()) err = fill(p->b2, N2, nblk2, blocks2, offs2, sec2);
+  if (!err.empty()) return fail(h, ITG_ERR_INVALID, err);
+
+  // sequential pair enumeration (contract_sequential.jl:26-39), with T2
+  // blocks bucketed by their contracted coordinates so the scan is
+  // O(nblk1 + nblk2 + npairs) while keeping the (block1 outer, block2 inner)
+  // order.
+  std::unordered_map<std::string, std::vector<int64_t>> by_key;
+  auto kkey = [&](const int64_t* blk, const int* pos) {
+    std::string k;
+    for (int i = 0; i < st.nk; ++i) k.append((const char*)&blk[pos[i]], sizeof(int64_t));
+    return k;
+  };
+  for (int64_t b = 0; b < nblk2; ++b) by_key[kkey(blocks2 + b * N2, st.k2)].push_back(b);
+  std::map<std::vector<int64_t>, int64_t> rindex;  // block coords -> R block index
+  BlockList& bR = p->bR;
+  bR.N = NR;
+  int64_t nnzR = 0;
+  for (int64_t b1 = 0; b1 < nblk1; ++b1) {
+    auto f = by_key.find(kkey(blocks1 + b1 * N1, st.k1));
+    if (f == by_key.end()) continue;
+    for (int64_t b2 : f->second) {
+      std::vector<int64_t> cR(NR);
+      for (int r = 0; r < NR; ++r)
+        cR[r] = st.r_src[r] == 1 ? blocks1[b1 * N1 + st.r_pos[r]] : blocks2[b2 * N2 + st.r_pos[r]];
+      auto ri = rindex.find(cR);
+      int64_t idx;
+      if (ri == rindex.end()) {
+        idx = (int64_t)bR.offs.size();
+        rindex.emplace(cR, idx);
+        int64_t sz = 1;
+        for (int r = 0; r < NR; ++r) {
+          const int64_t d = st.r_src[r] == 1 ? sec1[st.r_pos[r]][cR[r] - 1] : sec2[st.r_pos[r]][cR[r] - 1];
+          bR.dims.push_back((int32_t)d);
+          sz *= d;
+        }
+        if (NR == 0) bR.dims.push_back(1);
+        bR.offs.push_back(nnzR);
+        nnzR += sz;
+        for (int r = 0; r < NR; ++r) p->blocksR.push_back(cR[r]);
+      } else {
+        idx = ri->second;
+      }
+      p->pairs.push_back(PairSpec{b1, b2, idx});
+    }
+  }
+  bR.nnz = nnzR;
+  // group pairs by output block (stable: plan order within a block), as
+  // contract_generic.jl:57-60 does
+  std::stable_sort(p->pairs.begin(), p->pairs.end(),
+                   [](const PairSpec& a, const PairSpec& b) { return a.blkR < b.blkR; });
+  if (N1 == 0) p->b1.dims.assign((size_t)nblk1, 1);
+  if (N2 == 0) p->b2.dims.assign((size_t)nblk2, 1);
+  if (!p->pairs.empty()) {
+    DeviceGuard g(h->device);
+    p->ep.reset(new ExecPlan);
+    err = build_exec_plan(st, p->b1, p->b2, p->bR, p->pairs, nullptr, h->stream, p->ep.get());
+    if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
+  }
+  *out = p.release();
+  return ITG_OK;
+}
+
+itg_status itg_bs_plan_sizes(const itg_bs_plan* p, int64_t* nblkR, int64_t* nnzR,
+                             int64_t* npairs) {
+  if (!p) return ITG_ERR_INVALID;
+  if (nblkR) *nblkR = p->bR.nblk();
+  if (nnzR) *nnzR = p->bR.nnz;
+  if (npairs) *npairs = (int64_t)p->pairs.size();
+  return ITG_OK;
+}
+
+itg_status itg_bs_plan_blocks(const itg_bs_plan* p, int64_t* blocksR, int64_t* offsR) {
+  if (!p) return ITG_ERR_INVALID;
+  if (blocksR && !p->blocksR.empty())
+    std::memcpy(blocksR, p->blocksR.data(), p->blocksR.size() * sizeof(int64_t));
+  if (offsR && !p->bR.offs.empty())
+    std::memcpy(offsR, p->bR.offs.data(), p->bR.offs.size() * sizeof(int64_t));
+  return ITG_OK;
+}
+
+itg_status itg_bs_plan_pairs(const itg_bs_plan* p, int64_t* pairs) {
+  if (!p) return ITG_ERR_INVALID;
+  // report in enumeration order (block1 outer, block2 inner)
+  std::vector<PairSpec> v(p->pairs);
+  std::stable_sort(v.begin(), v.end(), [](const PairSpec& a, const PairSpec& b) {
+    return a.blk1 != b.blk1 ? a.blk1 < b.blk1 : a.blk2 < b.blk2;
+  });
+  for (size_t i = 0; i < v.size(); ++i) {
+    pairs[3 * i] = v[i].blk1;
+    pairs[3 * i + 1] = v[i].blk2;
+    pairs[3 * i + 2] = v[i].blkR;
+  }
+  return ITG_OK;
+}
+
+itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* R,
+                          const void* T1, const void* T2, const double* alpha,
+                          const double* pair_alpha, itg_prec prec, unsigned flags) {
+  if (!h || !p) return ITG_ERR_INVALID;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
+    return fail(h, ITG_ERR_INVALID, "unknown precision mode");
+  if (p->pairs.empty()) return ITG_OK;
+  if (!R || !T1 || !T2) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
+  const int cplx = dtype == ITG_C128;
+  if (cplx && prec != ITG_PREC_FP64)
+    return fail(h, ITG_ERR_UNSUPPORTED, "split precision is implemented for Float64 only");
+  DeviceGuard g(h->device);
+  ExecPlan* ep = p->ep.get();
+  if (pair_alpha) {
+    // per-pair factors are baked into the device tables: rebuild when they change
+    // pair_alpha is given in enumeration order; the plan stores pairs grouped
+    std::vector<PairSpec> enum_order(p->pairs);
+    std::vector<size_t> idx(p->pairs.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      const PairSpec &x = p->pairs[a], &y = p->pairs[b];
+      return x.blk1 != y.blk1 ? x.blk1 < y.blk1 : x.blk2 < y.blk2;
+    });
+    std::vector<double> pa(2 * p->pairs.size());
+    for (size_t e = 0; e < idx.size(); ++e) {
+      pa[2 * idx[e]] = pair_alpha[2 * e];
+      pa[2 * idx[e] + 1] = cplx ? pair_alpha[2 * e + 1] : 0.0;
+    }
+    if (!p->ep_alpha || p->ep_alpha_key != pa) {
+      (void)hipStreamSynchronize(h->stream);
+      p->ep_alpha.reset(new ExecPlan);
+      std::string err =
+          build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, pa.data(), h->stream, p->ep_alpha.get());
+      if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
+      p->ep_alpha_key = pa;
+    }
+    ep = p->ep_alpha.get();
+  }
+  return run_plan(h, *ep, cplx, (int)prec, T1, T2, R, flags, alpha, nullptr);
+}
+
+itg_status itg_bs_plan_destroy(itg_bs_plan* p) {
+  if (!p) return ITG_OK;
+  DeviceGuard g(p->device);
+  (void)hipDeviceSynchronize();
+  delete p;
+  return ITG_OK;
+}
+
+itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
+                          const int64_t* blockdims, const int64_t* offs_src,
+                          const int64_t* offs_dst, const int32_t* perm, const void* src,
+                          void* dst) {
+  if (!h) return ITG_ERR_INVALID;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (N < 0 || N > kMaxModes) return fail(h, ITG_ERR_UNSUPPORTED, "too many modes");
+  if (nblk == 0) return ITG_OK;
+  if (!blockdims || !offs_src || !offs_dst || (N > 0 && !perm) || !src || !dst)
+    return fail(h, ITG_ERR_INVALID, "null argument");
+  int p0[kMaxModes];
+  bool seen[kMaxModes] = {false};
+  for (int i = 0; i < N; ++i) {
+    const int q = perm[i] - 1;
+    if (q < 0 || q >= N || seen[q]) return fail(h, ITG_ERR_INVALID, "perm is not a permutation");
+    seen[q] = true;
+    p0[i] = q;
+  }
+  BlockList bl;
+  bl.N = N;
+  for (int64_t b = 0; b < nblk; ++b) {
+    int64_t sz = 1;
+    for (int i = 0; i < N; ++i) {
+      if (blockdims[b * N + i] < 0) return fail(h, ITG_ERR_INVALID, "negative dimension");
+      bl.dims.push_back((int32_t)blockdims[b * N + i]);
+      sz *= blockdims[b * N + i];
+    }
+    if (sz > INT32_MAX) return fail(h, ITG_ERR_UNSUPPORTED, "block too large");
+    if (N == 0) bl.dims.push_back(1);
+  }
+  bl.offs.assign(offs_src, offs_src + nblk);
+  std::vector<int64_t> od(offs_dst, offs_dst + nblk);
+  DeviceGuard g(h->device);
+  PermuteLaunch pl;
+  hipError_t e = build_permute_launch(&pl, N, p0, bl, &od, h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "permute tables");
+  const int cplx = dtype == ITG_C128;
+  e = launch_permute(h->stream, cplx, pl.nm, pl.perm, (const PermBlock*)pl.blocks.p,
+                     (const int64_t*)pl.prefix.p, pl.nblk, pl.tiles, pl.kind, src, dst, 1.0, 0.0,
+                     0.0, 0.0);
+  if (e != hipSuccess) return hip_fail(h, e, "permute launch");
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "permute");
+  return ITG_OK;
+}
+
+itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* R,
+                           const void* T1, const void* T2, itg_prec prec) {
+  if (!h || !p) return ITG_ERR_INVALID;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (p->pairs.empty() || !p->ep || p->ep->strategy != ExecPlan::GEMM) return ITG_OK;
+  const int cplx = dtype == ITG_C128;
+  DeviceGuard g(h->device);
+  ExecPlan& ep = *p->ep;
+  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx));
+  if (e != hipSuccess) return hip_fail(h, e, "workspace allocation");
+  hipEvent_t t0, t1;
+  if ((e = hipEventCreate(&t0)) != hipSuccess) return hip_fail(h, e, "event");
+  if ((e = hipEventCreate(&t1)) != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    return hip_fail(h, e, "event");
+  }
+  int best = -1;
+  float best_ms = 1e30f;
+  std::string err;
+  for (int cfg = 0; cfg < gemm_num_cfgs(); ++cfg) {
+    if (cplx && cfg != 0 && cfg != 2) continue;
+    // warm-up, then 3 timed runs
+    e = exec_run(ep, cplx, (int)prec, cfg, h->stream, h->ws, T1, T2, R, 0u, 1.0, 0.0, 0.0, 0.0, &err);
+    if (e != hipSuccess) break;
+    (void)hipEventRecord(t0, h->stream);
+    for (int r = 0; r < 3; ++r)
+      (void)exec_run(ep, cplx, (int)prec, cfg, h->stream, h->ws, T1, T2, R, 0u, 1.0, 0.0, 0.0, 0.0, &err);
+    (void)hipEventRecord(t1, h->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (e == hipErrorNotSupported) return fail(h, ITG_ERR_UNSUPPORTED, err);
+  if (e != hipSuccess) return fail(h, ITG_ERR_HIP, err);
+  if (best >= 0) h->tuned[tune_key(ep, cplx, (int)prec)] = best;
+  return ITG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Internal declarations shared by the host planner and the HIP kernels of
+// libitg.so. Nothing here is part of the C ABI (include/itg.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace itg {
+
+constexpr int kMaxModes = 8;  // modes per tensor after validation
+constexpr int kMaxKModes = 8;
+
+// ------------------------------------------------------------ descriptors
+// Column-major tensors; offsets and strides are in ELEMENTS of the dtype.
+
+// One block of a grouped permutation: dst = alpha*perm(src) + beta*dst.
+// dims are the src block's dims after mode merging (nm modes, uniform per
+// launch); the permutation itself is a launch argument.
+struct PermBlock {
+  int64_t off_src;
+  int64_t off_dst;
+  int32_t dims[kMaxModes];
+};
+
+// Small-K direct contraction: every output element of a group (output block)
+// sums, over the group's pairs in plan order, alpha_p * sum_k A[..]*B[..].
+struct DirectPair {
+  int64_t off_a;
+  int64_t off_b;
+  double alpha_re, alpha_im;
+  int32_t cstr_a[kMaxModes];  // stride in A of each R mode (0 if from B)
+  int32_t cstr_b[kMaxModes];  // stride in B of each R mode (0 if from A)
+  int32_t kofs;               // start of this pair's (ka, kb) offset list
+  int32_t nk;                 // number of contracted elements
+};
+struct DirectGroup {
+  int64_t off_c;
+  int32_t cdim[kMaxModes];
+  int32_t nelem;
+  int32_t p0, p1;  // pair range
+  int32_t pad;
+};
+
+// Grouped GEMM: C (M x N, column-major, ldc = M) of a group is
+// sum over its pairs p of alpha_p * op(A_p) * op(B_p).
+struct GemmPair {
+  int64_t off_a;
+  int64_t off_b;
+  int32_t lda, ldb;
+  int32_t K;
+  int32_t pad;
+  double alpha_re, alpha_im;
+};
+struct GemmGroup {
+  int64_t off_c;
+  int32_t M, N;
+  int32_t ldc;
+  int32_t p0, p1;
+  int32_t pad;
+};
+
+// --------------------------------------------------------- kernel launchers
+// All launchers enqueue on `stream` and return hipSuccess or the launch error.
+// `esize` is 1 for Float64 and 2 for ComplexF64.
+
+// Grouped permutation. perm[i] = src mode feeding dst mode i (0-based).
+// tile_prefix has nblk+1 entries (workgroups before block b).
+hipError_t launch_permute(hipStream_t s, int cplx, int nm, const int32_t* perm_host,
+                          const PermBlock* blocks_dev, const int64_t* tile_prefix_dev,
+                          int64_t nblk, int64_t total_tiles, int mode_kind,
+                          const void* src, void* dst, double a_re, double a_im,
+                          double b_re, double b_im);
+// Number of workgroups a block of the permutation needs (host helper that
+// must agree with the kernel's tiling). mode_kind: 0 = inner mode kept,
+// 1 = 2-D tile transpose of src mode 0 and src mode perm[0].
+int64_t permute_block_tiles(int nm, const int32_t* perm, const int32_t* dims, int mode_kind);
+
+hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pairs,
+                         const DirectGroup* groups, const int32_t* koffs,
+                         const int64_t* tile_prefix, int64_t ngroups, int64_t total_tiles,
+                         const void* A, const void* B, void* C, unsigned flags,
+                         double ga_re, double ga_im, double beta_re, double beta_im);
+constexpr int kDirectElemsPerWG = 256 * 2;
+
+// Grouped GEMM (FP64 MFMA). cfg selects the tile shape; ta/tb: A stored
+// K x M / B stored N x K. tile_prefix over `order` (groups sorted by cost).
+struct GemmTileCfg {
+  int bm, bn;
+};
+int gemm_num_cfgs();
+GemmTileCfg gemm_cfg(int cfg);
+hipError_t launch_gemm_f64(hipStream_t s, int cfg, int cplx, int ta, int tb,
+                           const GemmPair* pairs, const GemmGroup* groups,
+                           const int32_t* order, const int64_t* tile_prefix,
+                           int64_t ngroups, int64_t total_tiles, const void* A,
+                           const void* B, void* C, unsigned flags, double ga_re, double ga_im,
+                           double beta_re, double beta_im);
+
+// Split-precision grouped GEMM (real Float64 only). prec: 1 = BF16 slices,
+// 2 = INT8 residues (CRT). Needs workspace for the per-row/column scales.
+hipError_t launch_gemm_split(hipStream_t s, int prec, int cfg, int ta, int tb,
+                             const GemmPair* pairs, const GemmGroup* groups,
+                             const int32_t* order, const int64_t* tile_prefix,
+                             int64_t ngroups, int64_t total_tiles, const void* A,
+                             const void* B, void* C, double beta_re);
+
+}  // namespace itg

@@ -1,0 +1,642 @@
+// Host-side contraction planner of libitg.so. See itg_plan.h.
+#include "itg_plan.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace itg {
+
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+hipError_t DevBuf::upload(const void* host, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (n > bytes) {
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) return e;
+    bytes = n;
+  }
+  // the host staging vectors die when the planner returns: wait for the copy
+  hipError_t e = hipMemcpyAsync(p, host, n, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+// ------------------------------------------------------------- labels
+// Label rules of NDTensors.contract (contraction_logic.jl:5-62): a label
+// present in both inputs is contracted; every other label must appear once in
+// the output, which holds exactly the uncontracted labels.
+std::string analyze_labels(int NR, const int64_t* lR, int N1, const int64_t* l1, int N2,
+                           const int64_t* l2, Structure* st, bool* unsupported) {
+  *unsupported = false;
+  if (NR < 0 || N1 < 0 || N2 < 0) return "negative number of modes";
+  if (NR > kMaxModes || N1 > kMaxModes || N2 > kMaxModes) {
+    *unsupported = true;
+    return "more than 8 modes per tensor is not supported by this backend";
+  }
+  for (int i = 0; i < N1; ++i)
+    for (int j = i + 1; j < N1; ++j)
+      if (l1[i] == l1[j]) {
+        *unsupported = true;
+        return "repeated label within T1 (trace) is not supported";
+      }
+  for (int i = 0; i < N2; ++i)
+    for (int j = i + 1; j < N2; ++j)
+      if (l2[i] == l2[j]) {
+        *unsupported = true;
+        return "repeated label within T2 (trace) is not supported";
+      }
+  for (int i = 0; i < NR; ++i)
+    for (int j = i + 1; j < NR; ++j)
+      if (lR[i] == lR[j]) return "repeated label in R";
+  Structure s;
+  s.NR = NR;
+  s.N1 = N1;
+  s.N2 = N2;
+  s.nk = 0;
+  int pos1R[kMaxModes], pos2R[kMaxModes];
+  for (int i = 0; i < N1; ++i) {
+    int in2 = -1, inR = -1;
+    for (int j = 0; j < N2; ++j)
+      if (l2[j] == l1[i]) in2 = j;
+    for (int j = 0; j < NR; ++j)
+      if (lR[j] == l1[i]) inR = j;
+    if (in2 >= 0 && inR >= 0) {
+      *unsupported = true;
+      return "a label shared by T1, T2 and R (batch mode) is not supported";
+    }
+    if (in2 >= 0) {
+      s.k1[s.nk] = i;
+      s.k2[s.nk] = in2;
+      s.nk++;
+      pos1R[i] = -1;
+    } else {
+      if (inR < 0) return "a free label of T1 is missing from R";
+      pos1R[i] = inR;
+    }
+  }
+  for (int j = 0; j < N2; ++j) {
+    bool in1 = false;
+    for (int i = 0; i < N1; ++i)
+      if (l1[i] == l2[j]) in1 = true;
+    int inR = -1;
+    for (int r = 0; r < NR; ++r)
+      if (lR[r] == l2[j]) inR = r;
+    if (in1) {
+      pos2R[j] = -1;
+    } else {
+      if (inR < 0) return "a free label of T2 is missing from R";
+      pos2R[j] = inR;
+    }
+  }
+  if (NR != N1 + N2 - 2 * s.nk) return "R labels do not match the uncontracted labels";
+  for (int r = 0; r < NR; ++r) s.r_src[r] = 0;
+  for (int i = 0; i < N1; ++i)
+    if (pos1R[i] >= 0) {
+      s.r_src[pos1R[i]] = 1;
+      s.r_pos[pos1R[i]] = i;
+    }
+  for (int j = 0; j < N2; ++j)
+    if (pos2R[j] >= 0) {
+      s.r_src[pos2R[j]] = 2;
+      s.r_pos[pos2R[j]] = j;
+    }
+  for (int r = 0; r < NR; ++r)
+    if (s.r_src[r] == 0) return "an R label is not a free label of T1 or T2";
+  *st = s;
+  return "";
+}
+
+// --------------------------------------------------------- permutations
+namespace {
+
+// Merge src modes that stay adjacent (and in order) under perm. Returns the
+// merged permutation and, for each merged mode, the range of original modes.
+struct Merged {
+  int nm = 0;
+  int32_t perm[kMaxModes];
+  int first[kMaxModes], last[kMaxModes];  // original src modes of merged src mode
+};
+
+Merged merge_modes(int n, const int* perm) {
+  // group dst positions into runs where perm[j+1] == perm[j] + 1
+  Merged m;
+  int run_of_src[kMaxModes];
+  int runs = 0;
+  int run_first[kMaxModes], run_last[kMaxModes];
+  for (int j = 0; j < n; ++j) {
+    if (j > 0 && perm[j] == perm[j - 1] + 1) {
+      run_last[runs - 1] = perm[j];
+    } else {
+      run_first[runs] = perm[j];
+      run_last[runs] = perm[j];
+      runs++;
+    }
+  }
+  // merged src mode index = rank of run by its first src mode
+  int order[kMaxModes];
+  for (int r = 0; r < runs; ++r) order[r] = r;
+  std::sort(order, order + runs, [&](int a, int b) { return run_first[a] < run_first[b]; });
+  int merged_index_of_run[kMaxModes];
+  for (int i = 0; i < runs; ++i) merged_index_of_run[order[i]] = i;
+  m.nm = runs;
+  for (int r = 0; r < runs; ++r) {
+    m.perm[r] = merged_index_of_run[r];
+    m.first[merged_index_of_run[r]] = run_first[r];
+    m.last[merged_index_of_run[r]] = run_last[r];
+  }
+  (void)run_of_src;
+  return m;
+}
+
+hipError_t build_permute(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
+                         const std::vector<int64_t>* dst_offs, hipStream_t s) {
+  Merged m = merge_modes(n, perm);
+  if (m.nm == 0) {  // 0-dim tensor: one mode of extent 1
+    m.nm = 1;
+    m.perm[0] = 0;
+    m.first[0] = 0;
+    m.last[0] = -1;
+  }
+  pl->nm = m.nm;
+  for (int i = 0; i < kMaxModes; ++i) pl->perm[i] = i < m.nm ? m.perm[i] : 0;
+  pl->kind = (m.nm <= 1 || m.perm[0] == 0) ? 0 : 1;
+  std::vector<PermBlock> blocks(bl.nblk());
+  std::vector<int64_t> prefix(bl.nblk() + 1, 0);
+  for (int64_t b = 0; b < bl.nblk(); ++b) {
+    PermBlock& pb = blocks[b];
+    pb.off_src = bl.offs[b];
+    pb.off_dst = dst_offs ? (*dst_offs)[b] : bl.offs[b];
+    for (int i = 0; i < kMaxModes; ++i) pb.dims[i] = 1;
+    const int32_t* d = bl.d(b);
+    for (int i = 0; i < m.nm; ++i) {
+      int64_t prod = 1;
+      for (int k = m.first[i]; k <= m.last[i]; ++k) prod *= d[k];
+      pb.dims[i] = (int32_t)prod;
+    }
+    prefix[b + 1] = prefix[b] + permute_block_tiles(m.nm, pl->perm, pb.dims, pl->kind);
+  }
+  pl->nblk = bl.nblk();
+  pl->tiles = prefix.back();
+  hipError_t e = pl->blocks.upload(blocks.data(), blocks.size() * sizeof(PermBlock), s);
+  if (e != hipSuccess) return e;
+  return pl->prefix.upload(prefix.data(), prefix.size() * sizeof(int64_t), s);
+}
+
+std::vector<int> seq_concat(const std::vector<int>& a, const std::vector<int>& b) {
+  std::vector<int> r(a);
+  r.insert(r.end(), b.begin(), b.end());
+  return r;
+}
+bool is_identity(const std::vector<int>& v) {
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i] != (int)i) return false;
+  return true;
+}
+
+}  // namespace
+
+hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
+                                const std::vector<int64_t>* dst_offs, hipStream_t s) {
+  return build_permute(pl, n, perm, bl, dst_offs, s);
+}
+
+// ----------------------------------------------------------- exec plan
+std::string build_exec_plan(const Structure& st, const BlockList& b1, const BlockList& b2,
+                            const BlockList& bR, const std::vector<PairSpec>& pairs,
+                            const double* pair_alpha, hipStream_t s, ExecPlan* ep) {
+  ep->st = st;
+  const int NR = st.NR, nk = st.nk;
+  // per-pair shape figures
+  double mnk = 0.0, mn = 0.0, flops = 0.0;
+  int64_t maxdim = 0;
+  for (size_t p = 0; p < pairs.size(); ++p) {
+    const int32_t* d1 = b1.d(pairs[p].blk1);
+    const int32_t* d2 = b2.d(pairs[p].blk2);
+    double K = 1, M = 1, N = 1;
+    for (int i = 0; i < nk; ++i) K *= d1[st.k1[i]];
+    for (int r = 0; r < NR; ++r) {
+      if (st.r_src[r] == 1)
+        M *= d1[st.r_pos[r]];
+      else
+        N *= d2[st.r_pos[r]];
+    }
+    mnk += M * N * K;
+    mn += M * N;
+    flops += 2.0 * M * N * K;
+    maxdim = std::max<int64_t>(maxdim, (int64_t)std::max(M, N));
+  }
+  ep->flops = flops;
+  ep->max_dim = maxdim;
+  const double kavg = mn > 0 ? mnk / mn : 0.0;
+  const bool use_gemm = nk > 0 && kavg >= 24.0;
+
+  // groups: consecutive pairs sharing blkR
+  std::vector<std::pair<size_t, size_t>> granges;
+  for (size_t p = 0; p < pairs.size();) {
+    size_t q = p;
+    while (q < pairs.size() && pairs[q].blkR == pairs[p].blkR) ++q;
+    granges.push_back({p, q});
+    p = q;
+  }
+  auto alpha_of = [&](size_t p, double* re, double* im) {
+    if (pair_alpha) {
+      *re = pair_alpha[2 * p];
+      *im = pair_alpha[2 * p + 1];
+    } else {
+      *re = 1.0;
+      *im = 0.0;
+    }
+  };
+
+  if (!use_gemm) {
+    ep->strategy = ExecPlan::DIRECT;
+    std::vector<DirectPair> dp(pairs.size());
+    std::vector<DirectGroup> dg(granges.size());
+    std::vector<int32_t> koffs;
+    std::vector<int64_t> prefix(granges.size() + 1, 0);
+    for (size_t p = 0; p < pairs.size(); ++p) {
+      const int32_t* d1 = b1.d(pairs[p].blk1);
+      const int32_t* d2 = b2.d(pairs[p].blk2);
+      int64_t s1[kMaxModes], s2[kMaxModes];
+      {
+        int64_t a = 1;
+        for (int i = 0; i < st.N1; ++i) {
+          s1[i] = a;
+          a *= d1[i];
+        }
+        int64_t b = 1;
+        for (int i = 0; i < st.N2; ++i) {
+          s2[i] = b;
+          b *= d2[i];
+        }
+      }
+      DirectPair& x = dp[p];
+      std::memset(&x, 0, sizeof(x));
+      x.off_a = b1.offs[pairs[p].blk1];
+      x.off_b = b2.offs[pairs[p].blk2];
+      alpha_of(p, &x.alpha_re, &x.alpha_im);
+      for (int r = 0; r < NR; ++r) {
+        if (st.r_src[r] == 1)
+          x.cstr_a[r] = (int32_t)s1[st.r_pos[r]];
+        else
+          x.cstr_b[r] = (int32_t)s2[st.r_pos[r]];
+      }
+      int64_t K = 1;
+      for (int i = 0; i < nk; ++i) K *= d1[st.k1[i]];
+      x.kofs = (int32_t)(koffs.size() / 2);
+      x.nk = (int32_t)K;
+      // enumerate contracted multi-index, first contracted mode fastest
+      int64_t c[kMaxModes] = {0};
+      for (int64_t k = 0; k < K; ++k) {
+        int64_t oa = 0, ob = 0;
+        for (int i = 0; i < nk; ++i) {
+          oa += c[i] * s1[st.k1[i]];
+          ob += c[i] * s2[st.k2[i]];
+        }
+        koffs.push_back((int32_t)oa);
+        koffs.push_back((int32_t)ob);
+        for (int i = 0; i < nk; ++i) {
+          if (++c[i] < d1[st.k1[i]]) break;
+          c[i] = 0;
+        }
+      }
+    }
+    for (size_t g = 0; g < granges.size(); ++g) {
+      DirectGroup& y = dg[g];
+      std::memset(&y, 0, sizeof(y));
+      const int64_t br = pairs[granges[g].first].blkR;
+      y.off_c = bR.offs[br];
+      int64_t ne = 1;
+      for (int r = 0; r < kMaxModes; ++r) y.cdim[r] = 1;
+      for (int r = 0; r < NR; ++r) {
+        y.cdim[r] = bR.d(br)[r];
+        ne *= y.cdim[r];
+      }
+      y.nelem = (int32_t)ne;
+      y.p0 = (int32_t)granges[g].first;
+      y.p1 = (int32_t)granges[g].second;
+      prefix[g + 1] = prefix[g] + (ne + kDirectElemsPerWG - 1) / kDirectElemsPerWG;
+    }
+    if (koffs.empty()) koffs.push_back(0), koffs.push_back(0);
+    ep->d_ngroups = (int64_t)granges.size();
+    ep->d_tiles = prefix.back();
+    hipError_t e;
+    if ((e = ep->d_pairs.upload(dp.data(), dp.size() * sizeof(DirectPair), s)) != hipSuccess ||
+        (e = ep->d_groups.upload(dg.data(), dg.size() * sizeof(DirectGroup), s)) != hipSuccess ||
+        (e = ep->d_koffs.upload(koffs.data(), koffs.size() * sizeof(int32_t), s)) != hipSuccess ||
+        (e = ep->d_prefix.upload(prefix.data(), prefix.size() * sizeof(int64_t), s)) != hipSuccess)
+      return std::string("device table upload failed: ") + hipGetErrorString(e);
+    return "";
+  }
+
+  // ------------------------------------------------------------- GEMM
+  ep->strategy = ExecPlan::GEMM;
+  // orientation
+  std::vector<int> m1, n2;  // R-order positions in T1 / T2 of free modes
+  bool t1_first = true, t2_first = true;
+  {
+    bool seen2 = false, seen1 = false;
+    for (int r = 0; r < NR; ++r) {
+      if (st.r_src[r] == 1) {
+        m1.push_back(st.r_pos[r]);
+        if (seen2) t1_first = false;
+        seen1 = true;
+      } else {
+        n2.push_back(st.r_pos[r]);
+        if (seen1) t2_first = false;
+        seen2 = true;
+      }
+    }
+  }
+  bool swap = false, permC = false;
+  if (t1_first)
+    swap = false;
+  else if (t2_first)
+    swap = true;
+  else
+    permC = true;
+  // P = A operand (rows), Q = B operand (cols)
+  const BlockList& P = swap ? b2 : b1;
+  const BlockList& Q = swap ? b1 : b2;
+  const int NP = swap ? st.N2 : st.N1, NQ = swap ? st.N1 : st.N2;
+  std::vector<int> Mp = swap ? n2 : m1;  // rows: positions in P, R order
+  std::vector<int> Nq = swap ? m1 : n2;  // cols: positions in Q, R order
+  std::vector<int> kP(nk), kQ(nk);
+  for (int i = 0; i < nk; ++i) {
+    kP[i] = swap ? st.k2[i] : st.k1[i];
+    kQ[i] = swap ? st.k1[i] : st.k2[i];
+  }
+  // candidate X orders: as in P (ascending P positions) or as in Q
+  struct Choice {
+    std::vector<int> Xp, Xq;
+    bool permA, permB;
+    int ta, tb;
+    int64_t cost;
+  };
+  auto evaluate = [&](bool by_p) {
+    Choice c;
+    std::vector<int> idx(nk);
+    std::iota(idx.begin(), idx.end(), 0);
+    if (by_p)
+      std::sort(idx.begin(), idx.end(), [&](int a, int b) { return kP[a] < kP[b]; });
+    else
+      std::sort(idx.begin(), idx.end(), [&](int a, int b) { return kQ[a] < kQ[b]; });
+    for (int i : idx) {
+      c.Xp.push_back(kP[i]);
+      c.Xq.push_back(kQ[i]);
+    }
+    c.permA = false;
+    c.ta = 0;
+    if (is_identity(seq_concat(Mp, c.Xp)))
+      c.ta = 0;
+    else if (is_identity(seq_concat(c.Xp, Mp)))
+      c.ta = 1;
+    else
+      c.permA = true;
+    c.permB = false;
+    c.tb = 0;
+    if (is_identity(seq_concat(c.Xq, Nq)))
+      c.tb = 0;
+    else if (is_identity(seq_concat(Nq, c.Xq)))
+      c.tb = 1;
+    else
+      c.permB = true;
+    c.cost = (c.permA ? P.nnz : 0) + (c.permB ? Q.nnz : 0);
+    return c;
+  };
+  Choice c1 = evaluate(true), c2 = evaluate(false);
+  Choice& ch = (c2.cost < c1.cost) ? c2 : c1;
+  ep->swap = swap;
+  ep->permC = permC;
+  ep->permA = ch.permA;
+  ep->permB = ch.permB;
+  ep->ta = ch.permA ? 0 : ch.ta;
+  ep->tb = ch.permB ? 0 : ch.tb;
+  ep->nnzA = P.nnz;
+  ep->nnzB = Q.nnz;
+  ep->nnzC = bR.nnz;
+  (void)NP;
+  (void)NQ;
+  hipError_t e;
+  if (ch.permA) {
+    std::vector<int> perm = seq_concat(Mp, ch.Xp);  // new mode i = old mode perm[i]
+    if ((e = build_permute(&ep->pA, (int)perm.size(), perm.data(), P, nullptr, s)) != hipSuccess)
+      return std::string("device table upload failed: ") + hipGetErrorString(e);
+  }
+  if (ch.permB) {
+    std::vector<int> perm = seq_concat(ch.Xq, Nq);
+    if ((e = build_permute(&ep->pB, (int)perm.size(), perm.data(), Q, nullptr, s)) != hipSuccess)
+      return std::string("device table upload failed: ") + hipGetErrorString(e);
+  }
+  if (permC) {
+    // GEMM writes C in (T1 free modes in R order, T2 free modes in R order);
+    // R mode r = temp mode perm[r].
+    std::vector<int> perm(NR);
+    int a = 0, b = (int)m1.size();
+    for (int r = 0; r < NR; ++r) perm[r] = (st.r_src[r] == 1) ? a++ : b++;
+    // dims of the temp blocks
+    BlockList tmp;
+    tmp.N = NR;
+    tmp.offs = bR.offs;
+    tmp.nnz = bR.nnz;
+    tmp.dims.resize(bR.dims.size());
+    for (int64_t blk = 0; blk < bR.nblk(); ++blk) {
+      const int32_t* d = bR.d(blk);
+      int32_t* t = tmp.dims.data() + blk * NR;
+      for (int r = 0; r < NR; ++r) t[perm[r]] = d[r];
+    }
+    if ((e = build_permute(&ep->pC, NR, perm.data(), tmp, nullptr, s)) != hipSuccess)
+      return std::string("device table upload failed: ") + hipGetErrorString(e);
+  }
+  // GEMM pairs and groups
+  ep->h_pairs.resize(pairs.size());
+  ep->h_groups.resize(granges.size());
+  ep->group_cost.resize(granges.size());
+  for (size_t p = 0; p < pairs.size(); ++p) {
+    const int64_t bp = swap ? pairs[p].blk2 : pairs[p].blk1;
+    const int64_t bq = swap ? pairs[p].blk1 : pairs[p].blk2;
+    const int32_t* dP = P.d(bp);
+    const int32_t* dQ = Q.d(bq);
+    int64_t M = 1, N = 1, K = 1;
+    for (int x : Mp) M *= dP[x];
+    for (int x : Nq) N *= dQ[x];
+    for (int x : ch.Xp) K *= dP[x];
+    GemmPair& g = ep->h_pairs[p];
+    std::memset(&g, 0, sizeof(g));
+    g.off_a = P.offs[bp];
+    g.off_b = Q.offs[bq];
+    g.K = (int32_t)K;
+    g.lda = (int32_t)((ep->ta && !ch.permA) ? std::max<int64_t>(K, 1) : std::max<int64_t>(M, 1));
+    g.ldb = (int32_t)((ep->tb && !ch.permB) ? std::max<int64_t>(N, 1) : std::max<int64_t>(K, 1));
+    alpha_of(p, &g.alpha_re, &g.alpha_im);
+  }
+  for (size_t gi = 0; gi < granges.size(); ++gi) {
+    const size_t p = granges[gi].first;
+    const int64_t br = pairs[p].blkR;
+    const int64_t bp = swap ? pairs[p].blk2 : pairs[p].blk1;
+    const int64_t bq = swap ? pairs[p].blk1 : pairs[p].blk2;
+    int64_t M = 1, N = 1;
+    for (int x : Mp) M *= P.d(bp)[x];
+    for (int x : Nq) N *= Q.d(bq)[x];
+    GemmGroup& g = ep->h_groups[gi];
+    std::memset(&g, 0, sizeof(g));
+    g.off_c = bR.offs[br];
+    g.M = (int32_t)M;
+    g.N = (int32_t)N;
+    g.ldc = (int32_t)std::max<int64_t>(M, 1);
+    g.p0 = (int32_t)granges[gi].first;
+    g.p1 = (int32_t)granges[gi].second;
+    double cost = 0;
+    for (size_t q = granges[gi].first; q < granges[gi].second; ++q)
+      cost += 2.0 * (double)M * (double)N * (double)ep->h_pairs[q].K;
+    ep->group_cost[gi] = cost;
+  }
+  if ((e = ep->g_pairs.upload(ep->h_pairs.data(), ep->h_pairs.size() * sizeof(GemmPair), s)) !=
+          hipSuccess ||
+      (e = ep->g_groups.upload(ep->h_groups.data(), ep->h_groups.size() * sizeof(GemmGroup),
+                               s)) != hipSuccess)
+    return std::string("device table upload failed: ") + hipGetErrorString(e);
+  return "";
+}
+
+int default_gemm_cfg(const ExecPlan& ep, int cplx) {
+  // 128x128 tiles once the typical block is large enough to fill them;
+  // complex keeps to the tiles that fit its doubled accumulators.
+  if (cplx) return ep.max_dim >= 128 ? 2 : 0;
+  double big = 0, tot = 0;
+  for (size_t g = 0; g < ep.h_groups.size(); ++g) {
+    tot += ep.group_cost[g];
+    if (ep.h_groups[g].M >= 128 && ep.h_groups[g].N >= 128) big += ep.group_cost[g];
+  }
+  return (tot > 0 && big >= 0.5 * tot) ? 1 : 0;
+}
+
+hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
+  if (cplx && cfg != 0 && cfg != 2) cfg = 0;
+  auto it = ep.launches.find(cfg);
+  if (it != ep.launches.end()) return hipSuccess;
+  std::unique_ptr<GemmLaunch> gl(new GemmLaunch);
+  gl->cfg = cfg;
+  const GemmTileCfg tc = gemm_cfg(cfg);
+  const size_t ng = ep.h_groups.size();
+  std::vector<int32_t> order(ng);
+  std::iota(order.begin(), order.end(), 0);
+  // largest per-tile work first (per-tile cost ~ sum of K over the pairs)
+  std::vector<double> tile_work(ng);
+  for (size_t g = 0; g < ng; ++g) {
+    double k = 0;
+    for (int p = ep.h_groups[g].p0; p < ep.h_groups[g].p1; ++p) k += ep.h_pairs[p].K;
+    tile_work[g] = k;
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return tile_work[a] > tile_work[b]; });
+  std::vector<int64_t> prefix(ng + 1, 0);
+  for (size_t i = 0; i < ng; ++i) {
+    const GemmGroup& g = ep.h_groups[order[i]];
+    const int64_t t = (int64_t)((g.M + tc.bm - 1) / tc.bm) * ((g.N + tc.bn - 1) / tc.bn);
+    prefix[i + 1] = prefix[i] + t;
+  }
+  gl->ngroups = (int64_t)ng;
+  gl->tiles = prefix.back();
+  hipError_t e;
+  if ((e = gl->order.upload(order.data(), order.size() * sizeof(int32_t), s)) != hipSuccess)
+    return e;
+  if ((e = gl->prefix.upload(prefix.data(), prefix.size() * sizeof(int64_t), s)) != hipSuccess)
+    return e;
+  ep.launches[cfg] = std::move(gl);
+  return hipSuccess;
+}
+
+size_t exec_workspace_bytes(const ExecPlan& ep, int cplx) {
+  if (ep.strategy != ExecPlan::GEMM) return 0;
+  const size_t es = cplx ? 16 : 8;
+  size_t b = 0;
+  if (ep.permA) b += (size_t)ep.nnzA * es;
+  if (ep.permB) b += (size_t)ep.nnzB * es;
+  if (ep.permC) b += (size_t)ep.nnzC * es;
+  return b + 1024;
+}
+
+hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
+                    const void* T1, const void* T2, void* R, unsigned flags, double a_re,
+                    double a_im, double b_re, double b_im, std::string* err) {
+  hipError_t e;
+  if (ep.strategy == ExecPlan::DIRECT) {
+    e = launch_direct(s, cplx, ep.st.NR, (const DirectPair*)ep.d_pairs.p,
+                      (const DirectGroup*)ep.d_groups.p, (const int32_t*)ep.d_koffs.p,
+                      (const int64_t*)ep.d_prefix.p, ep.d_ngroups, ep.d_tiles, T1, T2, R, flags,
+                      a_re, a_im, b_re, b_im);
+    if (e != hipSuccess) *err = std::string("direct kernel launch: ") + hipGetErrorString(e);
+    return e;
+  }
+  const size_t es = cplx ? 16 : 8;
+  char* w = (char*)ws;
+  const void* A = ep.swap ? T2 : T1;
+  const void* B = ep.swap ? T1 : T2;
+  unsigned gflags = ep.swap ? (((flags & 1u) << 1) | ((flags & 2u) >> 1)) : flags;
+  if (ep.permA) {
+    e = launch_permute(s, cplx, ep.pA.nm, ep.pA.perm, (const PermBlock*)ep.pA.blocks.p,
+                       (const int64_t*)ep.pA.prefix.p, ep.pA.nblk, ep.pA.tiles, ep.pA.kind, A, w,
+                       1.0, 0.0, 0.0, 0.0);
+    if (e != hipSuccess) {
+      *err = std::string("permute launch: ") + hipGetErrorString(e);
+      return e;
+    }
+    A = w;
+    w += (size_t)ep.nnzA * es;
+  }
+  if (ep.permB) {
+    e = launch_permute(s, cplx, ep.pB.nm, ep.pB.perm, (const PermBlock*)ep.pB.blocks.p,
+                       (const int64_t*)ep.pB.prefix.p, ep.pB.nblk, ep.pB.tiles, ep.pB.kind, B, w,
+                       1.0, 0.0, 0.0, 0.0);
+    if (e != hipSuccess) {
+      *err = std::string("permute launch: ") + hipGetErrorString(e);
+      return e;
+    }
+    B = w;
+    w += (size_t)ep.nnzB * es;
+  }
+  void* C = R;
+  double gbr = b_re, gbi = b_im;
+  if (ep.permC) {
+    C = w;
+    gbr = 0.0;
+    gbi = 0.0;
+  }
+  if ((e = ensure_gemm_launch(ep, cfg, cplx, s)) != hipSuccess) {
+    *err = std::string("gemm table upload: ") + hipGetErrorString(e);
+    return e;
+  }
+  int ecfg = (cplx && cfg != 0 && cfg != 2) ? 0 : cfg;
+  GemmLaunch& gl = *ep.launches[ecfg];
+  if (prec == 0 || cplx) {
+    e = launch_gemm_f64(s, ecfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
+                        (const GemmGroup*)ep.g_groups.p, (const int32_t*)gl.order.p,
+                        (const int64_t*)gl.prefix.p, gl.ngroups, gl.tiles, A, B, C, gflags, a_re,
+                        a_im, gbr, gbi);
+  } else {
+    e = hipErrorNotSupported;
+  }
+  if (e != hipSuccess) {
+    *err = std::string("gemm launch: ") + hipGetErrorString(e);
+    return e;
+  }
+  if (ep.permC) {
+    e = launch_permute(s, cplx, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,
+                       (const int64_t*)ep.pC.prefix.p, ep.pC.nblk, ep.pC.tiles, ep.pC.kind, C, R,
+                       1.0, 0.0, b_re, b_im);
+    if (e != hipSuccess) *err = std::string("permute launch: ") + hipGetErrorString(e);
+  }
+  return e;
+}
+
+}  // namespace itg

@@ -1,0 +1,124 @@
+// Host-side contraction planner of libitg.so (no device code).
+//
+// A contraction (dense, or one QN block-sparse step) is lowered to a list of
+// "pairs" (block of T1, block of T2 -> block of R) grouped by output block.
+// The planner picks one of two strategies for the whole list:
+//   DIRECT : small contracted extent; every output element is summed in place
+//            from strided reads (no permutations).
+//   GEMM   : transpose-transpose-GEMM-transpose. Operands are used in place
+//            when their modes are already grouped as (rows, contracted) or
+//            (contracted, rows) -- the GEMM kernel takes either orientation --
+//            and otherwise permuted once (grouped over blocks) into a
+//            workspace. If R's modes interleave the two operands' free modes,
+//            C is computed in (T1-free, T2-free) order and permuted into R.
+// The choice of contracted-mode order and of C = T1*T2 versus C^T = T2^T*T1^T
+// minimises the bytes permuted, in the spirit of NDTensors'
+// ContractionProperties (NDTensors/src/tensoroperations/contraction_logic.jl:
+// 283-651) but decided once per plan instead of once per block pair.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "itg_internal.h"
+
+namespace itg {
+
+struct Structure {
+  int NR = 0, N1 = 0, N2 = 0;
+  int r_src[kMaxModes];  // 1 or 2
+  int r_pos[kMaxModes];  // position of R mode i in its source tensor
+  int nk = 0;
+  int k1[kMaxModes];  // contracted positions in T1 (ascending)
+  int k2[kMaxModes];  // matching positions in T2
+};
+
+// Returns "" on success or an error message. `unsupported` is set when the
+// labels are valid for NDTensors but outside this backend (too many modes,
+// batch / trace labels).
+std::string analyze_labels(int NR, const int64_t* lR, int N1, const int64_t* l1, int N2,
+                           const int64_t* l2, Structure* st, bool* unsupported);
+
+struct PairSpec {
+  int64_t blk1, blk2, blkR;  // indices into the block lists
+};
+
+struct BlockList {
+  int N = 0;
+  std::vector<int32_t> dims;  // N per block
+  std::vector<int64_t> offs;
+  int64_t nnz = 0;
+  int64_t nblk() const { return (int64_t)offs.size(); }
+  const int32_t* d(int64_t b) const { return dims.data() + b * N; }
+};
+
+// Device-resident table (freed with the owner).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  hipError_t upload(const void* host, size_t n, hipStream_t s);
+};
+
+struct PermuteLaunch {
+  int nm = 0;
+  int32_t perm[kMaxModes];
+  int kind = 0;
+  int64_t nblk = 0, tiles = 0;
+  DevBuf blocks, prefix;
+};
+
+struct GemmLaunch {
+  int cfg = -1;
+  int64_t ngroups = 0, tiles = 0;
+  DevBuf order, prefix;
+};
+
+struct ExecPlan {
+  enum Strategy { DIRECT, GEMM } strategy = DIRECT;
+  Structure st;
+  // DIRECT
+  int64_t d_ngroups = 0, d_tiles = 0;
+  DevBuf d_pairs, d_groups, d_koffs, d_prefix;
+  // GEMM
+  bool swap = false, permA = false, permB = false, permC = false;
+  int ta = 0, tb = 0;
+  int64_t nnzA = 0, nnzB = 0, nnzC = 0;  // element counts of the (possibly permuted) operands
+  PermuteLaunch pA, pB, pC;
+  DevBuf g_pairs, g_groups;
+  std::vector<GemmPair> h_pairs;
+  std::vector<GemmGroup> h_groups;
+  std::vector<double> group_cost;  // flops per group (for LPT ordering)
+  std::map<int, std::unique_ptr<GemmLaunch>> launches;  // per tile cfg
+  double flops = 0.0;                                   // real flops (2*M*N*K summed)
+  int64_t max_dim = 0;
+};
+
+// Build the execution plan of `pairs` (grouped: all pairs of one output block
+// must be contiguous, in plan order). alpha per pair is taken from
+// pair_alpha (2 doubles per pair) or 1.
+std::string build_exec_plan(const Structure& st, const BlockList& b1, const BlockList& b2,
+                            const BlockList& bR, const std::vector<PairSpec>& pairs,
+                            const double* pair_alpha, hipStream_t s, ExecPlan* ep);
+
+// Build the device tables of a grouped permutation (dst offsets default to
+// the src offsets). perm: new mode i = old mode perm[i], 0-based.
+hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
+                                const std::vector<int64_t>* dst_offs, hipStream_t s);
+
+// Launch the plan. ws: workspace of at least exec_workspace_bytes().
+size_t exec_workspace_bytes(const ExecPlan& ep, int cplx);
+hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
+                    const void* T1, const void* T2, void* R, unsigned flags, double a_re,
+                    double a_im, double b_re, double b_im, std::string* err);
+// Tile cfg picked by the built-in heuristic (the autotuner may override).
+int default_gemm_cfg(const ExecPlan& ep, int cplx);
+hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s);
+
+}  // namespace itg

@@ -1,0 +1,166 @@
+// Direct (small-K) grouped tensor contraction for gfx950.
+//
+// For contractions whose contracted extent is small (the MPO applications of
+// an effective Hamiltonian contract over (w, s) with K <= 10, or <= 2 per QN
+// block pair), TTGT's permutations would move every byte several times for a
+// handful of flops. This kernel instead reads A and B in place through their
+// strides and writes each output element once:
+//
+//   C[c] = beta*C[c] (dense entry only) + sum_{pairs p of c's block}
+//          alpha_p * sum_k A_p[a(c) + ka_p(k)] * B_p[b(c) + kb_p(k)]
+//
+// One workgroup covers kDirectElemsPerWG consecutive elements of one output
+// block (column-major), so writes are contiguous and reads are contiguous
+// along any output mode that is also a fastest-varying mode of A or B. The
+// pairs of a block are summed in plan order: deterministic, no atomics.
+// Replaces the per-pair permute + BLAS loop of contract_generic.jl:78-129 for
+// this shape class.
+#include "itg_internal.h"
+
+namespace itg {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = kDirectElemsPerWG / kThreads;
+
+__device__ inline int64_t find_group(const int64_t* prefix, int64_t n, int64_t wg) {
+  int64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= wg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int CPLX>
+__global__ void __launch_bounds__(kThreads)
+    direct_kernel(int nr, const DirectPair* __restrict__ pairs,
+                  const DirectGroup* __restrict__ groups, const int32_t* __restrict__ koffs,
+                  const int64_t* __restrict__ prefix, int64_t ngroups,
+                  const double* __restrict__ A, const double* __restrict__ B,
+                  double* __restrict__ C, unsigned flags, double ga_re, double ga_im,
+                  double beta_re, double beta_im) {
+  const int64_t wg = blockIdx.x;
+  const int64_t g = find_group(prefix, ngroups, wg);
+  const DirectGroup grp = groups[g];
+  const int64_t e0 = (wg - prefix[g]) * kDirectElemsPerWG;
+  const bool conja = (flags & 1u) != 0, conjb = (flags & 2u) != 0;
+  const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
+
+  int32_t coord[kPerThread][kMaxModes];
+  bool live[kPerThread];
+  double acc_re[kPerThread], acc_im[kPerThread];
+#pragma unroll
+  for (int r = 0; r < kPerThread; ++r) {
+    const int64_t e = e0 + r * kThreads + threadIdx.x;
+    live[r] = e < grp.nelem;
+    int64_t rem = live[r] ? e : 0;
+    for (int i = 0; i < kMaxModes; ++i) {
+      if (i < nr) {
+        const int64_t d = grp.cdim[i];
+        const int64_t q = rem / d;
+        coord[r][i] = (int32_t)(rem - q * d);
+        rem = q;
+      } else {
+        coord[r][i] = 0;
+      }
+    }
+    acc_re[r] = 0.0;
+    acc_im[r] = 0.0;
+  }
+
+  for (int p = grp.p0; p < grp.p1; ++p) {
+    const DirectPair pr = pairs[p];
+    const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
+    const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
+    int64_t oa[kPerThread], ob[kPerThread];
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+      int64_t a = pr.off_a, b = pr.off_b;
+      for (int i = 0; i < nr; ++i) {
+        a += (int64_t)coord[r][i] * pr.cstr_a[i];
+        b += (int64_t)coord[r][i] * pr.cstr_b[i];
+      }
+      oa[r] = a;
+      ob[r] = b;
+    }
+    double s_re[kPerThread], s_im[kPerThread];
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+      s_re[r] = 0.0;
+      s_im[r] = 0.0;
+    }
+    const int32_t* ko = koffs + 2 * (int64_t)pr.kofs;
+    for (int k = 0; k < pr.nk; ++k) {
+      const int32_t ka = ko[2 * k], kb = ko[2 * k + 1];
+#pragma unroll
+      for (int r = 0; r < kPerThread; ++r) {
+        if (!live[r]) continue;
+        if (CPLX) {
+          const double2 x = reinterpret_cast<const double2*>(A)[oa[r] + ka];
+          const double2 y = reinterpret_cast<const double2*>(B)[ob[r] + kb];
+          const double xi = conja ? -x.y : x.y;
+          const double yi = conjb ? -y.y : y.y;
+          s_re[r] = fma(x.x, y.x, fma(-xi, yi, s_re[r]));
+          s_im[r] = fma(x.x, yi, fma(xi, y.x, s_im[r]));
+        } else {
+          s_re[r] = fma(A[oa[r] + ka], B[ob[r] + kb], s_re[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+      if (CPLX) {
+        acc_re[r] += al_re * s_re[r] - al_im * s_im[r];
+        acc_im[r] += al_re * s_im[r] + al_im * s_re[r];
+      } else {
+        acc_re[r] = fma(al_re, s_re[r], acc_re[r]);
+      }
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < kPerThread; ++r) {
+    if (!live[r]) continue;
+    const int64_t o = grp.off_c + e0 + r * kThreads + threadIdx.x;
+    if (CPLX) {
+      double2 out = make_double2(acc_re[r], acc_im[r]);
+      if (use_beta) {
+        const double2 c = reinterpret_cast<double2*>(C)[o];
+        out.x += beta_re * c.x - beta_im * c.y;
+        out.y += beta_re * c.y + beta_im * c.x;
+      }
+      reinterpret_cast<double2*>(C)[o] = out;
+    } else {
+      double out = acc_re[r];
+      if (use_beta) out = fma(beta_re, C[o], out);
+      C[o] = out;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pairs,
+                         const DirectGroup* groups, const int32_t* koffs,
+                         const int64_t* tile_prefix, int64_t ngroups, int64_t total_tiles,
+                         const void* A, const void* B, void* C, unsigned flags, double ga_re,
+                         double ga_im, double beta_re, double beta_im) {
+  if (total_tiles == 0) return hipSuccess;
+  dim3 grid((unsigned)total_tiles), block(kThreads);
+  if (cplx)
+    hipLaunchKernelGGL(direct_kernel<1>, grid, block, 0, s, nr, pairs, groups, koffs,
+                       tile_prefix, ngroups, (const double*)A, (const double*)B, (double*)C,
+                       flags, ga_re, ga_im, beta_re, beta_im);
+  else
+    hipLaunchKernelGGL(direct_kernel<0>, grid, block, 0, s, nr, pairs, groups, koffs,
+                       tile_prefix, ngroups, (const double*)A, (const double*)B, (double*)C,
+                       flags, ga_re, ga_im, beta_re, beta_im);
+  return hipGetLastError();
+}
+
+}  // namespace itg

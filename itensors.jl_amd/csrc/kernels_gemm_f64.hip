@@ -1,0 +1,322 @@
+// Grouped FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64).
+//
+// One launch covers every output block ("group") of a contraction plan: a
+// dense contraction is one group with one pair; a QN block-sparse step has
+// one group per output block and one pair per allowed (block1, block2).
+// A workgroup owns one BM x BN tile of one group and walks the group's pairs
+// in plan order, accumulating in registers, then writes the tile once:
+//
+//   C_g = beta*C_g (dense entry only) + sum_{p in g} alpha_p * op(A_p) op(B_p)
+//
+// so every output element has exactly one writer and a fixed summation
+// order (deterministic, no atomics). Groups are visited largest-first via the
+// `order` table so the long tiles start early (LPT load balance).
+//
+// Tiles: 256 threads = 4 wave64s in a 2x2 grid; each wave owns a
+// (BM/2) x (BN/2) sub-tile as (BM/32) x (BN/32) MFMA 16x16 blocks of FP64
+// accumulators. A and B tiles (BK = 16) are staged in LDS; the next K-slice
+// is fetched into registers while the current one is multiplied. The MFMA is
+// issued as D = B^T A^T so that each lane's accumulators run down a column
+// of C, which makes the column-major epilogue stores contiguous.
+//
+// Complex data (interleaved re/im) uses the 4-real-product form on the same
+// tiles, with conj(A)/conj(B) folded into the sign of the imaginary parts.
+// Replaces BLAS dgemm/zgemm behind mul!! (NDTensors/src/abstractarray/mul.jl:1-12,
+// NDTensors/src/array/mul.jl:1-4) and the per-pair loop of
+// NDTensors/src/blocksparse/contract_generic.jl:78-129.
+#include "itg_internal.h"
+
+namespace itg {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+constexpr int BK = 16;
+constexpr int PAD = 1;
+
+__device__ inline int64_t find_slot(const int64_t* prefix, int64_t n, int64_t wg) {
+  int64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= wg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int BM, int BN, int CPLX, int TA, int TB>
+__global__ void __launch_bounds__(kThreads)
+    gemm_f64_kernel(const GemmPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
+                    const int32_t* __restrict__ order, const int64_t* __restrict__ prefix,
+                    int64_t ngroups, const double* __restrict__ Ag,
+                    const double* __restrict__ Bg, double* __restrict__ Cg, unsigned flags,
+                    double ga_re, double ga_im, double beta_re, double beta_im) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int NA = BM * BK / kThreads;  // A elements loaded per thread
+  constexpr int NB = BN * BK / kThreads;
+  constexpr int NPLANE = CPLX ? 2 : 1;
+
+  __shared__ double As[NPLANE][BK][BM + PAD];
+  __shared__ double Bs[NPLANE][BK][BN + PAD];
+
+  const int64_t wg = blockIdx.x;
+  const int64_t slot = find_slot(prefix, ngroups, wg);
+  const GemmGroup grp = groups[order[slot]];
+  const int64_t t = wg - prefix[slot];
+  const int tiles_m = (grp.M + BM - 1) / BM;
+  const int tm = (int)(t % tiles_m), tn = (int)(t / tiles_m);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  const double sa = (flags & 1u) ? -1.0 : 1.0;  // conj(A): negate Im(A)
+  const double sb = (flags & 2u) ? -1.0 : 1.0;
+
+  v4d acc[NPLANE][FM][FN];
+#pragma unroll
+  for (int c = 0; c < NPLANE; ++c)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[c][i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  double ra[NPLANE][NA], rb[NPLANE][NB];
+
+  for (int p = grp.p0; p < grp.p1; ++p) {
+    const GemmPair pr = pairs[p];
+    const int K = pr.K;
+    const int M = grp.M, N = grp.N;
+    const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
+    const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
+
+    auto load_regs = [&](int k0) {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const int idx = tid + r * kThreads;
+        int m, k;
+        if (TA) {
+          k = idx % BK;
+          m = idx / BK;
+        } else {
+          m = idx % BM;
+          k = idx / BM;
+        }
+        const int gm = m0 + m, gk = k0 + k;
+        const bool in = gm < M && gk < K;
+        const int64_t off = TA ? ((int64_t)gm * pr.lda + gk) : ((int64_t)gk * pr.lda + gm);
+        if (CPLX) {
+          double2 v = in ? reinterpret_cast<const double2*>(Ag)[pr.off_a + off]
+                         : make_double2(0.0, 0.0);
+          v.y *= sa;
+          ra[0][r] = al_re * v.x - al_im * v.y;
+          ra[NPLANE - 1][r] = al_re * v.y + al_im * v.x;
+        } else {
+          ra[0][r] = in ? al_re * Ag[pr.off_a + off] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int idx = tid + r * kThreads;
+        int n, k;
+        if (TB) {
+          n = idx % BN;
+          k = idx / BN;
+        } else {
+          k = idx % BK;
+          n = idx / BK;
+        }
+        const int gn = n0 + n, gk = k0 + k;
+        const bool in = gn < N && gk < K;
+        const int64_t off = TB ? ((int64_t)gk * pr.ldb + gn) : ((int64_t)gn * pr.ldb + gk);
+        if (CPLX) {
+          double2 v = in ? reinterpret_cast<const double2*>(Bg)[pr.off_b + off]
+                         : make_double2(0.0, 0.0);
+          rb[0][r] = v.x;
+          rb[NPLANE - 1][r] = sb * v.y;
+        } else {
+          rb[0][r] = in ? Bg[pr.off_b + off] : 0.0;
+        }
+      }
+    };
+    auto store_lds = [&]() {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const int idx = tid + r * kThreads;
+        int m, k;
+        if (TA) {
+          k = idx % BK;
+          m = idx / BK;
+        } else {
+          m = idx % BM;
+          k = idx / BM;
+        }
+#pragma unroll
+        for (int c = 0; c < NPLANE; ++c) As[c][k][m] = ra[c][r];
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int idx = tid + r * kThreads;
+        int n, k;
+        if (TB) {
+          n = idx % BN;
+          k = idx / BN;
+        } else {
+          k = idx % BK;
+          n = idx / BK;
+        }
+#pragma unroll
+        for (int c = 0; c < NPLANE; ++c) Bs[c][k][n] = rb[c][r];
+      }
+    };
+
+    if (K <= 0) continue;
+    load_regs(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+      __syncthreads();
+      store_lds();
+      __syncthreads();
+      if (k0 + BK < K) load_regs(k0 + BK);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        double af[NPLANE][FM], bf[NPLANE][FN];
+#pragma unroll
+        for (int c = 0; c < NPLANE; ++c) {
+#pragma unroll
+          for (int i = 0; i < FM; ++i) af[c][i] = As[c][kk + lk][wm0 + i * 16 + l16];
+#pragma unroll
+          for (int j = 0; j < FN; ++j) bf[c][j] = Bs[c][kk + lk][wn0 + j * 16 + l16];
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            if (CPLX) {
+              // re += Bre*Are - Bim*Aim ; im += Bre*Aim + Bim*Are
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[0][j], af[0][i],
+                                                                  acc[0][i][j], 0, 0, 0);
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  -bf[NPLANE - 1][j], af[NPLANE - 1][i], acc[0][i][j], 0, 0, 0);
+              acc[NPLANE - 1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  bf[0][j], af[NPLANE - 1][i], acc[NPLANE - 1][i][j], 0, 0, 0);
+              acc[NPLANE - 1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  bf[NPLANE - 1][j], af[0][i], acc[NPLANE - 1][i][j], 0, 0, 0);
+            } else {
+              acc[0][i][j] =
+                  __builtin_amdgcn_mfma_f64_16x16x4f64(bf[0][j], af[0][i], acc[0][i][j], 0, 0, 0);
+            }
+          }
+      }
+    }
+  }
+
+  // Epilogue. v_mfma_f64_16x16x4_f64 leaves D[i][j] in lane j + 16*(i%4),
+  // register i/4 (rows interleaved across the four lane groups), so with
+  // D = (A B)^T this lane holds C[m = l16][n = lk + 4*r] of each 16x16 block.
+  const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int gm = m0 + wm0 + i * 16 + l16;
+    if (gm >= grp.M) continue;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gn = n0 + wn0 + j * 16 + lk + 4 * r;
+        if (gn >= grp.N) continue;
+        const int64_t o = grp.off_c + gm + (int64_t)gn * grp.ldc;
+        if (CPLX) {
+          double2 out = make_double2(acc[0][i][j][r], acc[NPLANE - 1][i][j][r]);
+          if (use_beta) {
+            const double2 c = reinterpret_cast<double2*>(Cg)[o];
+            out.x += beta_re * c.x - beta_im * c.y;
+            out.y += beta_re * c.y + beta_im * c.x;
+          }
+          reinterpret_cast<double2*>(Cg)[o] = out;
+        } else {
+          double out = acc[0][i][j][r];
+          if (use_beta) out = fma(beta_re, Cg[o], out);
+          Cg[o] = out;
+        }
+      }
+    }
+  }
+}
+
+struct CfgEntry {
+  int bm, bn;
+};
+constexpr CfgEntry kCfgs[] = {{64, 64}, {128, 128}, {128, 64}, {64, 128}};
+
+template <int BM, int BN, int CPLX>
+hipError_t launch_tt(hipStream_t s, int ta, int tb, dim3 grid, const GemmPair* pairs,
+                     const GemmGroup* groups, const int32_t* order, const int64_t* prefix,
+                     int64_t ngroups, const double* A, const double* B, double* C,
+                     unsigned flags, double ar, double ai, double br, double bi) {
+  dim3 block(kThreads);
+  if (!ta && !tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 0, 0>), grid, block, 0, s, pairs, groups,
+                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 0, 1>), grid, block, 0, s, pairs, groups,
+                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 1, 0>), grid, block, 0, s, pairs, groups,
+                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 1, 1>), grid, block, 0, s, pairs, groups,
+                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int gemm_num_cfgs() { return (int)(sizeof(kCfgs) / sizeof(kCfgs[0])); }
+GemmTileCfg gemm_cfg(int cfg) { return GemmTileCfg{kCfgs[cfg].bm, kCfgs[cfg].bn}; }
+
+hipError_t launch_gemm_f64(hipStream_t s, int cfg, int cplx, int ta, int tb,
+                           const GemmPair* pairs, const GemmGroup* groups, const int32_t* order,
+                           const int64_t* tile_prefix, int64_t ngroups, int64_t total_tiles,
+                           const void* A, const void* B, void* C, unsigned flags,
+                           double ga_re, double ga_im, double beta_re, double beta_im) {
+  if (total_tiles == 0) return hipSuccess;
+  dim3 grid((unsigned)total_tiles);
+  const double* a = (const double*)A;
+  const double* b = (const double*)B;
+  double* c = (double*)C;
+  if (cplx) {
+    // complex accumulators double the register tile: keep tiles <= 128x64
+    switch (cfg) {
+      case 2:
+        return launch_tt<128, 64, 1>(s, ta, tb, grid, pairs, groups, order, tile_prefix,
+                                     ngroups, a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+      default:
+        return launch_tt<64, 64, 1>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
+                                    a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+    }
+  }
+  switch (cfg) {
+    case 1:
+      return launch_tt<128, 128, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
+                                    a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+    case 2:
+      return launch_tt<128, 64, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
+                                   a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+    case 3:
+      return launch_tt<64, 128, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
+                                   a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+    default:
+      return launch_tt<64, 64, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
+                                  a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+  }
+}
+
+}  // namespace itg

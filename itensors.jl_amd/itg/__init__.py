@@ -1,0 +1,20 @@
+"""itg -- MI355X (gfx950) backend for the tensor contraction path of
+ITensors.jl / NDTensors.jl.
+
+Layers (host side mirrors the reference's interface, device side is libitg.so):
+  itg.itensors    Index / QN / ITensor, `*` and `contract`
+  itg.ndtensors   NDTensors.contract / contract! / permutedims on device storage
+  itg.chainrules  rrule(contract) with a device adjoint
+  itg.configs     effective-Hamiltonian workloads (dense and U(1) block-sparse)
+  itg._lib        the ctypes image of the C ABI (include/itg.h)
+"""
+from ._lib import LIB_PATH, ItgError, ItgUnsupported, lib  # noqa: F401
+from . import ndtensors  # noqa: F401
+from .itensors import (  # noqa: F401
+    QN, Index, ITensor, In, Neither, Out, compute_contraction_labels, contract,
+    contract_network, dag, eachblock, flux_of_block, itensor_from_array, nzblocks, prime,
+    random_itensor, blocksparse_itensor,
+)
+from .ndtensors import precision, set_precision  # noqa: F401
+
+__version__ = "0.1.0"

@@ -1,0 +1,408 @@
+"""ITensors-level interface of the MI355X backend: Index / QN / ITensor and
+`*` / `contract`, with the reference's semantics.
+
+  QN arithmetic               src/lib/QuantumNumbers/src/qn.jl:6-30,178-207
+  Index identity, dag         src/index.jl:24-33,354-355,362
+  contraction labels          src/indexset.jl:672-707 (arrow check :684-688)
+  flux, nzblocks              src/indexset.jl:876-883, src/qn/qnindexset.jl:9-18
+  ITensor(flux, inds), random_itensor
+                              src/qn/qnitensor.jl:158-166,383-460
+  A * B, contract(A, B)       src/tensor_operations/tensor_algebra.jl:1-77
+  contract(As; sequence)      src/tensor_operations/tensor_algebra.jl:106-159
+  dag(T)                      src/itensor.jl:1806-1846
+
+Storage is an `ndtensors.DenseTensor` (no QNs) or `ndtensors.BlockSparseTensor`
+(QN indices) on the HIP device; every contraction runs in libitg.so.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+import warnings
+from collections import OrderedDict
+
+import numpy as np
+
+from . import ndtensors as ND
+
+Out, In, Neither = 1, -1, 0
+
+
+# -------------------------------------------------------------------- QN
+class QN:
+    """Up to four named values, each with an optional modulus; a missing name
+    counts as zero (qn.jl:6-30)."""
+
+    __slots__ = ("vals",)
+    maxQNs = 4
+
+    def __init__(self, *args):
+        vals = {}
+        if len(args) == 1 and isinstance(args[0], (int, np.integer)):
+            args = (("", int(args[0])),)
+        elif len(args) == 2 and all(isinstance(a, (int, np.integer)) for a in args):
+            args = (("", int(args[0]), int(args[1])),)
+        for a in args:
+            name, v = a[0], int(a[1])
+            m = int(a[2]) if len(a) > 2 else 1
+            vals[name] = (v % m if m > 1 else v, m)
+        if len(vals) > self.maxQNs:
+            raise ValueError("a QN holds at most 4 named values")
+        self.vals = vals
+
+    @classmethod
+    def _of(cls, vals):
+        q = cls.__new__(cls)
+        q.vals = vals
+        return q
+
+    def __add__(self, other):
+        out = dict(self.vals)
+        for n, (v, m) in other.vals.items():
+            if n in out:
+                v0, m0 = out[n]
+                if m0 != m:
+                    raise ValueError("QNVals must have same modulus")
+                s = v0 + v
+                out[n] = (s % m if m > 1 else s, m)
+            else:
+                if len(out) >= self.maxQNs:
+                    raise ValueError("Cannot add QN, maximum number of QNVals reached")
+                out[n] = (v, m)
+        return QN._of(out)
+
+    def __neg__(self):
+        return QN._of({n: ((-v) % m if m > 1 else -v, m) for n, (v, m) in self.vals.items()})
+
+    def __sub__(self, other):
+        return self + (-other)
+
+    def __rmul__(self, d):
+        """Arrow * QN (qn.jl:156-162)."""
+        if d == 1:
+            return self
+        if d == -1:
+            return -self
+        return QN._of({n: (0, m) for n, (v, m) in self.vals.items()})
+
+    def _canon(self):
+        return tuple(sorted((n, v) for n, (v, m) in self.vals.items() if v != 0))
+
+    def __eq__(self, other):
+        return isinstance(other, QN) and self._canon() == other._canon()
+
+    def __hash__(self):
+        return hash(self._canon())
+
+    def __repr__(self):
+        return "QN(" + ", ".join(f"({n!r}, {v})" for n, (v, m) in self.vals.items()) + ")"
+
+
+# ------------------------------------------------------------------- Index
+_next_id = itertools.count(1)
+
+
+class Index:
+    """An index: identity (id, plev, tags), a space (an int, or a list of
+    (QN, dim) sectors) and an arrow. QN indices default to Out
+    (src/qn/qnindex.jl:128)."""
+
+    __slots__ = ("id", "space", "dir", "tags", "plev")
+
+    def __init__(self, space, tags="", plev=0, dir=None, id=None):
+        if isinstance(space, (int, np.integer)):
+            self.space = int(space)
+        else:
+            self.space = tuple((q if isinstance(q, QN) else QN(q), int(d)) for q, d in space)
+        self.tags = tags
+        self.plev = int(plev)
+        self.id = next(_next_id) if id is None else id
+        if dir is None:
+            dir = Out if self.hasqns else Neither
+        self.dir = dir
+
+    @property
+    def hasqns(self):
+        return not isinstance(self.space, int)
+
+    @property
+    def dim(self):
+        return self.space if not self.hasqns else sum(d for _, d in self.space)
+
+    def nblocks(self):
+        return 1 if not self.hasqns else len(self.space)
+
+    def blockdims(self):
+        return (self.space,) if not self.hasqns else tuple(d for _, d in self.space)
+
+    def qn(self, b):
+        """QN of sector b (1-based, as Block numbers are)."""
+        return self.space[b - 1][0]
+
+    def __eq__(self, other):
+        return isinstance(other, Index) and (self.id, self.plev, self.tags) == (other.id, other.plev, other.tags)
+
+    def __hash__(self):
+        return hash((self.id, self.plev, self.tags))
+
+    def __repr__(self):
+        return f"(dim={self.dim}|id={self.id % 1000}|{self.tags!r}){chr(39) * self.plev}"
+
+    def prime(self, n=1):
+        return Index(self.space, self.tags, self.plev + n, self.dir, self.id)
+
+    def sim(self):
+        return Index(self.space, self.tags, self.plev, self.dir)
+
+    def dag(self):
+        return Index(self.space, self.tags, self.plev, -self.dir, self.id)
+
+
+def dag_index(i):
+    """src/index.jl:362: flip the arrow, keep the identity."""
+    return i.dag()
+
+
+def prime(i, n=1):
+    return i.prime(n)
+
+
+def flux_of_block(inds, block):
+    """flux(inds, block) = sum_n dir(i_n)*qn(i_n, block_n) (src/indexset.jl:876-883)."""
+    q = QN()
+    for i, b in zip(inds, block):
+        q = q + i.dir * i.qn(b)
+    return q
+
+
+def eachblock(inds):
+    """Block grid in column-major order (blockdims.jl:104-106), 1-based."""
+    ranges = [range(1, i.nblocks() + 1) for i in inds]
+    for rev in itertools.product(*reversed(ranges)):
+        yield tuple(reversed(rev))
+
+
+def nzblocks(flux, inds):
+    """Blocks whose flux equals `flux` (src/qn/qnindexset.jl:9-18)."""
+    return [b for b in eachblock(inds) if flux_of_block(inds, b) == flux]
+
+
+def compute_contraction_labels(Ais, Bis):
+    """src/indexset.jl:672-707 -- matched pairs get -1, -2, ... in
+    (A position outer, B position inner) order; the rest are numbered from
+    ncont+1 over A then B. Matching QN indices must have opposite arrows."""
+    have_qns = all(i.hasqns for i in Ais) and all(j.hasqns for j in Bis)
+    la = [0] * len(Ais)
+    lb = [0] * len(Bis)
+    ncont = 0
+    for i, a in enumerate(Ais):
+        for j, b in enumerate(Bis):
+            if a == b:
+                if have_qns and a.dir != -b.dir:
+                    raise ValueError(
+                        f"Attempting to contract IndexSet {Ais} with IndexSet {Bis}: QN indices "
+                        f"must have opposite direction to contract, but {a} and {b} do not")
+                ncont += 1
+                la[i] = lb[j] = -ncont
+    u = ncont
+    for i in range(len(la)):
+        if la[i] == 0:
+            u += 1
+            la[i] = u
+    for j in range(len(lb)):
+        if lb[j] == 0:
+            u += 1
+            lb[j] = u
+    return tuple(la), tuple(lb)
+
+
+# ------------------------------------------------------------------ ITensor
+class ITensor:
+    """An ITensor: indices + NDTensors storage on the device."""
+
+    def __init__(self, tensor, inds):
+        self.tensor = tensor
+        self.inds = tuple(inds)
+        if len(self.inds) != tensor.ndims:
+            raise ValueError("number of indices does not match the storage")
+
+    # ---- introspection
+    @property
+    def eltype(self):
+        return self.tensor.eltype
+
+    def hasqns(self):
+        return isinstance(self.tensor, ND.BlockSparseTensor)
+
+    def nnzblocks(self):
+        return self.tensor.nnzblocks() if self.hasqns() else 1
+
+    def nnz(self):
+        return self.tensor.nnz()
+
+    def blocks(self):
+        return list(self.tensor.blockoffsets) if self.hasqns() else [(1,) * len(self.inds)]
+
+    def flux(self):
+        """Flux of the first stored block (src/qn/flux.jl:55-60); None when
+        there are no blocks or no QNs."""
+        if not self.hasqns() or self.nnzblocks() == 0:
+            return None
+        return flux_of_block(self.inds, next(iter(self.tensor.blockoffsets)))
+
+    def checkflux(self):
+        f = self.flux()
+        for b in self.tensor.blockoffsets:
+            if flux_of_block(self.inds, b) != f:
+                raise ValueError("ITensor has blocks of different flux")
+        return True
+
+    def array(self, *order):
+        """Dense host array (numpy), optionally in a given index order."""
+        a = self.tensor.to_numpy_dense() if self.hasqns() else self.tensor.to_numpy()
+        if order:
+            perm = [self.inds.index(i) for i in order]
+            a = np.transpose(a, perm)
+        return a
+
+    def scalar(self):
+        """T[] for a 0-index ITensor."""
+        a = self.array()
+        return a.reshape(-1)[0] if a.size else 0.0
+
+    def norm(self):
+        import torch
+        return float(torch.linalg.vector_norm(self.tensor.data).item())
+
+    def __mul__(self, other):
+        if isinstance(other, ITensor):
+            return contract(self, other)
+        return ITensor(ND._with_data(self.tensor, self.tensor.data * other), self.inds)
+
+    __rmul__ = __mul__
+
+    def __repr__(self):
+        kind = "BlockSparse" if self.hasqns() else "Dense"
+        return f"ITensor({kind}, inds={self.inds}, eltype={self.eltype})"
+
+
+def dag(T):
+    """dag(T): conjugate the data and flip every arrow (src/itensor.jl:1806-1846).
+    Real data is aliased, as the reference does when aliasing is allowed."""
+    data = T.tensor.data
+    if data.is_complex():
+        data = data.conj().resolve_conj()
+    return ITensor(ND._with_data(T.tensor, data), tuple(i.dag() for i in T.inds))
+
+
+# --------------------------------------------------------------- building
+def _torch_dtype(dtype):
+    import torch
+    if dtype in (float, np.float64, "Float64", torch.float64):
+        return torch.float64
+    if dtype in (complex, np.complex128, "ComplexF64", torch.complex128):
+        return torch.complex128
+    raise ValueError(f"unsupported element type {dtype}")
+
+
+def itensor_from_array(a, *inds, device="cuda"):
+    """Dense ITensor holding the (column-major) array `a`."""
+    return ITensor(ND.DenseTensor.from_numpy(a, device), inds)
+
+
+def random_itensor(*args, dtype=np.float64, rng=None, device="cuda"):
+    """random_itensor([flux,] inds...): dense N(0,1) entries, or for QN
+    indices the blocks allowed by `flux` (default QN()) in eachblock order,
+    data drawn in storage order (src/qn/qnitensor.jl:383-460). Complex data
+    draws the real and imaginary parts as independent N(0,1)."""
+    import torch
+    if rng is None:
+        rng = np.random.default_rng()
+    if args and isinstance(args[0], QN):
+        flux, inds = args[0], args[1:]
+    else:
+        flux, inds = None, args
+    if len(inds) == 1 and isinstance(inds[0], (tuple, list)):
+        inds = tuple(inds[0])
+    tdt = _torch_dtype(dtype)
+    cplx = tdt == torch.complex128
+    qns = bool(inds) and all(i.hasqns for i in inds)
+    if not qns:
+        n = math.prod(i.dim for i in inds)
+        x = rng.standard_normal(n)
+        if cplx:
+            x = x + 1j * rng.standard_normal(n)
+        data = torch.from_numpy(np.ascontiguousarray(x)).to(device=device, dtype=tdt)
+        return ITensor(ND.DenseTensor(data, tuple(i.dim for i in inds)), inds)
+    if flux is None:
+        flux = QN()
+    blocks = nzblocks(flux, inds)
+    if not blocks:
+        raise ValueError(f"ITensor with flux={flux} resulted in no allowed blocks")
+    offs = OrderedDict()
+    nnz = 0
+    bdims = tuple(i.blockdims() for i in inds)
+    for b in blocks:
+        offs[b] = nnz
+        nnz += math.prod(bdims[m][c - 1] for m, c in enumerate(b))
+    x = rng.standard_normal(nnz)
+    if cplx:
+        x = x + 1j * rng.standard_normal(nnz)
+    data = torch.from_numpy(np.ascontiguousarray(x)).to(device=device, dtype=tdt)
+    return ITensor(ND.BlockSparseTensor(data, offs, bdims), inds)
+
+
+def blocksparse_itensor(data, blocks, inds, device="cuda"):
+    """ITensor from host data (flat, storage order) and an ordered block list."""
+    import torch
+    offs = OrderedDict()
+    nnz = 0
+    bdims = tuple(i.blockdims() for i in inds)
+    for b in blocks:
+        offs[tuple(b)] = nnz
+        nnz += math.prod(bdims[m][c - 1] for m, c in enumerate(b))
+    t = torch.from_numpy(np.ascontiguousarray(np.asarray(data))).to(device)
+    if t.numel() != nnz:
+        raise ValueError("data length does not match the blocks")
+    return ITensor(ND.BlockSparseTensor(t, offs, bdims), inds)
+
+
+# ------------------------------------------------------------- contraction
+_warn_order = [14]
+
+
+def contract(A, B, *, prec=None, conjA=False, conjB=False):
+    """A * B (src/tensor_operations/tensor_algebra.jl:1-77): contract every
+    index A and B share; the result holds A's free indices (in A's order)
+    then B's. QN indices must meet with opposite arrows."""
+    if len(A.inds) + len(B.inds) >= _warn_order[0] * 2:
+        warnings.warn("contraction of high-order tensors")
+    lA, lB = compute_contraction_labels(A.inds, B.inds)
+    lR = ND.contract_labels(lA, lB)
+    indsR = tuple(A.inds[lA.index(x)] if x in lA else B.inds[lB.index(x)] for x in lR)
+    if A.hasqns() != B.hasqns():
+        raise ND.ItgUnsupported(2, "contracting a QN ITensor with a dense one")
+    if len(lR) == 0 and not A.hasqns():
+        pass
+    T = ND.contract(A.tensor, lA, B.tensor, lB, lR, prec=prec, conj1=conjA, conj2=conjB)
+    return ITensor(T, indsR)
+
+
+def contract_network(As, sequence=None, *, prec=None):
+    """contract(As; sequence) with the default left-associative fold
+    (tensor_algebra.jl:121-134). `sequence` may be a nested list of
+    positions, e.g. [[0, 1], 2]."""
+    As = list(As)
+    if sequence is None or sequence == "left_associative":
+        out = As[0]
+        for B in As[1:]:
+            out = contract(out, B, prec=prec)
+        return out
+
+    def rec(s):
+        if isinstance(s, int):
+            return As[s]
+        a, b = s
+        return contract(rec(a), rec(b), prec=prec)
+
+    return rec(sequence)

@@ -1,0 +1,415 @@
+"""NDTensors-level interface of the MI355X backend.
+
+Mirrors the NDTensors functions the reference's `*` / `contract` path goes
+through, with the same argument meaning (label tuples in the ncon
+convention, column-major storage, alpha/beta with beta == 0 overwriting):
+
+  contract(T1, labels1, T2, labels2[, labelsR])    generic_tensor_operations.jl:87-119
+  contract_(R, labelsR, T1, labels1, T2, labels2, alpha, beta)
+                                   == contract!   dense/tensoralgebra/contract.jl:160-216
+  contract_labels(labels1, labels2)                contraction_logic.jl:5-34
+  contract_blockoffsets(...)                       blocksparse/contract_sequential.jl:1-41
+  permutedims(T, perm) / permutedims_(R, T, perm, alpha, beta)
+                                                   dense/densetensor.jl:212-275
+
+Data lives on the HIP device as flat torch tensors; every contraction runs in
+libitg.so. There is no CPU fallback: without the library or a GPU the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from collections import OrderedDict
+
+import numpy as np
+
+from . import _lib
+from ._lib import ItgError, ItgUnsupported  # noqa: F401  (re-export)
+
+try:  # torch provides device memory and streams
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+# ------------------------------------------------------------------ flags
+# Process-global settings in the style of NDTensors' Refs (using_tblis,
+# gemm_backend: NDTensors/src/NDTensors.jl:243-255,
+# abstractarray/tensoralgebra/contract.jl:15-24).
+_precision = ["fp64"]
+
+
+def set_precision(mode: str) -> str:
+    """Select the arithmetic of GEMM-shaped contractions: "fp64" (FP64 MFMA),
+    "split_bf16" (BF16 slices on MFMA, FP64 recombination) or "split_int8"
+    (INT8 residues on MFMA, CRT recombination). Returns the previous mode."""
+    if mode not in _lib.PREC:
+        raise ValueError(f"unknown precision mode {mode!r}")
+    old = _precision[0]
+    _precision[0] = mode
+    return old
+
+
+def precision() -> str:
+    return _precision[0]
+
+
+# ------------------------------------------------------------------ types
+def _dtype_code(tdtype):
+    if tdtype == torch.float64:
+        return _lib.ITG_F64
+    if tdtype == torch.complex128:
+        return _lib.ITG_C128
+    raise ItgUnsupported(_lib.ITG_ERR_UNSUPPORTED, f"dtype {tdtype} (Float64 and ComplexF64 only)")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
+
+
+def _handle(dev):
+    h = _lib.handle_for(dev.index if dev.index is not None else torch.cuda.current_device())
+    h.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    return h
+
+
+class DenseTensor:
+    """Dense storage + dims (NDTensors/src/dense/dense.jl:5-16): a flat device
+    vector holding a column-major array."""
+
+    def __init__(self, data, dims):
+        dims = tuple(int(d) for d in dims)
+        if data.dim() != 1 or data.numel() != math.prod(dims):
+            raise ValueError("data must be a flat vector with prod(dims) elements")
+        self.data = data
+        self.dims = dims
+
+    @property
+    def eltype(self):
+        return self.data.dtype
+
+    @property
+    def ndims(self):
+        return len(self.dims)
+
+    def nnz(self):
+        return self.data.numel()
+
+    def to_numpy(self):
+        return self.data.detach().cpu().numpy().reshape(self.dims, order="F")
+
+    @staticmethod
+    def from_numpy(a, device="cuda"):
+        a = np.asarray(a)
+        flat = np.asfortranarray(a).reshape(-1, order="F")
+        return DenseTensor(torch.from_numpy(np.ascontiguousarray(flat)).to(device), a.shape)
+
+
+class BlockSparseTensor:
+    """BlockSparse storage: flat device data + insertion-ordered
+    {Block (1-based tuple): offset} + per-mode sector sizes
+    (NDTensors/src/blocksparse/blocksparse.jl:5-13, blockoffsets.jl:7-11,
+    blockdims.jl:8-19)."""
+
+    def __init__(self, data, blockoffsets, blockdims):
+        self.data = data
+        self.blockoffsets = OrderedDict((tuple(int(c) for c in b), int(o)) for b, o in blockoffsets.items())
+        self.blockdims = tuple(tuple(int(x) for x in bd) for bd in blockdims)
+
+    @property
+    def eltype(self):
+        return self.data.dtype
+
+    @property
+    def ndims(self):
+        return len(self.blockdims)
+
+    @property
+    def dims(self):
+        return tuple(sum(bd) for bd in self.blockdims)
+
+    def blockdims_of(self, block):
+        return tuple(self.blockdims[i][c - 1] for i, c in enumerate(block))
+
+    def nnzblocks(self):
+        return len(self.blockoffsets)
+
+    def nnz(self):
+        return sum(math.prod(self.blockdims_of(b)) for b in self.blockoffsets)
+
+    def block_numpy(self, block, host=None):
+        host = self.data.detach().cpu().numpy() if host is None else host
+        d = self.blockdims_of(block)
+        o = self.blockoffsets[block]
+        return host[o:o + math.prod(d)].reshape(d, order="F")
+
+    def to_numpy_dense(self):
+        """dense(T) (blocksparsetensor.jl:357-368) on the host."""
+        host = self.data.detach().cpu().numpy()
+        out = np.zeros(self.dims, dtype=host.dtype)
+        starts = [np.concatenate([[0], np.cumsum(bd)]) for bd in self.blockdims]
+        for b in self.blockoffsets:
+            sl = tuple(slice(starts[i][c - 1], starts[i][c]) for i, c in enumerate(b))
+            out[sl] = self.block_numpy(b, host)
+        return out
+
+
+def contract_labels(labels1, labels2):
+    """Output labels: the positive labels of T1 in order, then those of T2
+    (NDTensors/src/tensoroperations/contraction_logic.jl:5-34)."""
+    return tuple(x for x in labels1 if x > 0) + tuple(x for x in labels2 if x > 0)
+
+
+def _out_dims(T1, l1, T2, l2, lR):
+    out = []
+    for r in lR:
+        if r in l1:
+            out.append(T1.dims[l1.index(r)])
+        elif r in l2:
+            out.append(T2.dims[l2.index(r)])
+        else:
+            raise ValueError(f"label {r} of R is not in T1 or T2")
+    return tuple(out)
+
+
+def _promote(T1, T2):
+    """Mixed real/complex inputs are promoted by allocation, as
+    dense/tensoralgebra/contract.jl:196-211 does."""
+    if T1.eltype == T2.eltype:
+        return T1, T2
+    if T1.eltype == torch.float64:
+        T1 = _with_data(T1, T1.data.to(torch.complex128))
+    else:
+        T2 = _with_data(T2, T2.data.to(torch.complex128))
+    return T1, T2
+
+
+def _with_data(T, data):
+    if isinstance(T, DenseTensor):
+        return DenseTensor(data, T.dims)
+    return BlockSparseTensor(data, T.blockoffsets, T.blockdims)
+
+
+def contract_(R, labelsR, T1, labels1, T2, labels2, alpha=1.0, beta=0.0, *, prec=None,
+              conj1=False, conj2=False):
+    """contract!(R, labelsR, T1, labels1, T2, labels2, alpha, beta) for dense
+    tensors: R = alpha*T1*T2 + beta*R; beta == 0 never reads R."""
+    if not (isinstance(R, DenseTensor) and isinstance(T1, DenseTensor) and isinstance(T2, DenseTensor)):
+        raise TypeError("contract_ takes DenseTensors (block-sparse outputs come from contract)")
+    T1, T2 = _promote(T1, T2)
+    if R.eltype != T1.eltype:
+        raise TypeError(f"output eltype {R.eltype} does not match promoted input eltype {T1.eltype}")
+    dev = R.data.device
+    h = _handle(dev)
+    dt = _dtype_code(R.eltype)
+    lR, l1, l2 = _lib.i64(labelsR), _lib.i64(labels1), _lib.i64(labels2)
+    dR, d1, d2 = _lib.i64(R.dims), _lib.i64(T1.dims), _lib.i64(T2.dims)
+    a, _ka = _lib.scalar2(alpha)
+    b, _kb = _lib.scalar2(beta)
+    flags = (_lib.ITG_FLAG_CONJ_A if conj1 else 0) | (_lib.ITG_FLAG_CONJ_B if conj2 else 0)
+    p = _lib.PREC[prec or _precision[0]]
+    args = []
+    for arr in (dR, lR, d1, l1, d2, l2):
+        ptr, keep = _lib.ptr64(arr)
+        args.append((ptr, keep))
+    st = _lib.lib().itg_contract_dense_ex(
+        h.h, dt,
+        len(R.dims), args[0][0], args[1][0], _ptr(R.data),
+        len(T1.dims), args[2][0], args[3][0], _ptr(T1.data),
+        len(T2.dims), args[4][0], args[5][0], _ptr(T2.data),
+        a, b, p, flags)
+    h.check(st)
+    return R
+
+
+# ------------------------------------------------------- block-sparse plans
+class BSPlan:
+    """A libitg block-sparse contraction plan: the output block list (in the
+    reference's first-appearance order), its offsets, and the device tables of
+    the grouped launches."""
+
+    def __init__(self, T1, labels1, T2, labels2, labelsR):
+        dev = T1.data.device
+        self.h = _handle(dev)
+        L = _lib.lib()
+
+        def side(T):
+            nb = T.nnzblocks()
+            blocks = _lib.i64([c for b in T.blockoffsets for c in b]) if nb else _lib.i64([])
+            offs = _lib.i64(list(T.blockoffsets.values()))
+            nsec = _lib.i64([len(bd) for bd in T.blockdims])
+            secd = _lib.i64([x for bd in T.blockdims for x in bd])
+            return nb, blocks, offs, nsec, secd
+
+        n1, b1, o1, ns1, sd1 = side(T1)
+        n2, b2, o2, ns2, sd2 = side(T2)
+        l1, l2, lR = _lib.i64(labels1), _lib.i64(labels2), _lib.i64(labelsR)
+        keep = []
+
+        def pp(a):
+            ptr, k = _lib.ptr64(a)
+            keep.append(k)
+            return ptr
+
+        plan = ctypes.c_void_p()
+        st = L.itg_bs_plan_create(self.h.h, T1.ndims, n1, pp(b1), pp(o1), pp(ns1), pp(sd1), pp(l1),
+                                  T2.ndims, n2, pp(b2), pp(o2), pp(ns2), pp(sd2), pp(l2),
+                                  len(labelsR), pp(lR), ctypes.byref(plan))
+        self.h.check(st)
+        self.plan = plan
+        nblk, nnz, npairs = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        L.itg_bs_plan_sizes(plan, ctypes.byref(nblk), ctypes.byref(nnz), ctypes.byref(npairs))
+        self.nblkR, self.nnzR, self.npairs = nblk.value, nnz.value, npairs.value
+        NR = len(labelsR)
+        blocks = np.zeros(max(NR * self.nblkR, 1), dtype=np.int64)
+        offs = np.zeros(max(self.nblkR, 1), dtype=np.int64)
+        L.itg_bs_plan_blocks(plan, blocks.ctypes.data_as(_lib.I64P), offs.ctypes.data_as(_lib.I64P))
+        self.blocksR = [tuple(int(x) for x in blocks[i * NR:(i + 1) * NR]) for i in range(self.nblkR)]
+        self.offsR = [int(x) for x in offs[:self.nblkR]]
+
+    def pairs(self):
+        L = _lib.lib()
+        arr = np.zeros(max(3 * self.npairs, 1), dtype=np.int64)
+        L.itg_bs_plan_pairs(self.plan, arr.ctypes.data_as(_lib.I64P))
+        return [tuple(int(x) for x in arr[3 * i:3 * i + 3]) for i in range(self.npairs)]
+
+    def execute(self, R_data, T1_data, T2_data, alpha=1.0, prec=None, flags=0, pair_alpha=None):
+        h = _handle(R_data.device)
+        dt = _dtype_code(R_data.dtype)
+        a, _ka = _lib.scalar2(alpha)
+        pa = None
+        if pair_alpha is not None:
+            arr = np.zeros(2 * self.npairs, dtype=np.float64)
+            c = np.asarray(pair_alpha, dtype=np.complex128)
+            arr[0::2], arr[1::2] = c.real, c.imag
+            self._pa_keep = arr
+            pa = arr.ctypes.data_as(_lib.DP)
+        st = _lib.lib().itg_bs_execute(h.h, self.plan, dt, _ptr(R_data), _ptr(T1_data), _ptr(T2_data),
+                                       a, pa, _lib.PREC[prec or _precision[0]], int(flags))
+        h.check(st)
+
+    def autotune(self, R_data, T1_data, T2_data, prec=None):
+        h = _handle(R_data.device)
+        st = _lib.lib().itg_bs_autotune(h.h, self.plan, _dtype_code(R_data.dtype), _ptr(R_data),
+                                        _ptr(T1_data), _ptr(T2_data), _lib.PREC[prec or _precision[0]])
+        h.check(st)
+
+    def __del__(self):
+        try:
+            if getattr(self, "plan", None) and _lib._lib is not None:
+                _lib._lib.itg_bs_plan_destroy(self.plan)
+        except Exception:
+            pass
+
+
+_plan_cache: "OrderedDict[tuple, BSPlan]" = OrderedDict()
+_PLAN_CACHE_MAX = 64
+
+
+def _structure_key(T):
+    return (tuple(T.blockoffsets.items()), T.blockdims)
+
+
+def bs_plan(T1, labels1, T2, labels2, labelsR):
+    """Plan of a block-sparse contraction, cached by block structure (the
+    structure repeats every time a network is applied)."""
+    key = (_structure_key(T1), tuple(labels1), _structure_key(T2), tuple(labels2), tuple(labelsR),
+           T1.data.device.index)
+    p = _plan_cache.get(key)
+    if p is not None:
+        _plan_cache.move_to_end(key)
+        return p
+    p = BSPlan(T1, labels1, T2, labels2, labelsR)
+    _plan_cache[key] = p
+    while len(_plan_cache) > _PLAN_CACHE_MAX:
+        _plan_cache.popitem(last=False)
+    return p
+
+
+def contract_blockoffsets(T1, labels1, T2, labels2, labelsR):
+    """(blockoffsetsR, nnzR, plan) with plan = [(block1, block2, blockR)] as
+    1-based block tuples in the reference's order
+    (blocksparse/contract_sequential.jl:1-41)."""
+    p = bs_plan(T1, labels1, T2, labels2, labelsR)
+    offs = OrderedDict(zip(p.blocksR, p.offsR))
+    b1 = list(T1.blockoffsets)
+    b2 = list(T2.blockoffsets)
+    plan = [(b1[i], b2[j], p.blocksR[r]) for i, j, r in p.pairs()]
+    return offs, p.nnzR, plan
+
+
+def contract(T1, labels1, T2, labels2, labelsR=None, *, alpha=1.0, prec=None, conj1=False,
+             conj2=False):
+    """NDTensors.contract(T1, labels1, T2, labels2[, labelsR]): allocate the
+    (uninitialised) output and contract into it."""
+    labels1, labels2 = tuple(labels1), tuple(labels2)
+    if labelsR is None:
+        labelsR = contract_labels(labels1, labels2)
+    labelsR = tuple(labelsR)
+    T1, T2 = _promote(T1, T2)
+    if isinstance(T1, DenseTensor) and isinstance(T2, DenseTensor):
+        dims = _out_dims(T1, labels1, T2, labels2, labelsR)
+        R = DenseTensor(torch.empty(math.prod(dims), dtype=T1.eltype, device=T1.data.device), dims)
+        return contract_(R, labelsR, T1, labels1, T2, labels2, alpha, 0.0, prec=prec,
+                         conj1=conj1, conj2=conj2)
+    if isinstance(T1, BlockSparseTensor) and isinstance(T2, BlockSparseTensor):
+        p = bs_plan(T1, labels1, T2, labels2, labelsR)
+        blockdims = []
+        for r in labelsR:
+            if r in labels1:
+                blockdims.append(T1.blockdims[labels1.index(r)])
+            else:
+                blockdims.append(T2.blockdims[labels2.index(r)])
+        data = torch.empty(p.nnzR, dtype=T1.eltype, device=T1.data.device)
+        R = BlockSparseTensor(data, OrderedDict(zip(p.blocksR, p.offsR)), blockdims)
+        if p.npairs:
+            flags = (_lib.ITG_FLAG_CONJ_A if conj1 else 0) | (_lib.ITG_FLAG_CONJ_B if conj2 else 0)
+            p.execute(data, T1.data, T2.data, alpha=alpha, prec=prec, flags=flags)
+        return R
+    raise ItgUnsupported(_lib.ITG_ERR_UNSUPPORTED, "mixed Dense / BlockSparse contraction")
+
+
+def permutedims_(R, T, perm, alpha=1.0, beta=0.0):
+    """R = alpha*permutedims(T, perm) + beta*R (perm 1-based, Julia's)."""
+    h = _handle(T.data.device)
+    d, _k = _lib.ptr64(_lib.i64(T.dims))
+    pp, _kp = _lib.ptr32([int(x) for x in perm])
+    a, _ka = _lib.scalar2(alpha)
+    b, _kb = _lib.scalar2(beta)
+    h.check(_lib.lib().itg_permute(h.h, _dtype_code(T.eltype), T.ndims, d, pp, _ptr(T.data),
+                                   _ptr(R.data), a, b))
+    return R
+
+
+def permutedims(T, perm):
+    """permutedims(T, perm) for Dense and BlockSparse tensors (perm 1-based).
+    Block-sparse: every block is permuted; block coordinates are permuted
+    and the blocks keep their order and offsets."""
+    perm = tuple(int(x) for x in perm)
+    if isinstance(T, DenseTensor):
+        dims = tuple(T.dims[p - 1] for p in perm)
+        R = DenseTensor(torch.empty_like(T.data), dims)
+        return permutedims_(R, T, perm)
+    h = _handle(T.data.device)
+    blocks = list(T.blockoffsets)
+    bd = _lib.i64([d for b in blocks for d in T.blockdims_of(b)])
+    offs = _lib.i64(list(T.blockoffsets.values()))
+    keep = []
+
+    def pp(a):
+        p, k = _lib.ptr64(a)
+        keep.append(k)
+        return p
+
+    # output offsets accumulate in block order (blockoffsets.jl:103-111)
+    dst = []
+    acc = 0
+    for b in blocks:
+        dst.append(acc)
+        acc += math.prod(T.blockdims_of(b))
+    out = torch.empty(acc, dtype=T.eltype, device=T.data.device)
+    p32, _k32 = _lib.ptr32(list(perm))
+    h.check(_lib.lib().itg_bs_permute(h.h, _dtype_code(T.eltype), T.ndims, len(blocks), pp(bd),
+                                      pp(offs), pp(_lib.i64(dst)), p32, _ptr(T.data), _ptr(out)))
+    new_offs = OrderedDict((tuple(b[p - 1] for p in perm), o) for b, o in zip(blocks, dst))
+    return BlockSparseTensor(out, new_offs, tuple(T.blockdims[p - 1] for p in perm))

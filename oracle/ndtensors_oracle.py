@@ -1,0 +1,429 @@
+"""CPU oracle for the NDTensors contraction path -- TEST INFRASTRUCTURE ONLY.
+
+This module is a from-scratch numpy restatement of what ITensors.jl /
+NDTensors.jl compute on the `*` / `contract` path. It exists to CHECK the
+MI355X backend: only the test-suite, the smoke check and the CPU-baseline leg
+of the benchmark may import it. The product (the `itg` package and libitg.so)
+never imports, calls or links anything in this directory.
+
+Each function cites the reference file:line it restates (paths relative to the
+ITensors.jl source tree, v0.9.27 / NDTensors v0.4.27).
+
+Pinning status (what checks each function; see tests/test_oracle.py):
+  qn_add / qn_eq / flux / nzblocks   -- known-answer tests from the reference's
+                                        test_qnitensor.jl:564-583, :937-947,
+                                        brute-force flux invariants
+  compute_contraction_labels         -- closed-form rules + arrow-error KAT
+                                        (test_qnitensor.jl:1799-1805)
+  contract_labels / contract_inds    -- closed-form rules, brute force vs einsum
+  contract_blockoffsets_sequential   -- brute force over all block pairs, flux
+                                        invariant flux(C)=flux(A)+flux(B), the
+                                        zero-block case (test_threading.jl:60-78),
+                                        densify-and-compare
+  dense_contract                     -- textbook reductions: matrix product vs
+                                        explicit loops, outer product, scalar
+                                        results, alpha/beta + NaN overwrite
+                                        (test_dense.jl:237-287)
+  blocksparse_contract               -- densify(result) == dense_contract(
+                                        densify(A), densify(B)) on random
+                                        structures
+  rrule_contract                     -- central finite differences
+"""
+from __future__ import annotations
+
+import itertools
+from collections import OrderedDict
+
+import numpy as np
+
+# ---------------------------------------------------------------- QN algebra
+# A QN is a dict {name: (value, modulus)}; a missing name means value 0.
+# Restates src/lib/QuantumNumbers/src/qn.jl:6-30 (up to 4 named values, an
+# optional modulus) and :178-207 (+ adds matching names, missing = 0).
+
+
+def QN(*vals):
+    """QN(v) (unnamed), QN(("Sz", v)), QN(("P", v, 2), ("Sz", v)) ...
+
+    qn.jl:23-75: a bare integer is an unnamed value; tuples are
+    (name, value[, modulus]).
+    """
+    out = {}
+    if len(vals) == 1 and isinstance(vals[0], (int, np.integer)):
+        vals = (("", int(vals[0])),)
+    if len(vals) == 2 and isinstance(vals[0], (int, np.integer)) and isinstance(vals[1], (int, np.integer)):
+        vals = (("", int(vals[0]), int(vals[1])),)
+    for v in vals:
+        name, val = v[0], int(v[1])
+        mod = int(v[2]) if len(v) > 2 else 1
+        if mod > 1:
+            val %= mod
+        out[name] = (val, mod)
+    return out
+
+
+def qn_add(a, b):
+    """qn.jl:178-207 -- elementwise addition of matching names (mod m)."""
+    out = dict(a)
+    for name, (v, m) in b.items():
+        if name in out:
+            va, ma = out[name]
+            if ma != m:
+                raise ValueError("QNVals must have same modulus")
+            s = va + v
+            out[name] = (s % m if m > 1 else s, m)
+        else:
+            out[name] = (v, m)
+    return out
+
+
+def qn_scale(d, a):
+    """Arrow * QN (qn.jl:156-162): multiply every value by the arrow (+1/-1)."""
+    return {n: ((d * v) % m if m > 1 else d * v, m) for n, (v, m) in a.items()}
+
+
+def qn_eq(a, b):
+    """Equality with missing names treated as zero (qn.jl:248-280)."""
+    for name in set(a) | set(b):
+        va = a.get(name, (0, 1))[0]
+        vb = b.get(name, (0, 1))[0]
+        if va != vb:
+            return False
+    return True
+
+
+# ------------------------------------------------------------------- Index
+class Index:
+    """Index identity (id, plev, tags) + space + arrow (src/index.jl:24-33).
+
+    `space` is an int (dense) or a list of (QN, dim) sectors (QNIndex,
+    src/qn/qnindex.jl:6-8). dir: +1 = Out, -1 = In, 0 = Neither; QN indices
+    default to Out (qnindex.jl:128).
+    """
+
+    _next_id = [1]
+
+    def __init__(self, space, tags="", plev=0, dir=None, id=None):
+        self.space = space
+        self.tags = tags
+        self.plev = plev
+        if id is None:
+            id = Index._next_id[0]
+            Index._next_id[0] += 1
+        self.id = id
+        if dir is None:
+            dir = 1 if self.hasqns else 0
+        self.dir = dir
+
+    @property
+    def hasqns(self):
+        return not isinstance(self.space, (int, np.integer))
+
+    @property
+    def dim(self):
+        return int(self.space) if not self.hasqns else sum(d for _, d in self.space)
+
+    def nblocks(self):
+        return 1 if not self.hasqns else len(self.space)
+
+    def blockdim(self, b):
+        return self.dim if not self.hasqns else self.space[b][1]
+
+    def qn(self, b):
+        return self.space[b][0]
+
+    def __eq__(self, other):
+        # src/index.jl:354-355: id, plev and tags; the arrow is not compared
+        return (self.id, self.plev, self.tags) == (other.id, other.plev, other.tags)
+
+    def __hash__(self):
+        return hash((self.id, self.plev, self.tags))
+
+    def dag(self):
+        """src/index.jl:362 -- flip the arrow, keep identity."""
+        return Index(self.space, self.tags, self.plev, -self.dir, self.id)
+
+    def prime(self, n=1):
+        return Index(self.space, self.tags, self.plev + n, self.dir, self.id)
+
+    def __repr__(self):
+        return f"Index(id={self.id}, dim={self.dim}, dir={self.dir}, plev={self.plev})"
+
+
+def compute_contraction_labels(Ais, Bis):
+    """src/indexset.jl:672-707.
+
+    Every (i, j) with Ais[i] == Bis[j] gets the next negative label
+    -1, -2, ... in (i outer, j inner) order; QN indices that match must have
+    opposite arrows. Free labels are then numbered ncont+1, ... over A, then B.
+    """
+    have_qns = all(i.hasqns for i in Ais) and all(j.hasqns for j in Bis)
+    la = [0] * len(Ais)
+    lb = [0] * len(Bis)
+    ncont = 0
+    for i, a in enumerate(Ais):
+        for j, b in enumerate(Bis):
+            if a == b:
+                if have_qns and a.dir != -b.dir:
+                    raise ValueError("QN indices must have opposite direction to contract")
+                ncont += 1
+                la[i] = lb[j] = -ncont
+    u = ncont
+    for i in range(len(la)):
+        if la[i] == 0:
+            u += 1
+            la[i] = u
+    for j in range(len(lb)):
+        if lb[j] == 0:
+            u += 1
+            lb[j] = u
+    return tuple(la), tuple(lb)
+
+
+def contract_labels(l1, l2):
+    """NDTensors/src/tensoroperations/contraction_logic.jl:5-34: output labels
+    are the positive labels of T1 in order, then those of T2."""
+    return tuple(x for x in l1 if x > 0) + tuple(x for x in l2 if x > 0)
+
+
+def contract_inds(is1, l1, is2, l2, lR):
+    """contraction_logic.jl:36-62: each output label takes the index of the
+    tensor (T1 first) that carries it."""
+    out = []
+    for r in lR:
+        if r in l1:
+            out.append(is1[l1.index(r)])
+        else:
+            out.append(is2[l2.index(r)])
+    return tuple(out)
+
+
+# --------------------------------------------------------- block structure
+def eachblock(inds):
+    """NDTensors/src/blocksparse/blockdims.jl:104-106: CartesianIndices over
+    the block grid, i.e. column-major (first mode fastest). 0-based here."""
+    shape = [i.nblocks() for i in inds]
+    for rev in itertools.product(*[range(n) for n in reversed(shape)]):
+        yield tuple(reversed(rev))
+
+
+def flux_of_block(inds, block):
+    """src/indexset.jl:876-883 with flux(i, b) = dir(i) * qn(i, b)
+    (src/qn/qnindex.jl:242)."""
+    tot = {}
+    for i, b in zip(inds, block):
+        tot = qn_add(tot, qn_scale(i.dir, i.qn(b)))
+    return tot
+
+
+def nzblocks(flux, inds):
+    """src/qn/qnindexset.jl:9-18: blocks (eachblock order) whose flux equals
+    the requested one."""
+    return [b for b in eachblock(inds) if qn_eq(flux_of_block(inds, b), flux)]
+
+
+def blockdims(inds, block):
+    return tuple(i.blockdim(b) for i, b in zip(inds, block))
+
+
+def blockoffsets(blocks, inds):
+    """NDTensors/src/blocksparse/blockoffsets.jl:70-79: offsets accumulate in
+    the given block order."""
+    offs = OrderedDict()
+    nnz = 0
+    for b in blocks:
+        offs[b] = nnz
+        nnz += int(np.prod(blockdims(inds, b)))
+    return offs, nnz
+
+
+class BlockSparse:
+    """A BlockSparseTensor: flat data + ordered {block: offset} + inds
+    (NDTensors/src/blocksparse/blocksparse.jl:5-13). Blocks are 0-based
+    tuples here (the reference's Block{N} is the same tuple plus one)."""
+
+    def __init__(self, data, offsets, inds):
+        self.data = data
+        self.offsets = offsets
+        self.inds = tuple(inds)
+
+    def block(self, b):
+        """blockview (blocksparsetensor.jl:327-352): a column-major view."""
+        dims = blockdims(self.inds, b)
+        o = self.offsets[b]
+        n = int(np.prod(dims))
+        return self.data[o:o + n].reshape(dims, order="F")
+
+    def dense(self):
+        """dense(T) (blocksparsetensor.jl:357-368): scatter blocks into a
+        zero array."""
+        shape = tuple(i.dim for i in self.inds)
+        out = np.zeros(shape, dtype=self.data.dtype)
+        starts = [np.concatenate([[0], np.cumsum([i.blockdim(b) for b in range(i.nblocks())])]) for i in self.inds]
+        for b in self.offsets:
+            sl = tuple(slice(starts[m][bm], starts[m][bm + 1]) for m, bm in enumerate(b))
+            out[sl] = self.block(b)
+        return out
+
+
+def random_blocksparse(rng, flux, inds, dtype=np.float64):
+    """ITensor(ElT, flux, inds) + randn! (src/qn/qnitensor.jl:158-166,
+    :387-393): allowed blocks in eachblock order, data filled in flat
+    storage order from the generator (N(0,1) real; for complex the real and
+    imaginary parts are each N(0,1))."""
+    blocks = nzblocks(flux, inds)
+    if not blocks:
+        raise ValueError("ITensor with flux resulted in no allowed blocks")
+    offs, nnz = blockoffsets(blocks, inds)
+    data = rng.standard_normal(nnz)
+    if np.dtype(dtype).kind == "c":
+        data = data + 1j * rng.standard_normal(nnz)
+    return BlockSparse(data.astype(dtype), offs, inds)
+
+
+# ------------------------------------------------------- dense contraction
+def _einsum_spec(l1, l2, lR):
+    letters = {}
+
+    def let(x):
+        if x not in letters:
+            letters[x] = chr(ord("a") + len(letters)) if len(letters) < 26 else chr(ord("A") + len(letters) - 26)
+        return letters[x]
+
+    return "".join(let(x) for x in l1) + "," + "".join(let(x) for x in l2) + "->" + "".join(let(x) for x in lR)
+
+
+def dense_contract(R, lR, A, lA, B, lB, alpha=1.0, beta=0.0, exact=False):
+    """R = alpha * A*B + beta * R in the label convention
+    (NDTensors/src/dense/tensoralgebra/contract.jl:160-216).
+
+    beta == 0 overwrites R without reading it (NaN-safe,
+    NDTensors/test/test_dense.jl:237-259). `R` may be None when beta == 0.
+    `exact=True` evaluates in extended precision (numpy longdouble, 63-bit
+    mantissa) for use as a tight value oracle; otherwise float64 BLAS.
+    Returns the new R (a new array).
+    """
+    spec = _einsum_spec(tuple(lA), tuple(lB), tuple(lR))
+    cplx = np.iscomplexobj(A) or np.iscomplexobj(B) or np.iscomplexobj(alpha) or (R is not None and np.iscomplexobj(R))
+    if exact:
+        dt = np.clongdouble if cplx else np.longdouble
+        AB = np.einsum(spec, np.asarray(A, dtype=dt), np.asarray(B, dtype=dt), optimize=False)
+    else:
+        AB = np.einsum(spec, A, B, optimize=True)
+    out = alpha * AB
+    if beta != 0:
+        out = out + beta * R
+    return out
+
+
+def contract_dense(A, lA, B, lB, lR=None, **kw):
+    """NDTensors.contract(T1, l1, T2, l2[, lR]) for Dense data
+    (generic_tensor_operations.jl:87-119): allocate R and contract into it."""
+    if lR is None:
+        lR = contract_labels(lA, lB)
+    return dense_contract(None, lR, A, lA, B, lB, **kw), lR
+
+
+# ---------------------------------------------- block-sparse contraction
+def _find_matching_positions(t1, t2):
+    """contract_utilities.jl:45-55 (0 = absent; 1-based positions)."""
+    out = []
+    for x in t1:
+        pos = 0
+        for j, y in enumerate(t2):
+            if x == y:
+                pos = j + 1
+        out.append(pos)
+    return out
+
+
+def contract_blockoffsets_sequential(T1, l1, T2, l2, lR):
+    """NDTensors/src/blocksparse/contract_sequential.jl:1-41 with the block
+    rules of contract_utilities.jl:57-91.
+
+    Iterates block1 over T1's blocks (outer), block2 over T2's (inner); a pair
+    contracts when every contracted mode has the same block number. The
+    output block takes the free block numbers into their R positions. Output
+    offsets are assigned on first appearance. Returns (offsetsR, nnzR, plan)
+    with plan = [(block1, block2, blockR), ...].
+    """
+    indsR = contract_inds(T1.inds, l1, T2.inds, l2, lR)
+    m12 = _find_matching_positions(l1, l2)
+    m1R = _find_matching_positions(l1, lR)
+    m2R = _find_matching_positions(l2, lR)
+    offsR = OrderedDict()
+    nnzR = 0
+    plan = []
+    for b1 in T1.offsets:
+        for b2 in T2.offsets:
+            ok = all(p == 0 or b1[i] == b2[p - 1] for i, p in enumerate(m12))
+            if not ok:
+                continue
+            bR = [0] * len(lR)
+            for i, p in enumerate(m1R):
+                if p > 0:
+                    bR[p - 1] = b1[i]
+            for i, p in enumerate(m2R):
+                if p > 0:
+                    bR[p - 1] = b2[i]
+            bR = tuple(bR)
+            plan.append((b1, b2, bR))
+            if bR not in offsR:
+                offsR[bR] = nnzR
+                nnzR += int(np.prod(blockdims(indsR, bR)))
+    return offsR, nnzR, plan, indsR
+
+
+def blocksparse_contract(T1, l1, T2, l2, lR=None, alpha=1.0, exact=False, conj1=False, conj2=False):
+    """NDTensors.contract(::BlockSparseTensor, ...) (blocksparse/contract.jl:3-76,
+    contract_generic.jl:37-129): plan, allocate R, then for each output block
+    the pairs in plan order with beta = 0 for the first and 1 afterwards.
+    Returns (R, plan)."""
+    if lR is None:
+        lR = contract_labels(l1, l2)
+    offsR, nnzR, plan, indsR = contract_blockoffsets_sequential(T1, l1, T2, l2, lR)
+    cplx = np.iscomplexobj(T1.data) or np.iscomplexobj(T2.data) or np.iscomplexobj(alpha)
+    dt = np.complex128 if cplx else np.float64
+    if exact:
+        dt = np.clongdouble if cplx else np.longdouble
+    data = np.full(nnzR, np.nan, dtype=dt)  # uninitialised output (similar.jl:5-8)
+    R = BlockSparse(data, offsR, indsR)
+    written = set()
+    for b1, b2, bR in plan:
+        A = T1.block(b1)
+        B = T2.block(b2)
+        if conj1:
+            A = np.conj(A)
+        if conj2:
+            B = np.conj(B)
+        view = R.block(bR)
+        beta = 1.0 if bR in written else 0.0
+        view[...] = dense_contract(view, lR, A, l1, B, l2, alpha, beta, exact=exact)
+        written.add(bR)
+    return R, plan
+
+
+# ------------------------------------------------------------------- adjoint
+def rrule_contract(x1, l1, x2, l2, ybar, lR):
+    """ext/ITensorsChainRulesCoreExt/itensor.jl:94-103 on dense data:
+    x1bar = ybar * dag(x2), x2bar = dag(x1) * ybar, projected to each input's
+    element type (projection.jl:1-10). dag conjugates the data."""
+    x1bar = np.einsum(_einsum_spec(tuple(lR), tuple(l2), tuple(l1)), ybar, np.conj(x2))
+    x2bar = np.einsum(_einsum_spec(tuple(l1), tuple(lR), tuple(l2)), np.conj(x1), ybar)
+    if not np.iscomplexobj(x1):
+        x1bar = np.real(x1bar)
+    if not np.iscomplexobj(x2):
+        x2bar = np.real(x2bar)
+    return x1bar, x2bar
+
+
+# ------------------------------------------------ H_eff network (oracle)
+def heff_apply_dense(L, psi, W1, W2, R, exact=False):
+    """((((L*psi)*W1)*W2)*R) with the left-associative fold of
+    src/tensor_operations/tensor_algebra.jl:121-134, on dense arrays with
+    mode orders L[a',w,a], psi[a,s1,s2,b], W[wl,s,s',wr], R[b',w,b]."""
+    X, lX = contract_dense(L, (1, 2, -1), psi, (-1, 3, 4, 5), exact=exact)      # X[a',w,s1,s2,b]
+    Y, lY = contract_dense(X, (1, -1, -2, 4, 5), W1, (-1, -2, 6, 7), exact=exact)  # Y[a',s2,b,s1',w']
+    Z, lZ = contract_dense(Y, (1, -1, 3, 4, -2), W2, (-2, -1, 6, 7), exact=exact)  # Z[a',b,s1',s2',w'']
+    O, lO = contract_dense(Z, (1, -1, 3, 4, -2), R, (5, -2, -1), exact=exact)     # O[a',s1',s2',b']
+    return O
