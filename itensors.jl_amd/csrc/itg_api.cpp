@@ -599,7 +599,7 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   float best_ms = 1e30f;
   std::string err;
   for (int cfg = 0; cfg < gemm_num_cfgs(); ++cfg) {
-    if (cplx && cfg != 0 && cfg != 2) continue;
+    if (!gemm_cfg_ok(cfg, cplx)) continue;
     // warm-up, then 3 timed runs
     e = exec_run(ep, cplx, (int)prec, cfg, h->stream, h->ws, T1, T2, R, 0u, 1.0, 0.0, 0.0, 0.0, &err);
     if (e != hipSuccess) break;

@@ -85,25 +85,26 @@ hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pair
 constexpr int kDirectElemsPerWG = 256 * 2;
 
 // Grouped GEMM (FP64 MFMA). cfg selects the tile shape; ta/tb: A stored
-// K x M / B stored N x K. tile_prefix over `order` (groups sorted by cost).
+// K x M / B stored N x K. `tiles` holds one {group, tile index in group} per
+// workgroup ({-1, -1} = idle slot), laid out so that workgroup w runs on
+// XCD w % 8.
 struct GemmTileCfg {
   int bm, bn;
 };
 int gemm_num_cfgs();
 GemmTileCfg gemm_cfg(int cfg);
+bool gemm_cfg_ok(int cfg, int cplx);
 hipError_t launch_gemm_f64(hipStream_t s, int cfg, int cplx, int ta, int tb,
-                           const GemmPair* pairs, const GemmGroup* groups,
-                           const int32_t* order, const int64_t* tile_prefix,
-                           int64_t ngroups, int64_t total_tiles, const void* A,
-                           const void* B, void* C, unsigned flags, double ga_re, double ga_im,
-                           double beta_re, double beta_im);
+                           const GemmPair* pairs, const GemmGroup* groups, const int2* tiles,
+                           int64_t grid_size, const void* A, const void* B, void* C,
+                           unsigned flags, double ga_re, double ga_im, double beta_re,
+                           double beta_im);
 
 // Split-precision grouped GEMM (real Float64 only). prec: 1 = BF16 slices,
-// 2 = INT8 residues (CRT). Needs workspace for the per-row/column scales.
+// 2 = INT8 residues (CRT).
 hipError_t launch_gemm_split(hipStream_t s, int prec, int cfg, int ta, int tb,
-                             const GemmPair* pairs, const GemmGroup* groups,
-                             const int32_t* order, const int64_t* tile_prefix,
-                             int64_t ngroups, int64_t total_tiles, const void* A,
-                             const void* B, void* C, double beta_re);
+                             const GemmPair* pairs, const GemmGroup* groups, const int2* tiles,
+                             int64_t grid_size, const void* A, const void* B, void* C,
+                             double ga_re, double beta_re);
 
 }  // namespace itg

@@ -521,39 +521,65 @@ int default_gemm_cfg(const ExecPlan& ep, int cplx) {
 }
 
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
-  if (cplx && cfg != 0 && cfg != 2) cfg = 0;
+  if (!gemm_cfg_ok(cfg, cplx)) cfg = 0;
   auto it = ep.launches.find(cfg);
   if (it != ep.launches.end()) return hipSuccess;
   std::unique_ptr<GemmLaunch> gl(new GemmLaunch);
   gl->cfg = cfg;
-  const GemmTileCfg tc = gemm_cfg(cfg);
-  const size_t ng = ep.h_groups.size();
-  std::vector<int32_t> order(ng);
-  std::iota(order.begin(), order.end(), 0);
-  // largest per-tile work first (per-tile cost ~ sum of K over the pairs)
-  std::vector<double> tile_work(ng);
-  for (size_t g = 0; g < ng; ++g) {
-    double k = 0;
-    for (int p = ep.h_groups[g].p0; p < ep.h_groups[g].p1; ++p) k += ep.h_pairs[p].K;
-    tile_work[g] = k;
-  }
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return tile_work[a] > tile_work[b]; });
-  std::vector<int64_t> prefix(ng + 1, 0);
-  for (size_t i = 0; i < ng; ++i) {
-    const GemmGroup& g = ep.h_groups[order[i]];
-    const int64_t t = (int64_t)((g.M + tc.bm - 1) / tc.bm) * ((g.N + tc.bn - 1) / tc.bn);
-    prefix[i + 1] = prefix[i] + t;
-  }
-  gl->ngroups = (int64_t)ng;
-  gl->tiles = prefix.back();
-  hipError_t e;
-  if ((e = gl->order.upload(order.data(), order.size() * sizeof(int32_t), s)) != hipSuccess)
-    return e;
-  if ((e = gl->prefix.upload(prefix.data(), prefix.size() * sizeof(int64_t), s)) != hipSuccess)
-    return e;
+  std::vector<int2> table = xcd_tile_table(ep, gemm_cfg(cfg));
+  gl->grid = (int64_t)table.size();
+  hipError_t e = gl->tiles.upload(table.data(), table.size() * sizeof(int2), s);
+  if (e != hipSuccess) return e;
   ep.launches[cfg] = std::move(gl);
   return hipSuccess;
+}
+
+// Workgroup w of a launch is dispatched to XCD w % kXcds. Cut the tiles of
+// every group into chunks (a group's tiles in tm-fastest order; big groups
+// are split so that every XCD gets several chunks), assign chunks to XCDs
+// largest-work-first to the least loaded one, and interleave the per-XCD
+// lists so each XCD runs its own chunks back to back: the tiles of a chunk
+// share their A and B panels through that XCD's L2.
+std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc) {
+  constexpr int kXcds = 8;
+  struct Chunk {
+    int32_t g, t0, t1;
+    double work;
+  };
+  std::vector<Chunk> chunks;
+  int64_t total = 0;
+  std::vector<int32_t> gt(ep.h_groups.size());
+  for (size_t g = 0; g < ep.h_groups.size(); ++g) {
+    const GemmGroup& G = ep.h_groups[g];
+    gt[g] = (G.M <= 0 || G.N <= 0) ? 0 : ((G.M + tc.bm - 1) / tc.bm) * ((G.N + tc.bn - 1) / tc.bn);
+    total += gt[g];
+  }
+  const int64_t unit = std::max<int64_t>(1, (total + 8 * kXcds - 1) / (8 * kXcds));
+  for (size_t g = 0; g < ep.h_groups.size(); ++g) {
+    double k = 0;
+    for (int p = ep.h_groups[g].p0; p < ep.h_groups[g].p1; ++p) k += std::max(ep.h_pairs[p].K, 1);
+    for (int64_t t0 = 0; t0 < gt[g]; t0 += unit) {
+      const int64_t t1 = std::min<int64_t>(gt[g], t0 + unit);
+      chunks.push_back(Chunk{(int32_t)g, (int32_t)t0, (int32_t)t1, k * (double)(t1 - t0)});
+    }
+  }
+  std::stable_sort(chunks.begin(), chunks.end(),
+                   [](const Chunk& a, const Chunk& b) { return a.work > b.work; });
+  std::vector<std::vector<int2>> bins(kXcds);
+  std::vector<double> load(kXcds, 0.0);
+  for (const Chunk& c : chunks) {
+    int best = 0;
+    for (int x = 1; x < kXcds; ++x)
+      if (load[x] < load[best]) best = x;
+    load[best] += c.work;
+    for (int32_t t = c.t0; t < c.t1; ++t) bins[best].push_back(int2{c.g, t});
+  }
+  size_t len = 0;
+  for (auto& b : bins) len = std::max(len, b.size());
+  std::vector<int2> table(len * kXcds, int2{-1, -1});
+  for (int x = 0; x < kXcds; ++x)
+    for (size_t i = 0; i < bins[x].size(); ++i) table[i * kXcds + x] = bins[x][i];
+  return table;
 }
 
 size_t exec_workspace_bytes(const ExecPlan& ep, int cplx) {
@@ -616,15 +642,23 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     *err = std::string("gemm table upload: ") + hipGetErrorString(e);
     return e;
   }
-  int ecfg = (cplx && cfg != 0 && cfg != 2) ? 0 : cfg;
+  int ecfg = gemm_cfg_ok(cfg, cplx) ? cfg : 0;
   GemmLaunch& gl = *ep.launches[ecfg];
-  if (prec == 0 || cplx) {
+  if (prec == 0) {
     e = launch_gemm_f64(s, ecfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
-                        (const GemmGroup*)ep.g_groups.p, (const int32_t*)gl.order.p,
-                        (const int64_t*)gl.prefix.p, gl.ngroups, gl.tiles, A, B, C, gflags, a_re,
-                        a_im, gbr, gbi);
+                        (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B,
+                        C, gflags, a_re, a_im, gbr, gbi);
+  } else if (cplx) {
+    *err = "split precision is implemented for Float64 only";
+    return hipErrorNotSupported;
   } else {
-    e = hipErrorNotSupported;
+    e = launch_gemm_split(s, prec, ecfg, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
+                          (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B,
+                          C, a_re, gbr);
+    if (e == hipErrorNotSupported) {
+      *err = "this split-precision form is not available";
+      return e;
+    }
   }
   if (e != hipSuccess) {
     *err = std::string("gemm launch: ") + hipGetErrorString(e);

@@ -76,8 +76,8 @@ struct PermuteLaunch {
 
 struct GemmLaunch {
   int cfg = -1;
-  int64_t ngroups = 0, tiles = 0;
-  DevBuf order, prefix;
+  int64_t grid = 0;  // workgroups, including idle padding slots
+  DevBuf tiles;      // int2 {group, tile} per workgroup
 };
 
 struct ExecPlan {
@@ -120,5 +120,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
 // Tile cfg picked by the built-in heuristic (the autotuner may override).
 int default_gemm_cfg(const ExecPlan& ep, int cplx);
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s);
+// XCD-balanced workgroup -> {group, tile} table for tile shape tc.
+std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc);
 
 }  // namespace itg

@@ -13,6 +13,9 @@
 // block (column-major), so writes are contiguous and reads are contiguous
 // along any output mode that is also a fastest-varying mode of A or B. The
 // pairs of a block are summed in plan order: deterministic, no atomics.
+// The number of output modes is a template parameter so that the per-mode
+// coordinates stay in registers; index arithmetic is 32-bit (blocks hold
+// fewer than 2^31 elements, which the planner checks).
 // Replaces the per-pair permute + BLAS loop of contract_generic.jl:78-129 for
 // this shape class.
 #include "itg_internal.h"
@@ -36,58 +39,71 @@ __device__ inline int64_t find_group(const int64_t* prefix, int64_t n, int64_t w
   return lo;
 }
 
-template <int CPLX>
+template <int CPLX, int NR>
 __global__ void __launch_bounds__(kThreads)
-    direct_kernel(int nr, const DirectPair* __restrict__ pairs,
-                  const DirectGroup* __restrict__ groups, const int32_t* __restrict__ koffs,
-                  const int64_t* __restrict__ prefix, int64_t ngroups,
-                  const double* __restrict__ A, const double* __restrict__ B,
+    direct_kernel(const DirectPair* __restrict__ pairs, const DirectGroup* __restrict__ groups,
+                  const int32_t* __restrict__ koffs, const int64_t* __restrict__ prefix,
+                  int64_t ngroups, const double* __restrict__ A, const double* __restrict__ B,
                   double* __restrict__ C, unsigned flags, double ga_re, double ga_im,
                   double beta_re, double beta_im) {
   const int64_t wg = blockIdx.x;
   const int64_t g = find_group(prefix, ngroups, wg);
-  const DirectGroup grp = groups[g];
-  const int64_t e0 = (wg - prefix[g]) * kDirectElemsPerWG;
+  const DirectGroup& grp = groups[g];
+  const uint32_t nelem = (uint32_t)grp.nelem;
+  const uint32_t e0 = (uint32_t)(wg - prefix[g]) * kDirectElemsPerWG;
   const bool conja = (flags & 1u) != 0, conjb = (flags & 2u) != 0;
   const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
+  uint32_t cdim[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) cdim[i] = (uint32_t)grp.cdim[i];
 
-  int32_t coord[kPerThread][kMaxModes];
+  uint32_t coord[kPerThread][NR > 0 ? NR : 1];
   bool live[kPerThread];
   double acc_re[kPerThread], acc_im[kPerThread];
 #pragma unroll
   for (int r = 0; r < kPerThread; ++r) {
-    const int64_t e = e0 + r * kThreads + threadIdx.x;
-    live[r] = e < grp.nelem;
-    int64_t rem = live[r] ? e : 0;
-    for (int i = 0; i < kMaxModes; ++i) {
-      if (i < nr) {
-        const int64_t d = grp.cdim[i];
-        const int64_t q = rem / d;
-        coord[r][i] = (int32_t)(rem - q * d);
+    const uint32_t e = e0 + r * kThreads + threadIdx.x;
+    live[r] = e < nelem;
+    uint32_t rem = live[r] ? e : 0u;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      if (i + 1 < NR) {
+        const uint32_t q = rem / cdim[i];
+        coord[r][i] = rem - q * cdim[i];
         rem = q;
       } else {
-        coord[r][i] = 0;
+        coord[r][i] = rem;
       }
     }
     acc_re[r] = 0.0;
     acc_im[r] = 0.0;
   }
 
-  for (int p = grp.p0; p < grp.p1; ++p) {
-    const DirectPair pr = pairs[p];
+  const int p1 = grp.p1;
+  for (int p = grp.p0; p < p1; ++p) {
+    const DirectPair& pr = pairs[p];
     const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
     const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
-    int64_t oa[kPerThread], ob[kPerThread];
+    int32_t cs_a[NR > 0 ? NR : 1], cs_b[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      cs_a[i] = pr.cstr_a[i];
+      cs_b[i] = pr.cstr_b[i];
+    }
+    uint32_t oa[kPerThread], ob[kPerThread];
 #pragma unroll
     for (int r = 0; r < kPerThread; ++r) {
-      int64_t a = pr.off_a, b = pr.off_b;
-      for (int i = 0; i < nr; ++i) {
-        a += (int64_t)coord[r][i] * pr.cstr_a[i];
-        b += (int64_t)coord[r][i] * pr.cstr_b[i];
+      uint32_t a = 0, b = 0;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        a += coord[r][i] * (uint32_t)cs_a[i];
+        b += coord[r][i] * (uint32_t)cs_b[i];
       }
       oa[r] = a;
       ob[r] = b;
     }
+    const double* Ap = A + (CPLX ? 2 : 1) * pr.off_a;
+    const double* Bp = B + (CPLX ? 2 : 1) * pr.off_b;
     double s_re[kPerThread], s_im[kPerThread];
 #pragma unroll
     for (int r = 0; r < kPerThread; ++r) {
@@ -95,20 +111,21 @@ __global__ void __launch_bounds__(kThreads)
       s_im[r] = 0.0;
     }
     const int32_t* ko = koffs + 2 * (int64_t)pr.kofs;
-    for (int k = 0; k < pr.nk; ++k) {
-      const int32_t ka = ko[2 * k], kb = ko[2 * k + 1];
+    const int nk = pr.nk;
+    for (int k = 0; k < nk; ++k) {
+      const uint32_t ka = (uint32_t)ko[2 * k], kb = (uint32_t)ko[2 * k + 1];
 #pragma unroll
       for (int r = 0; r < kPerThread; ++r) {
         if (!live[r]) continue;
         if (CPLX) {
-          const double2 x = reinterpret_cast<const double2*>(A)[oa[r] + ka];
-          const double2 y = reinterpret_cast<const double2*>(B)[ob[r] + kb];
+          const double2 x = reinterpret_cast<const double2*>(Ap)[oa[r] + ka];
+          const double2 y = reinterpret_cast<const double2*>(Bp)[ob[r] + kb];
           const double xi = conja ? -x.y : x.y;
           const double yi = conjb ? -y.y : y.y;
           s_re[r] = fma(x.x, y.x, fma(-xi, yi, s_re[r]));
           s_im[r] = fma(x.x, yi, fma(xi, y.x, s_im[r]));
         } else {
-          s_re[r] = fma(A[oa[r] + ka], B[ob[r] + kb], s_re[r]);
+          s_re[r] = fma(Ap[oa[r] + ka], Bp[ob[r] + kb], s_re[r]);
         }
       }
     }
@@ -126,7 +143,7 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
   for (int r = 0; r < kPerThread; ++r) {
     if (!live[r]) continue;
-    const int64_t o = grp.off_c + e0 + r * kThreads + threadIdx.x;
+    const int64_t o = grp.off_c + (int64_t)(e0 + r * kThreads + threadIdx.x);
     if (CPLX) {
       double2 out = make_double2(acc_re[r], acc_im[r]);
       if (use_beta) {
@@ -143,6 +160,42 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+template <int CPLX, int NR>
+hipError_t launch_nr(hipStream_t s, dim3 grid, const DirectPair* pairs, const DirectGroup* groups,
+                     const int32_t* koffs, const int64_t* prefix, int64_t ngroups, const void* A,
+                     const void* B, void* C, unsigned flags, double ar, double ai, double br,
+                     double bi) {
+  hipLaunchKernelGGL((direct_kernel<CPLX, NR>), grid, dim3(kThreads), 0, s, pairs, groups, koffs,
+                     prefix, ngroups, (const double*)A, (const double*)B, (double*)C, flags, ar,
+                     ai, br, bi);
+  return hipGetLastError();
+}
+
+template <int CPLX>
+hipError_t launch_c(hipStream_t s, int nr, dim3 grid, const DirectPair* pairs,
+                    const DirectGroup* groups, const int32_t* koffs, const int64_t* prefix,
+                    int64_t ngroups, const void* A, const void* B, void* C, unsigned flags,
+                    double ar, double ai, double br, double bi) {
+#define ITG_NR_CASE(n)                                                                        \
+  case n:                                                                                     \
+    return launch_nr<CPLX, n>(s, grid, pairs, groups, koffs, prefix, ngroups, A, B, C, flags, \
+                              ar, ai, br, bi);
+  switch (nr) {
+    ITG_NR_CASE(0)
+    ITG_NR_CASE(1)
+    ITG_NR_CASE(2)
+    ITG_NR_CASE(3)
+    ITG_NR_CASE(4)
+    ITG_NR_CASE(5)
+    ITG_NR_CASE(6)
+    ITG_NR_CASE(7)
+    ITG_NR_CASE(8)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef ITG_NR_CASE
+}
+
 }  // namespace
 
 hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pairs,
@@ -151,16 +204,12 @@ hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pair
                          const void* A, const void* B, void* C, unsigned flags, double ga_re,
                          double ga_im, double beta_re, double beta_im) {
   if (total_tiles == 0) return hipSuccess;
-  dim3 grid((unsigned)total_tiles), block(kThreads);
+  dim3 grid((unsigned)total_tiles);
   if (cplx)
-    hipLaunchKernelGGL(direct_kernel<1>, grid, block, 0, s, nr, pairs, groups, koffs,
-                       tile_prefix, ngroups, (const double*)A, (const double*)B, (double*)C,
-                       flags, ga_re, ga_im, beta_re, beta_im);
-  else
-    hipLaunchKernelGGL(direct_kernel<0>, grid, block, 0, s, nr, pairs, groups, koffs,
-                       tile_prefix, ngroups, (const double*)A, (const double*)B, (double*)C,
-                       flags, ga_re, ga_im, beta_re, beta_im);
-  return hipGetLastError();
+    return launch_c<1>(s, nr, grid, pairs, groups, koffs, tile_prefix, ngroups, A, B, C, flags,
+                       ga_re, ga_im, beta_re, beta_im);
+  return launch_c<0>(s, nr, grid, pairs, groups, koffs, tile_prefix, ngroups, A, B, C, flags,
+                     ga_re, ga_im, beta_re, beta_im);
 }
 
 }  // namespace itg

@@ -9,13 +9,19 @@
 //   C_g = beta*C_g (dense entry only) + sum_{p in g} alpha_p * op(A_p) op(B_p)
 //
 // so every output element has exactly one writer and a fixed summation
-// order (deterministic, no atomics). Groups are visited largest-first via the
-// `order` table so the long tiles start early (LPT load balance).
+// order (deterministic, no atomics).
+//
+// Scheduling: the host builds a tile table in which workgroup w runs on XCD
+// w % 8 (the hardware's round-robin dispatch). Each XCD receives whole groups
+// (or large chunks of a big group) so the A and B panels of a group are
+// fetched once into that XCD's L2 and shared by its tiles; chunks are spread
+// over the XCDs largest-first to balance the work.
 //
 // Tiles: 256 threads = 4 wave64s in a 2x2 grid; each wave owns a
 // (BM/2) x (BN/2) sub-tile as (BM/32) x (BN/32) MFMA 16x16 blocks of FP64
-// accumulators. A and B tiles (BK = 16) are staged in LDS; the next K-slice
-// is fetched into registers while the current one is multiplied. The MFMA is
+// accumulators. A and B K-slices (BK = 16) are double-buffered in LDS: the
+// next slice is fetched into registers while the current one is multiplied,
+// then written to the other buffer, with one barrier per slice. The MFMA is
 // issued as D = B^T A^T so that each lane's accumulators run down a column
 // of C, which makes the column-major epilogue stores contiguous.
 //
@@ -36,23 +42,10 @@ constexpr int kThreads = 256;
 constexpr int BK = 16;
 constexpr int PAD = 1;
 
-__device__ inline int64_t find_slot(const int64_t* prefix, int64_t n, int64_t wg) {
-  int64_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int64_t mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= wg)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
 template <int BM, int BN, int CPLX, int TA, int TB>
 __global__ void __launch_bounds__(kThreads)
     gemm_f64_kernel(const GemmPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
-                    const int32_t* __restrict__ order, const int64_t* __restrict__ prefix,
-                    int64_t ngroups, const double* __restrict__ Ag,
+                    const int2* __restrict__ tiles, const double* __restrict__ Ag,
                     const double* __restrict__ Bg, double* __restrict__ Cg, unsigned flags,
                     double ga_re, double ga_im, double beta_re, double beta_im) {
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -61,15 +54,14 @@ __global__ void __launch_bounds__(kThreads)
   constexpr int NB = BN * BK / kThreads;
   constexpr int NPLANE = CPLX ? 2 : 1;
 
-  __shared__ double As[NPLANE][BK][BM + PAD];
-  __shared__ double Bs[NPLANE][BK][BN + PAD];
+  __shared__ double As[2][NPLANE][BK][BM + PAD];
+  __shared__ double Bs[2][NPLANE][BK][BN + PAD];
 
-  const int64_t wg = blockIdx.x;
-  const int64_t slot = find_slot(prefix, ngroups, wg);
-  const GemmGroup grp = groups[order[slot]];
-  const int64_t t = wg - prefix[slot];
+  const int2 tt = tiles[blockIdx.x];
+  if (tt.x < 0) return;  // padding slot of the XCD-balanced table
+  const GemmGroup grp = groups[tt.x];
   const int tiles_m = (grp.M + BM - 1) / BM;
-  const int tm = (int)(t % tiles_m), tn = (int)(t / tiles_m);
+  const int tm = tt.y % tiles_m, tn = tt.y / tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
 
   const int tid = threadIdx.x;
@@ -102,14 +94,8 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
         const int idx = tid + r * kThreads;
-        int m, k;
-        if (TA) {
-          k = idx % BK;
-          m = idx / BK;
-        } else {
-          m = idx % BM;
-          k = idx / BM;
-        }
+        const int m = TA ? idx / BK : idx % BM;
+        const int k = TA ? idx % BK : idx / BM;
         const int gm = m0 + m, gk = k0 + k;
         const bool in = gm < M && gk < K;
         const int64_t off = TA ? ((int64_t)gm * pr.lda + gk) : ((int64_t)gk * pr.lda + gm);
@@ -126,14 +112,8 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
         const int idx = tid + r * kThreads;
-        int n, k;
-        if (TB) {
-          n = idx % BN;
-          k = idx / BN;
-        } else {
-          k = idx % BK;
-          n = idx / BK;
-        }
+        const int n = TB ? idx % BN : idx / BK;
+        const int k = TB ? idx / BN : idx % BK;
         const int gn = n0 + n, gk = k0 + k;
         const bool in = gn < N && gk < K;
         const int64_t off = TB ? ((int64_t)gk * pr.ldb + gn) : ((int64_t)gn * pr.ldb + gk);
@@ -147,53 +127,44 @@ __global__ void __launch_bounds__(kThreads)
         }
       }
     };
-    auto store_lds = [&]() {
+    auto store_lds = [&](int buf) {
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
         const int idx = tid + r * kThreads;
-        int m, k;
-        if (TA) {
-          k = idx % BK;
-          m = idx / BK;
-        } else {
-          m = idx % BM;
-          k = idx / BM;
-        }
+        const int m = TA ? idx / BK : idx % BM;
+        const int k = TA ? idx % BK : idx / BM;
 #pragma unroll
-        for (int c = 0; c < NPLANE; ++c) As[c][k][m] = ra[c][r];
+        for (int c = 0; c < NPLANE; ++c) As[buf][c][k][m] = ra[c][r];
       }
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
         const int idx = tid + r * kThreads;
-        int n, k;
-        if (TB) {
-          n = idx % BN;
-          k = idx / BN;
-        } else {
-          k = idx % BK;
-          n = idx / BK;
-        }
+        const int n = TB ? idx % BN : idx / BK;
+        const int k = TB ? idx / BN : idx % BK;
 #pragma unroll
-        for (int c = 0; c < NPLANE; ++c) Bs[c][k][n] = rb[c][r];
+        for (int c = 0; c < NPLANE; ++c) Bs[buf][c][k][n] = rb[c][r];
       }
     };
 
     if (K <= 0) continue;
+    // every wave is past the previous pair's last read of buffer 0 (the
+    // barrier that closes each K-slice), so it can be refilled here
     load_regs(0);
+    store_lds(0);
+    __syncthreads();
+    int buf = 0;
     for (int k0 = 0; k0 < K; k0 += BK) {
-      __syncthreads();
-      store_lds();
-      __syncthreads();
-      if (k0 + BK < K) load_regs(k0 + BK);
+      const bool more = k0 + BK < K;
+      if (more) load_regs(k0 + BK);
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         double af[NPLANE][FM], bf[NPLANE][FN];
 #pragma unroll
         for (int c = 0; c < NPLANE; ++c) {
 #pragma unroll
-          for (int i = 0; i < FM; ++i) af[c][i] = As[c][kk + lk][wm0 + i * 16 + l16];
+          for (int i = 0; i < FM; ++i) af[c][i] = As[buf][c][kk + lk][wm0 + i * 16 + l16];
 #pragma unroll
-          for (int j = 0; j < FN; ++j) bf[c][j] = Bs[c][kk + lk][wn0 + j * 16 + l16];
+          for (int j = 0; j < FN; ++j) bf[c][j] = Bs[buf][c][kk + lk][wn0 + j * 16 + l16];
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -215,6 +186,11 @@ __global__ void __launch_bounds__(kThreads)
             }
           }
       }
+      // the other buffer was last read in the previous slice, before the
+      // barrier that ended it
+      if (more) store_lds(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
     }
   }
 
@@ -258,22 +234,22 @@ constexpr CfgEntry kCfgs[] = {{64, 64}, {128, 128}, {128, 64}, {64, 128}};
 
 template <int BM, int BN, int CPLX>
 hipError_t launch_tt(hipStream_t s, int ta, int tb, dim3 grid, const GemmPair* pairs,
-                     const GemmGroup* groups, const int32_t* order, const int64_t* prefix,
-                     int64_t ngroups, const double* A, const double* B, double* C,
-                     unsigned flags, double ar, double ai, double br, double bi) {
+                     const GemmGroup* groups, const int2* tiles, const double* A,
+                     const double* B, double* C, unsigned flags, double ar, double ai, double br,
+                     double bi) {
   dim3 block(kThreads);
   if (!ta && !tb)
     hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 0, 0>), grid, block, 0, s, pairs, groups,
-                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+                       tiles, A, B, C, flags, ar, ai, br, bi);
   else if (!ta && tb)
     hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 0, 1>), grid, block, 0, s, pairs, groups,
-                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+                       tiles, A, B, C, flags, ar, ai, br, bi);
   else if (ta && !tb)
     hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 1, 0>), grid, block, 0, s, pairs, groups,
-                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+                       tiles, A, B, C, flags, ar, ai, br, bi);
   else
     hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 1, 1>), grid, block, 0, s, pairs, groups,
-                       order, prefix, ngroups, A, B, C, flags, ar, ai, br, bi);
+                       tiles, A, B, C, flags, ar, ai, br, bi);
   return hipGetLastError();
 }
 
@@ -281,41 +257,40 @@ hipError_t launch_tt(hipStream_t s, int ta, int tb, dim3 grid, const GemmPair* p
 
 int gemm_num_cfgs() { return (int)(sizeof(kCfgs) / sizeof(kCfgs[0])); }
 GemmTileCfg gemm_cfg(int cfg) { return GemmTileCfg{kCfgs[cfg].bm, kCfgs[cfg].bn}; }
+bool gemm_cfg_ok(int cfg, int cplx) { return !cplx || cfg == 0 || cfg == 2; }
 
 hipError_t launch_gemm_f64(hipStream_t s, int cfg, int cplx, int ta, int tb,
-                           const GemmPair* pairs, const GemmGroup* groups, const int32_t* order,
-                           const int64_t* tile_prefix, int64_t ngroups, int64_t total_tiles,
-                           const void* A, const void* B, void* C, unsigned flags,
-                           double ga_re, double ga_im, double beta_re, double beta_im) {
-  if (total_tiles == 0) return hipSuccess;
-  dim3 grid((unsigned)total_tiles);
+                           const GemmPair* pairs, const GemmGroup* groups, const int2* tiles,
+                           int64_t grid_size, const void* A, const void* B, void* C,
+                           unsigned flags, double ga_re, double ga_im, double beta_re,
+                           double beta_im) {
+  if (grid_size == 0) return hipSuccess;
+  if (!gemm_cfg_ok(cfg, cplx)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)grid_size);
   const double* a = (const double*)A;
   const double* b = (const double*)B;
   double* c = (double*)C;
   if (cplx) {
     // complex accumulators double the register tile: keep tiles <= 128x64
-    switch (cfg) {
-      case 2:
-        return launch_tt<128, 64, 1>(s, ta, tb, grid, pairs, groups, order, tile_prefix,
-                                     ngroups, a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
-      default:
-        return launch_tt<64, 64, 1>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
-                                    a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
-    }
+    if (cfg == 2)
+      return launch_tt<128, 64, 1>(s, ta, tb, grid, pairs, groups, tiles, a, b, c, flags, ga_re,
+                                   ga_im, beta_re, beta_im);
+    return launch_tt<64, 64, 1>(s, ta, tb, grid, pairs, groups, tiles, a, b, c, flags, ga_re,
+                                ga_im, beta_re, beta_im);
   }
   switch (cfg) {
     case 1:
-      return launch_tt<128, 128, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
-                                    a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+      return launch_tt<128, 128, 0>(s, ta, tb, grid, pairs, groups, tiles, a, b, c, flags, ga_re,
+                                    ga_im, beta_re, beta_im);
     case 2:
-      return launch_tt<128, 64, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
-                                   a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+      return launch_tt<128, 64, 0>(s, ta, tb, grid, pairs, groups, tiles, a, b, c, flags, ga_re,
+                                   ga_im, beta_re, beta_im);
     case 3:
-      return launch_tt<64, 128, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
-                                   a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+      return launch_tt<64, 128, 0>(s, ta, tb, grid, pairs, groups, tiles, a, b, c, flags, ga_re,
+                                   ga_im, beta_re, beta_im);
     default:
-      return launch_tt<64, 64, 0>(s, ta, tb, grid, pairs, groups, order, tile_prefix, ngroups,
-                                  a, b, c, flags, ga_re, ga_im, beta_re, beta_im);
+      return launch_tt<64, 64, 0>(s, ta, tb, grid, pairs, groups, tiles, a, b, c, flags, ga_re,
+                                  ga_im, beta_re, beta_im);
   }
 }
 

@@ -4,8 +4,8 @@
 namespace itg {
 
 hipError_t launch_gemm_split(hipStream_t, int, int, int, int, const GemmPair*, const GemmGroup*,
-                             const int32_t*, const int64_t*, int64_t, int64_t, const void*,
-                             const void*, void*, double) {
+                             const int2*, int64_t, const void*, const void*, void*, double,
+                             double) {
   return hipErrorNotSupported;
 }
 
