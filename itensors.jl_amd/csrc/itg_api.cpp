@@ -107,7 +107,7 @@ itg_status run_plan(itg_handle* h, ExecPlan& ep, int cplx, int prec, const void*
                     const double* beta) {
   const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
   const double b_re = beta ? beta[0] : 0.0, b_im = (beta && cplx) ? beta[1] : 0.0;
-  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx));
+  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx, prec));
   if (e != hipSuccess) return hip_fail(h, e, "workspace allocation");
   std::string err;
   const int cfg = pick_cfg(h, ep, cplx, prec);
@@ -587,7 +587,8 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   const int cplx = dtype == ITG_C128;
   DeviceGuard g(h->device);
   ExecPlan& ep = *p->ep;
-  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx));
+  if (prec != ITG_PREC_FP64) return ITG_OK;  // the split GEMM has one tile shape
+  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx, (int)prec));
   if (e != hipSuccess) return hip_fail(h, e, "workspace allocation");
   hipEvent_t t0, t1;
   if ((e = hipEventCreate(&t0)) != hipSuccess) return hip_fail(h, e, "event");

@@ -100,11 +100,38 @@ hipError_t launch_gemm_f64(hipStream_t s, int cfg, int cplx, int ta, int tb,
                            unsigned flags, double ga_re, double ga_im, double beta_re,
                            double beta_im);
 
-// Split-precision grouped GEMM (real Float64 only). prec: 1 = BF16 slices,
-// 2 = INT8 residues (CRT).
-hipError_t launch_gemm_split(hipStream_t s, int prec, int cfg, int ta, int tb,
-                             const GemmPair* pairs, const GemmGroup* groups, const int2* tiles,
-                             int64_t grid_size, const void* A, const void* B, void* C,
-                             double ga_re, double beta_re);
+// ------------------------------------------------ split-precision GEMM
+// FP64 operands are split into kSliceCount int8 slices per element with one
+// power-of-two scale per row (M row of A, N column of B); see kernels_split.hip.
+constexpr int kSliceModes = 4;  // row / K modes of a sliced operand after merging
+constexpr int kSliceCount = 7;  // int8 slices per FP64 value (7 bits each)
+
+// One block of an operand to slice: rows x K read through mode strides.
+struct SliceBlock {
+  int64_t off;        // element offset of the block in the FP64 operand
+  int64_t plane_off;  // byte offset of slice plane 0 (planes are R x Kpad, row-major)
+  int64_t exp_off;    // index of the block's first row exponent
+  int32_t R, K, Kpad;
+  int32_t nrm, nkm;  // number of row / K modes
+  int32_t rdim[kSliceModes], rstr[kSliceModes];
+  int32_t kdim[kSliceModes], kstr[kSliceModes];
+};
+struct SplitPair {
+  int64_t a_planes, b_planes;    // byte offsets of slice plane 0
+  int64_t a_pstride, b_pstride;  // bytes between consecutive slice planes
+  int32_t a_exp, b_exp;          // first row exponent of the A block / B block
+  int32_t kpad, pad;
+  double alpha;
+};
+
+int split_slices(int prec);  // slices per value for a precision mode (0: none)
+int64_t slice_block_wgs(int32_t rows);
+hipError_t launch_slice(hipStream_t s, int prec, const SliceBlock* blocks, const int64_t* prefix,
+                        int64_t nblk, int64_t grid, const void* src, void* planes, void* exps);
+// Tile shape of the split GEMM: 64 x 64 (the tile table uses gemm cfg 0).
+hipError_t launch_split_gemm(hipStream_t s, int prec, const SplitPair* pairs,
+                             const GemmGroup* groups, const int2* tiles, int64_t grid,
+                             const void* planesA, const void* planesB, const void* expA,
+                             const void* expB, void* C, double ga, double beta);
 
 }  // namespace itg

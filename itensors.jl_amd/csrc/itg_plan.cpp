@@ -505,6 +505,113 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
       (e = ep->g_groups.upload(ep->h_groups.data(), ep->h_groups.size() * sizeof(GemmGroup),
                                s)) != hipSuccess)
     return std::string("device table upload failed: ") + hipGetErrorString(e);
+  std::string serr = build_split_tables(ep, P, Q, Mp, Nq, ch.Xp, ch.Xq, pairs, swap, s);
+  if (!serr.empty()) return serr;
+  return "";
+}
+
+namespace {
+
+// Mode list (positions in a block, in order, first fastest) -> merged
+// (dims, strides) of a column-major block. Adjacent positions merge.
+bool slice_modes(const std::vector<int>& pos, const int32_t* dims, int N, int32_t* mdim,
+                 int32_t* mstr, int32_t* nm, int64_t* total) {
+  int64_t st[kMaxModes];
+  int64_t a = 1;
+  for (int i = 0; i < N; ++i) {
+    st[i] = a;
+    a *= dims[i];
+  }
+  int n = 0;
+  int64_t tot = 1;
+  for (size_t t = 0; t < pos.size(); ++t) {
+    const int p = pos[t];
+    tot *= dims[p];
+    if (n > 0 && t > 0 && pos[t - 1] + 1 == p) {
+      mdim[n - 1] *= dims[p];
+      continue;
+    }
+    if (n == kSliceModes) return false;
+    mdim[n] = dims[p];
+    mstr[n] = (int32_t)st[p];
+    ++n;
+  }
+  *nm = n;
+  *total = tot;
+  return true;
+}
+
+}  // namespace
+
+std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList& Q,
+                               const std::vector<int>& Mp, const std::vector<int>& Nq,
+                               const std::vector<int>& Xp, const std::vector<int>& Xq,
+                               const std::vector<PairSpec>& pairs, bool swap, hipStream_t s) {
+  ep->split_ok = false;
+  auto side = [&](const BlockList& T, const std::vector<int>& rows, const std::vector<int>& ks,
+                  std::vector<SliceBlock>* out, std::vector<int64_t>* prefix, int64_t* plane_bytes,
+                  int64_t* nrows) -> std::string {
+    out->resize(T.nblk());
+    prefix->assign(T.nblk() + 1, 0);
+    int64_t pb = 0, er = 0;
+    for (int64_t b = 0; b < T.nblk(); ++b) {
+      SliceBlock& d = (*out)[b];
+      std::memset(&d, 0, sizeof(d));
+      int64_t R = 1, K = 1;
+      for (int i = 0; i < kSliceModes; ++i) d.rdim[i] = d.kdim[i] = 1;
+      if (!slice_modes(rows, T.d(b), T.N, d.rdim, d.rstr, &d.nrm, &R) ||
+          !slice_modes(ks, T.d(b), T.N, d.kdim, d.kstr, &d.nkm, &K))
+        return "too many modes for the split-precision operand reader";
+      if (K > 65536) return "contracted extent above 65536 per block pair";
+      d.off = T.offs[b];
+      d.R = (int32_t)R;
+      d.K = (int32_t)K;
+      d.Kpad = (int32_t)((K + 31) / 32 * 32);
+      d.plane_off = pb;
+      d.exp_off = er;
+      pb += (int64_t)kSliceCount * R * d.Kpad;
+      er += R;
+      (*prefix)[b + 1] = (*prefix)[b] + slice_block_wgs(d.R);
+    }
+    *plane_bytes = pb;
+    *nrows = er;
+    return "";
+  };
+  std::vector<SliceBlock> dA, dB;
+  std::vector<int64_t> pA, pB;
+  std::string err = side(P, Mp, Xp, &dA, &pA, &ep->sA_plane_bytes, &ep->sA_rows);
+  if (err.empty()) err = side(Q, Nq, Xq, &dB, &pB, &ep->sB_plane_bytes, &ep->sB_rows);
+  if (!err.empty()) {
+    ep->split_why = err;
+    return "";  // the FP64 path still works
+  }
+  std::vector<SplitPair> sp(pairs.size());
+  for (size_t p = 0; p < pairs.size(); ++p) {
+    const int64_t bp = swap ? pairs[p].blk2 : pairs[p].blk1;
+    const int64_t bq = swap ? pairs[p].blk1 : pairs[p].blk2;
+    SplitPair& x = sp[p];
+    std::memset(&x, 0, sizeof(x));
+    x.a_planes = dA[bp].plane_off;
+    x.b_planes = dB[bq].plane_off;
+    x.a_pstride = (int64_t)dA[bp].R * dA[bp].Kpad;
+    x.b_pstride = (int64_t)dB[bq].R * dB[bq].Kpad;
+    x.a_exp = (int32_t)dA[bp].exp_off;
+    x.b_exp = (int32_t)dB[bq].exp_off;
+    x.kpad = dA[bp].K == 0 ? 0 : dA[bp].Kpad;
+    x.alpha = ep->h_pairs[p].alpha_re;
+  }
+  hipError_t e;
+  if ((e = ep->sA_desc.upload(dA.data(), dA.size() * sizeof(SliceBlock), s)) != hipSuccess ||
+      (e = ep->sA_prefix.upload(pA.data(), pA.size() * sizeof(int64_t), s)) != hipSuccess ||
+      (e = ep->sB_desc.upload(dB.data(), dB.size() * sizeof(SliceBlock), s)) != hipSuccess ||
+      (e = ep->sB_prefix.upload(pB.data(), pB.size() * sizeof(int64_t), s)) != hipSuccess ||
+      (e = ep->s_pairs.upload(sp.data(), sp.size() * sizeof(SplitPair), s)) != hipSuccess)
+    return std::string("device table upload failed: ") + hipGetErrorString(e);
+  ep->sA_blocks = P.nblk();
+  ep->sB_blocks = Q.nblk();
+  ep->sA_grid = pA.back();
+  ep->sB_grid = pB.back();
+  ep->split_ok = true;
   return "";
 }
 
@@ -582,15 +689,81 @@ std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc) {
   return table;
 }
 
-size_t exec_workspace_bytes(const ExecPlan& ep, int cplx) {
+namespace {
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec) {
   if (ep.strategy != ExecPlan::GEMM) return 0;
   const size_t es = cplx ? 16 : 8;
   size_t b = 0;
-  if (ep.permA) b += (size_t)ep.nnzA * es;
-  if (ep.permB) b += (size_t)ep.nnzB * es;
-  if (ep.permC) b += (size_t)ep.nnzC * es;
+  if (prec != 0 && ep.split_ok && !cplx) {
+    b += align256((size_t)ep.sA_plane_bytes) + align256((size_t)ep.sB_plane_bytes);
+    b += align256((size_t)ep.sA_rows * 4) + align256((size_t)ep.sB_rows * 4);
+  } else {
+    if (ep.permA) b += align256((size_t)ep.nnzA * es);
+    if (ep.permB) b += align256((size_t)ep.nnzB * es);
+  }
+  if (ep.permC) b += align256((size_t)ep.nnzC * es);
   return b + 1024;
 }
+
+namespace {
+
+hipError_t exec_split(ExecPlan& ep, int prec, hipStream_t s, void* ws, const void* T1,
+                      const void* T2, void* R, double a_re, double b_re, std::string* err) {
+  if (!ep.split_ok) {
+    *err = "split precision is unavailable for this contraction: " + ep.split_why;
+    return hipErrorNotSupported;
+  }
+  if (split_slices(prec) == 0) {
+    *err = "this split-precision form is not available";
+    return hipErrorNotSupported;
+  }
+  const void* A = ep.swap ? T2 : T1;
+  const void* B = ep.swap ? T1 : T2;
+  char* w = (char*)ws;
+  void* planesA = w;
+  w += align256((size_t)ep.sA_plane_bytes);
+  void* planesB = w;
+  w += align256((size_t)ep.sB_plane_bytes);
+  void* expA = w;
+  w += align256((size_t)ep.sA_rows * 4);
+  void* expB = w;
+  w += align256((size_t)ep.sB_rows * 4);
+  hipError_t e = launch_slice(s, prec, (const SliceBlock*)ep.sA_desc.p,
+                              (const int64_t*)ep.sA_prefix.p, ep.sA_blocks, ep.sA_grid, A,
+                              planesA, expA);
+  if (e == hipSuccess)
+    e = launch_slice(s, prec, (const SliceBlock*)ep.sB_desc.p, (const int64_t*)ep.sB_prefix.p,
+                     ep.sB_blocks, ep.sB_grid, B, planesB, expB);
+  if (e != hipSuccess) {
+    *err = std::string("slice launch: ") + hipGetErrorString(e);
+    return e;
+  }
+  if ((e = ensure_gemm_launch(ep, 0, 0, s)) != hipSuccess) {
+    *err = std::string("gemm table upload: ") + hipGetErrorString(e);
+    return e;
+  }
+  GemmLaunch& gl = *ep.launches[0];
+  void* C = ep.permC ? (void*)w : R;
+  e = launch_split_gemm(s, prec, (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.g_groups.p,
+                        (const int2*)gl.tiles.p, gl.grid, planesA, planesB, expA, expB, C, a_re,
+                        ep.permC ? 0.0 : b_re);
+  if (e != hipSuccess) {
+    *err = std::string("split gemm launch: ") + hipGetErrorString(e);
+    return e;
+  }
+  if (ep.permC) {
+    e = launch_permute(s, 0, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,
+                       (const int64_t*)ep.pC.prefix.p, ep.pC.nblk, ep.pC.tiles, ep.pC.kind, C, R,
+                       1.0, 0.0, b_re, 0.0);
+    if (e != hipSuccess) *err = std::string("permute launch: ") + hipGetErrorString(e);
+  }
+  return e;
+}
+
+}  // namespace
 
 hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                     const void* T1, const void* T2, void* R, unsigned flags, double a_re,
@@ -603,6 +776,13 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
                       a_re, a_im, b_re, b_im);
     if (e != hipSuccess) *err = std::string("direct kernel launch: ") + hipGetErrorString(e);
     return e;
+  }
+  if (prec != 0) {
+    if (cplx) {
+      *err = "split precision is implemented for Float64 only";
+      return hipErrorNotSupported;
+    }
+    return exec_split(ep, prec, s, ws, T1, T2, R, a_re, b_re, err);
   }
   const size_t es = cplx ? 16 : 8;
   char* w = (char*)ws;
@@ -644,22 +824,9 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
   }
   int ecfg = gemm_cfg_ok(cfg, cplx) ? cfg : 0;
   GemmLaunch& gl = *ep.launches[ecfg];
-  if (prec == 0) {
-    e = launch_gemm_f64(s, ecfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
-                        (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B,
-                        C, gflags, a_re, a_im, gbr, gbi);
-  } else if (cplx) {
-    *err = "split precision is implemented for Float64 only";
-    return hipErrorNotSupported;
-  } else {
-    e = launch_gemm_split(s, prec, ecfg, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
-                          (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B,
-                          C, a_re, gbr);
-    if (e == hipErrorNotSupported) {
-      *err = "this split-precision form is not available";
-      return e;
-    }
-  }
+  e = launch_gemm_f64(s, ecfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
+                      (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B, C,
+                      gflags, a_re, a_im, gbr, gbi);
   if (e != hipSuccess) {
     *err = std::string("gemm launch: ") + hipGetErrorString(e);
     return e;

@@ -98,6 +98,13 @@ struct ExecPlan {
   std::map<int, std::unique_ptr<GemmLaunch>> launches;  // per tile cfg
   double flops = 0.0;                                   // real flops (2*M*N*K summed)
   int64_t max_dim = 0;
+  // split precision (real data): slice tables of the A-side operand P and
+  // the B-side operand Q, read in place (no permutation), and the pair table
+  bool split_ok = false;
+  std::string split_why;  // reason when !split_ok
+  int64_t sA_blocks = 0, sA_grid = 0, sB_blocks = 0, sB_grid = 0;
+  int64_t sA_plane_bytes = 0, sB_plane_bytes = 0, sA_rows = 0, sB_rows = 0;
+  DevBuf sA_desc, sA_prefix, sB_desc, sB_prefix, s_pairs;
 };
 
 // Build the execution plan of `pairs` (grouped: all pairs of one output block
@@ -107,13 +114,19 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
                             const BlockList& bR, const std::vector<PairSpec>& pairs,
                             const double* pair_alpha, hipStream_t s, ExecPlan* ep);
 
+// Slice and pair tables of the split-precision GEMM (sets split_ok / split_why).
+std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList& Q,
+                               const std::vector<int>& Mp, const std::vector<int>& Nq,
+                               const std::vector<int>& Xp, const std::vector<int>& Xq,
+                               const std::vector<PairSpec>& pairs, bool swap, hipStream_t s);
+
 // Build the device tables of a grouped permutation (dst offsets default to
 // the src offsets). perm: new mode i = old mode perm[i], 0-based.
 hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
                                 const std::vector<int64_t>* dst_offs, hipStream_t s);
 
 // Launch the plan. ws: workspace of at least exec_workspace_bytes().
-size_t exec_workspace_bytes(const ExecPlan& ep, int cplx);
+size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec);
 hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                     const void* T1, const void* T2, void* R, unsigned flags, double a_re,
                     double a_im, double b_re, double b_im, std::string* err);

@@ -1,12 +1,308 @@
-// Split-precision grouped GEMM (placeholder until the sliced kernels land).
+// Split-precision grouped GEMM for gfx950: FP64 operands emulated with
+// integer slices on the INT8 matrix cores (Ozaki-style error-free splitting).
+//
+// 1. slice_kernel, once per operand: for every row r of every block (rows =
+//    the GEMM's M modes for A, N modes for B; the K modes are the columns),
+//    E_r = the smallest integer with max_k |x_rk| < 2^E_r, and
+//        x_rk = 2^(E_r-6) * sum_{i=1..S} v_i,rk * 2^(-7(i-1)) + residue,
+//    v_i in [-64, 64] (int8), |residue| <= 2^(E_r - 7S). The slices come
+//    from exact FP64 steps (y = x*2^(6-E); v = rint(y); y = (y-v)*128).
+//    The operand is read in place through its mode strides (no permutation
+//    pass) and each slice plane is written row-major with K padded to a
+//    multiple of 32 (zeros), so the GEMM reads 16-byte K chunks unchecked.
+// 2. split_gemm_kernel: C = sum_{pairs} alpha_p * A_p B_p, where for every
+//    pair the products v^A_i v^B_j with i + j <= S + 1 run on
+//    v_mfma_i32_32x32x32_i8, summed per level L = i + j in INT32 -- exact,
+//    since |sum| <= S * 64*64 * K < 2^31 for K <= 65536 (checked by the
+//    planner) -- and recombined in FP64 after the pair's K loop:
+//        C_mn += alpha * 2^(EA_m+EB_n-12) * sum_L 2^(-7(L-2)) S_L,mn.
+//    The error is that of representing A and B with 7S bits per row/column
+//    scale (about 2^-7S relative to the row maximum); no rounding occurs in
+//    the products or the INT32 sums.
+// Tiles: 256 threads = 4 wave64s (2x2), 64x64 per workgroup, 32x32 per wave;
+// K steps of 32 double-buffered in LDS (slice planes, 16-byte chunks with a
+// bank swizzle). The MFMA is issued as D = B^T A^T so that each lane's
+// accumulators run down a column of C (contiguous column-major stores).
 #include "itg_internal.h"
 
 namespace itg {
 
-hipError_t launch_gemm_split(hipStream_t, int, int, int, int, const GemmPair*, const GemmGroup*,
-                             const int2*, int64_t, const void*, const void*, void*, double,
-                             double) {
-  return hipErrorNotSupported;
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kThreads = 256;
+constexpr int kRowsPerWG = 64;  // slice kernel rows per workgroup
+constexpr int kNonFinite = 0x7fff;
+
+__device__ inline int64_t find_slot(const int64_t* prefix, int64_t n, int64_t wg) {
+  int64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= wg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline int64_t mode_offset(uint32_t idx, int n, const int32_t* dim,
+                                      const int32_t* str) {
+  int64_t o = 0;
+#pragma unroll
+  for (int i = 0; i < kSliceModes; ++i) {
+    if (i >= n) break;
+    if (i + 1 < n) {
+      const uint32_t q = idx / (uint32_t)dim[i];
+      o += (int64_t)(idx - q * (uint32_t)dim[i]) * str[i];
+      idx = q;
+    } else {
+      o += (int64_t)idx * str[i];
+    }
+  }
+  return o;
+}
+
+template <int S>
+__global__ void __launch_bounds__(kThreads)
+    slice_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
+                 int64_t nblk, const double* __restrict__ src, int8_t* __restrict__ planes,
+                 int32_t* __restrict__ exps) {
+  __shared__ double red[4][kRowsPerWG];
+  __shared__ int32_t sexp[kRowsPerWG];
+  const int64_t wg = blockIdx.x;
+  const int64_t b = find_slot(prefix, nblk, wg);
+  const SliceBlock& d = blocks[b];
+  const int r0 = (int)(wg - prefix[b]) * kRowsPerWG;
+  const int tid = threadIdx.x;
+  const double* x = src + d.off;
+
+  // phase 1: per-row max |x| over K (4 threads per row)
+  {
+    const int row = tid % kRowsPerWG, part = tid / kRowsPerWG;
+    double m = 0.0;
+    if (r0 + row < d.R) {
+      const int64_t ro = mode_offset((uint32_t)(r0 + row), d.nrm, d.rdim, d.rstr);
+      for (int k = part; k < d.K; k += 4) {
+        const double v = x[ro + mode_offset((uint32_t)k, d.nkm, d.kdim, d.kstr)];
+        m = fmax(m, fabs(v));
+        if (!isfinite(v)) m = INFINITY;
+      }
+    }
+    red[part][row] = m;
+  }
+  __syncthreads();
+  if (tid < kRowsPerWG) {
+    const double m = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+    int e = 0;
+    if (!isfinite(m))
+      e = kNonFinite;
+    else if (m > 0.0)
+      (void)frexp(m, &e);  // m = f*2^e, f in [0.5, 1): m < 2^e
+    sexp[tid] = e;
+    if (r0 + tid < d.R) exps[d.exp_off + r0 + tid] = e;
+  }
+  __syncthreads();
+
+  // phase 2: slice 16 consecutive k of one row per item
+  const int nchunk = d.Kpad / 16;
+  const int items = kRowsPerWG * nchunk;
+  const int64_t pstride = (int64_t)d.R * d.Kpad;
+  for (int it = tid; it < items; it += kThreads) {
+    const int row = it % kRowsPerWG, ch = it / kRowsPerWG;
+    const int gr = r0 + row;
+    if (gr >= d.R) continue;
+    const int e = sexp[row];
+    const int64_t ro = mode_offset((uint32_t)gr, d.nrm, d.rdim, d.rstr);
+    uint32_t packed[S][4];
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) packed[i][w] = 0u;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int k = ch * 16 + j;
+      double y = 0.0;
+      if (k < d.K && e != kNonFinite)
+        y = ldexp(x[ro + mode_offset((uint32_t)k, d.nkm, d.kdim, d.kstr)], 6 - e);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const double v = rint(y);
+        y = (y - v) * 128.0;
+        packed[i][j >> 2] |= ((uint32_t)((int32_t)v & 0xff)) << (8 * (j & 3));
+      }
+    }
+    int8_t* dst = planes + d.plane_off + (int64_t)gr * d.Kpad + ch * 16;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+      *reinterpret_cast<v4i*>(dst + i * pstride) =
+          v4i{(int)packed[i][0], (int)packed[i][1], (int)packed[i][2], (int)packed[i][3]};
+  }
+}
+
+constexpr int BM = 64, BN = 64, BKB = 32;  // BKB: K bytes per LDS stage
+
+template <int S>
+__global__ void __launch_bounds__(kThreads)
+    split_gemm_kernel(const SplitPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
+                      const int2* __restrict__ tiles, const int8_t* __restrict__ planesA,
+                      const int8_t* __restrict__ planesB, const int32_t* __restrict__ expA,
+                      const int32_t* __restrict__ expB, double* __restrict__ Cg, double ga,
+                      double beta) {
+  constexpr int NL = S;  // levels 2..S+1
+  constexpr int CH_A = S * BM * 2;  // 16-byte chunks per stage
+  constexpr int CH_B = S * BN * 2;
+  __shared__ v4i As[2][S][BM][2];
+  __shared__ v4i Bs[2][S][BN][2];
+
+  const int2 tt = tiles[blockIdx.x];
+  if (tt.x < 0) return;
+  const GemmGroup grp = groups[tt.x];
+  const int tiles_m = (grp.M + BM - 1) / BM;
+  const int tm = tt.y % tiles_m, tn = tt.y / tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+  const int l32 = lane & 31, lh = lane >> 5;
+
+  // this lane's output rows/cols: m = m0+wm0+l32 ; n = n0+wn0+(r%4)+8*(r/4)+4*lh
+  const int gm = m0 + wm0 + l32;
+  double acc[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+
+  for (int p = grp.p0; p < grp.p1; ++p) {
+    const SplitPair pr = pairs[p];
+    const int kpad = pr.kpad;
+    if (kpad <= 0) continue;
+    const int8_t* pa = planesA + pr.a_planes;
+    const int8_t* pb = planesB + pr.b_planes;
+    v16i lev[NL];
+#pragma unroll
+    for (int L = 0; L < NL; ++L) lev[L] = v16i{0};
+
+    v4i ra[(CH_A + kThreads - 1) / kThreads], rb[(CH_B + kThreads - 1) / kThreads];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < (CH_A + kThreads - 1) / kThreads; ++q) {
+        const int c = tid + q * kThreads;
+        if (c < CH_A) {
+          const int half = c & 1, row = (c >> 1) % BM, pl = c / (2 * BM);
+          const int g = m0 + row;
+          ra[q] = (g < grp.M) ? *reinterpret_cast<const v4i*>(pa + pl * pr.a_pstride +
+                                                              (int64_t)g * kpad + k0 + half * 16)
+                              : v4i{0, 0, 0, 0};
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < (CH_B + kThreads - 1) / kThreads; ++q) {
+        const int c = tid + q * kThreads;
+        if (c < CH_B) {
+          const int half = c & 1, row = (c >> 1) % BN, pl = c / (2 * BN);
+          const int g = n0 + row;
+          rb[q] = (g < grp.N) ? *reinterpret_cast<const v4i*>(pb + pl * pr.b_pstride +
+                                                              (int64_t)g * kpad + k0 + half * 16)
+                              : v4i{0, 0, 0, 0};
+        }
+      }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < (CH_A + kThreads - 1) / kThreads; ++q) {
+        const int c = tid + q * kThreads;
+        if (c < CH_A) {
+          const int half = c & 1, row = (c >> 1) % BM, pl = c / (2 * BM);
+          As[buf][pl][row][half ^ ((row >> 3) & 1)] = ra[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < (CH_B + kThreads - 1) / kThreads; ++q) {
+        const int c = tid + q * kThreads;
+        if (c < CH_B) {
+          const int half = c & 1, row = (c >> 1) % BN, pl = c / (2 * BN);
+          Bs[buf][pl][row][half ^ ((row >> 3) & 1)] = rb[q];
+        }
+      }
+    };
+
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < kpad; k0 += BKB) {
+      const bool more = k0 + BKB < kpad;
+      if (more) load(k0 + BKB);
+      const int ra_row = wm0 + l32, rb_row = wn0 + l32;
+      v4i av[S], bv[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        av[i] = As[buf][i][ra_row][lh ^ ((ra_row >> 3) & 1)];
+        bv[i] = Bs[buf][i][rb_row][lh ^ ((rb_row >> 3) & 1)];
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int j = 0; j < S - i; ++j)  // slice i of A with slice j of B, level i+j
+          lev[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av[i], lev[i + j], 0, 0, 0);
+      if (more) store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+
+    // FP64 recombination of the levels, scaled per row/column
+    const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
+      const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
+      double h = (double)lev[NL - 1][r];
+#pragma unroll
+      for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[L][r]);
+      double v = ldexp(h, ea + eb - 12);
+      if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
+      acc[r] = fma(ga * pr.alpha, v, acc[r]);
+    }
+  }
+
+  if (gm >= grp.M) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
+    if (gn >= grp.N) continue;
+    const int64_t o = grp.off_c + gm + (int64_t)gn * grp.ldc;
+    double out = acc[r];
+    if (beta != 0.0) out = fma(beta, Cg[o], out);
+    Cg[o] = out;
+  }
+}
+
+}  // namespace
+
+int split_slices(int prec) { return prec == 2 ? kSliceCount : 0; }
+
+hipError_t launch_slice(hipStream_t s, int prec, const SliceBlock* blocks, const int64_t* prefix,
+                        int64_t nblk, int64_t grid, const void* src, void* planes, void* exps) {
+  if (grid == 0) return hipSuccess;
+  if (prec != 2) return hipErrorNotSupported;
+  hipLaunchKernelGGL(slice_kernel<kSliceCount>, dim3((unsigned)grid), dim3(kThreads), 0, s, blocks,
+                     prefix, nblk, (const double*)src, (int8_t*)planes, (int32_t*)exps);
+  return hipGetLastError();
+}
+
+int64_t slice_block_wgs(int32_t rows) { return (rows + kRowsPerWG - 1) / kRowsPerWG; }
+
+hipError_t launch_split_gemm(hipStream_t s, int prec, const SplitPair* pairs,
+                             const GemmGroup* groups, const int2* tiles, int64_t grid,
+                             const void* planesA, const void* planesB, const void* expA,
+                             const void* expB, void* C, double ga, double beta) {
+  if (grid == 0) return hipSuccess;
+  if (prec != 2) return hipErrorNotSupported;
+  hipLaunchKernelGGL(split_gemm_kernel<kSliceCount>, dim3((unsigned)grid), dim3(kThreads), 0, s,
+                     pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
+                     (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta);
+  return hipGetLastError();
 }
 
 }  // namespace itg

@@ -1,0 +1,154 @@
+"""Split-precision contraction (FP64 operands as int8 slices on the INT8
+matrix cores, FP64 recombination) vs the oracle.
+
+Bar: relative Frobenius error <= 1e-12 against the extended-precision oracle,
+for dense products in every operand orientation, QN block-sparse steps and
+the full effective-Hamiltonian network. Rows with very different magnitudes
+are exercised because the scale is per row / per column.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from helpers import labels_for, rand, relerr
+
+pytestmark = pytest.mark.gpu
+SPLIT_RTOL = 1e-12
+PREC = "split_int8"
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    import itg
+    from itg import ndtensors as ND
+    from oracle import ndtensors_oracle as O
+    torch.cuda.init()
+    return torch, itg, ND, O
+
+
+def run(env, A, lA, B, lB, lR, alpha=1.0, beta=0.0, R0=None):
+    torch, itg, ND, O = env
+    TA, TB = ND.DenseTensor.from_numpy(A), ND.DenseTensor.from_numpy(B)
+    if R0 is None:
+        T = ND.contract(TA, lA, TB, lB, lR, alpha=alpha, prec=PREC)
+    else:
+        T = ND.DenseTensor.from_numpy(R0)
+        ND.contract_(T, lR, TA, lA, TB, lB, alpha, beta, prec=PREC)
+    torch.cuda.synchronize()
+    return T.to_numpy()
+
+
+def test_split_matrix_orientations(env):
+    _, _, _, O = env
+    rng = np.random.default_rng(11)
+    for (M, N, K) in [(37, 53, 70), (130, 65, 300), (1, 40, 64), (64, 64, 1000)]:
+        for ta, tb, tc in itertools.product([0, 1], repeat=3):
+            A = rand(rng, (K, M) if ta else (M, K))
+            B = rand(rng, (N, K) if tb else (K, N))
+            lA = (-1, 1) if ta else (1, -1)
+            lB = (2, -1) if tb else (-1, 2)
+            lR = (2, 1) if tc else (1, 2)
+            got = run(env, A, lA, B, lB, lR)
+            ref = O.dense_contract(None, lR, A, lA, B, lB, exact=True)
+            err = relerr(got, ref)
+            assert err < SPLIT_RTOL, (err, M, N, K, ta, tb, tc)
+
+
+def test_split_permutation_matrix(env):
+    _, _, _, O = env
+    rng = np.random.default_rng(12)
+    for pa, pb in itertools.product(range(3), range(3)):
+        la, lb = labels_for(3, 3, [(pa, pb)])
+        shA = [9, 7, 11]
+        shB = [6, 8, 5]
+        shA[pa] = shB[pb] = 40
+        A = rand(rng, tuple(shA))
+        B = rand(rng, tuple(shB))
+        free = [x for x in la if x > 0] + [x for x in lb if x > 0]
+        for lR in itertools.permutations(free):
+            got = run(env, A, la, B, lb, lR)
+            ref = O.dense_contract(None, lR, A, la, B, lb, exact=True)
+            err = relerr(got, ref)
+            assert err < SPLIT_RTOL, (err, pa, pb, lR)
+
+
+def test_split_row_scales_zero_rows_alpha_beta(env):
+    _, _, _, O = env
+    rng = np.random.default_rng(13)
+    A = rand(rng, (50, 96)) * np.exp2(rng.integers(-40, 40, size=(50, 1)))
+    B = rand(rng, (96, 30)) * np.exp2(rng.integers(-40, 40, size=(1, 30)))
+    A[7, :] = 0.0
+    B[:, 3] = 0.0
+    R1 = rand(rng, (50, 30))
+    got = run(env, A, (1, -1), B, (-1, 2), (1, 2), alpha=-1.5, beta=0.25, R0=R1)
+    ref = O.dense_contract(R1, (1, 2), A, (1, -1), B, (-1, 2), -1.5, 0.25, exact=True)
+    # elementwise relative to the row/column scale product
+    scale = np.abs(A).max(1, keepdims=True) * np.abs(B).max(0, keepdims=True) * 96 + np.abs(R1)
+    assert np.all(np.abs(got - ref) <= 1e-12 * scale + 1e-300)
+    assert np.all(got[7, :] == 0.25 * R1[7, :])
+    err = relerr(got, ref)
+    assert err < SPLIT_RTOL, err
+
+
+def test_split_nonfinite_propagates(env):
+    rng = np.random.default_rng(14)
+    A = rand(rng, (20, 64))
+    B = rand(rng, (64, 20))
+    A[3, 5] = np.nan
+    got = run(env, A, (1, -1), B, (-1, 2), (1, 2))
+    assert np.all(np.isnan(got[3, :]))
+    assert np.all(np.isfinite(np.delete(got, 3, axis=0)))
+
+
+def test_split_blocksparse_network_chi512(env):
+    torch, itg, ND, O = env
+    from itg.configs import heff_network
+    L, psi, W1, W2, R, ix = heff_network(512, qns=True, seed=0)
+    exact = L
+    for T in (psi, W1, W2, R):
+        exact = itg.contract(exact, T, prec="fp64")
+    cur = L
+    for T in (psi, W1, W2, R):
+        cur = itg.contract(cur, T, prec=PREC)
+    torch.cuda.synchronize()
+    assert list(cur.tensor.blockoffsets) == list(exact.tensor.blockoffsets)
+    err = relerr(cur.tensor.data.cpu().numpy(), exact.tensor.data.cpu().numpy())
+    assert err < SPLIT_RTOL, err
+
+
+def test_split_heff_chi2048_vs_oracle(env):
+    """The config-3 network (chi = 2048) in split precision against the numpy
+    oracle, overall and per output block."""
+    torch, itg, ND, O = env
+    from collections import OrderedDict
+    from itg.configs import heff_network
+    L, psi, W1, W2, R, ix = heff_network(2048, qns=True, seed=0)
+    cur = L
+    for T in (psi, W1, W2, R):
+        cur = itg.contract(cur, T, prec=PREC)
+    torch.cuda.synchronize()
+
+    def to_oracle(T):
+        host = T.tensor.data.detach().cpu().numpy()
+        offs = OrderedDict((tuple(c - 1 for c in b), o) for b, o in T.tensor.blockoffsets.items())
+        inds = []
+        for i in T.inds:
+            space = [(O.QN(*[(n, v, m) for n, (v, m) in q.vals.items()]), d) for q, d in i.space]
+            inds.append(O.Index(space, tags=i.tags, plev=i.plev, dir=i.dir, id=i.id))
+        return O.BlockSparse(host, offs, tuple(inds))
+
+    oc = to_oracle(L)
+    for T in (psi, W1, W2, R):
+        OT = to_oracle(T)
+        lA, lB = O.compute_contraction_labels(oc.inds, OT.inds)
+        oc, _ = O.blocksparse_contract(oc, lA, OT, lB, O.contract_labels(lA, lB))
+    x = cur.tensor.data.cpu().numpy()
+    err = relerr(x, oc.data)
+    assert err < SPLIT_RTOL, err
+    worst = 0.0
+    for b, o in oc.offsets.items():
+        n = oc.block(b).size
+        worst = max(worst, relerr(x[o:o + n], oc.data[o:o + n]))
+    assert worst < 1e-11, worst
