@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="replay launches without a HIP graph")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB 5 times (a known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -215,6 +218,13 @@ def main():
     }
     if head != "fp64":
         out["speedup_vs_fp64"] = results["fp64"]["ms"] / r["ms"]
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.randn(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps(out))
 
 

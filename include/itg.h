@@ -86,6 +86,18 @@ itg_status itg_synchronize(itg_handle* h);
 /* Library version, "major.minor.patch". */
 const char* itg_version(void);
 
+/* ---------------------------------------------------------- device memory */
+/* Device buffers for callers without their own GPU array package (the Julia
+ * extension's ItgVector storage type, which stands where CuArray/ROCArray
+ * stand for the reference's GPU extensions, NDTensors/ext/NDTensorsAMDGPUExt/
+ * adapt.jl:9-13). Copies are stream-ordered; the host-side ones return after
+ * the data has arrived. */
+itg_status itg_malloc(itg_handle* h, size_t bytes, void** ptr);
+itg_status itg_free(itg_handle* h, void* ptr);
+itg_status itg_memcpy_htod(itg_handle* h, void* dst, const void* src, size_t bytes);
+itg_status itg_memcpy_dtoh(itg_handle* h, void* dst, const void* src, size_t bytes);
+itg_status itg_memcpy_dtod(itg_handle* h, void* dst, const void* src, size_t bytes);
+
 /* --------------------------------------------------------------- permute */
 
 /* dst = alpha * permutedims(src, perm) + beta * dst.
@@ -187,6 +199,32 @@ itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
                           const int64_t* blockdims, const int64_t* offs_src,
                           const int64_t* offs_dst, const int32_t* perm,
                           const void* src, void* dst);
+
+/* ---------------------------------------------------- batched contractions */
+
+typedef struct itg_batch_plan itg_batch_plan;
+
+/* nbatch independent dense contractions sharing one label pattern, each with
+ * its own dimensions (dims1 is N1 x nbatch: item i at dims1[i*N1..]) and its
+ * own caller-owned device buffers T1[i], T2[i], R[i] (host arrays of device
+ * pointers, bound at creation). Each kernel of the contraction then runs as
+ * one grouped launch over all items, which makes a batch of small networks
+ * one captured launch per step instead of nbatch host round trips.
+ * No reference counterpart: NDTensors contracts the items one by one (a host
+ * loop over contract!, NDTensors/src/dense/tensoralgebra/contract.jl:160-216);
+ * this is the batched form of itg_contract_dense for callers that hold many
+ * small networks. */
+itg_status itg_batch_plan_create(itg_handle* h, itg_dtype dtype, int64_t nbatch,
+                                 int NR, const int64_t* dimsR, const int64_t* labelsR,
+                                 void* const* R,
+                                 int N1, const int64_t* dims1, const int64_t* labels1,
+                                 const void* const* T1,
+                                 int N2, const int64_t* dims2, const int64_t* labels2,
+                                 const void* const* T2, itg_batch_plan** out);
+/* R[i] = alpha * T1[i]*T2[i] for every item (beta = 0). */
+itg_status itg_batch_execute(itg_handle* h, itg_batch_plan* p, const double* alpha,
+                             itg_prec prec);
+itg_status itg_batch_plan_destroy(itg_batch_plan* p);
 
 /* ------------------------------------------------------------- autotuner */
 

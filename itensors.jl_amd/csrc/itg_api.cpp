@@ -43,6 +43,15 @@ struct itg_bs_plan {
   std::vector<double> ep_alpha_key;
 };
 
+struct itg_batch_plan {
+  int device = 0;
+  itg_dtype dtype = ITG_F64;
+  const void* base1 = nullptr;
+  const void* base2 = nullptr;
+  void* baseR = nullptr;
+  std::unique_ptr<ExecPlan> ep;
+};
+
 namespace {
 
 itg_status fail(itg_handle* h, itg_status st, const std::string& msg) {
@@ -166,6 +175,49 @@ itg_status itg_synchronize(itg_handle* h) {
   hipError_t e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
   return ITG_OK;
+}
+
+itg_status itg_malloc(itg_handle* h, size_t bytes, void** ptr) {
+  if (!h || !ptr) return ITG_ERR_INVALID;
+  *ptr = nullptr;
+  if (bytes == 0) return ITG_OK;
+  DeviceGuard g(h->device);
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return hip_fail(h, e, "hipMalloc");
+  return ITG_OK;
+}
+
+itg_status itg_free(itg_handle* h, void* ptr) {
+  if (!h) return ITG_ERR_INVALID;
+  if (!ptr) return ITG_OK;
+  DeviceGuard g(h->device);
+  // queued kernels may still use the buffer
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipFree(ptr);
+  if (e != hipSuccess) return hip_fail(h, e, "hipFree");
+  return ITG_OK;
+}
+
+static itg_status copy_bytes(itg_handle* h, void* dst, const void* src, size_t bytes,
+                             hipMemcpyKind kind, bool sync) {
+  if (!h) return ITG_ERR_INVALID;
+  if (bytes == 0) return ITG_OK;
+  if (!dst || !src) return fail(h, ITG_ERR_INVALID, "null pointer");
+  DeviceGuard g(h->device);
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, h->stream);
+  if (e == hipSuccess && sync) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync");
+  return ITG_OK;
+}
+
+itg_status itg_memcpy_htod(itg_handle* h, void* dst, const void* src, size_t bytes) {
+  return copy_bytes(h, dst, src, bytes, hipMemcpyHostToDevice, true);
+}
+itg_status itg_memcpy_dtoh(itg_handle* h, void* dst, const void* src, size_t bytes) {
+  return copy_bytes(h, dst, src, bytes, hipMemcpyDeviceToHost, true);
+}
+itg_status itg_memcpy_dtod(itg_handle* h, void* dst, const void* src, size_t bytes) {
+  return copy_bytes(h, dst, src, bytes, hipMemcpyDeviceToDevice, false);
 }
 
 itg_status itg_permute(itg_handle* h, itg_dtype dtype, int ndim, const int64_t* dims,
@@ -621,6 +673,111 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   if (e == hipErrorNotSupported) return fail(h, ITG_ERR_UNSUPPORTED, err);
   if (e != hipSuccess) return fail(h, ITG_ERR_HIP, err);
   if (best >= 0) h->tuned[tune_key(ep, cplx, (int)prec)] = best;
+  return ITG_OK;
+}
+
+// ------------------------------------------------------------- batches
+itg_status itg_batch_plan_create(itg_handle* h, itg_dtype dtype, int64_t nbatch, int NR,
+                                 const int64_t* dimsR, const int64_t* labelsR, void* const* R,
+                                 int N1, const int64_t* dims1, const int64_t* labels1,
+                                 const void* const* T1, int N2, const int64_t* dims2,
+                                 const int64_t* labels2, const void* const* T2,
+                                 itg_batch_plan** out) {
+  if (!h || !out) return ITG_ERR_INVALID;
+  *out = nullptr;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (nbatch <= 0) return fail(h, ITG_ERR_INVALID, "empty batch");
+  if (!R || !T1 || !T2 || (NR > 0 && (!dimsR || !labelsR)) || (N1 > 0 && (!dims1 || !labels1)) ||
+      (N2 > 0 && (!dims2 || !labels2)))
+    return fail(h, ITG_ERR_INVALID, "null argument");
+  Structure st;
+  bool unsup = false;
+  std::string m = analyze_labels(NR, labelsR, N1, labels1, N2, labels2, &st, &unsup);
+  if (!m.empty()) return fail(h, unsup ? ITG_ERR_UNSUPPORTED : ITG_ERR_INVALID, m);
+  const int64_t es = dtype == ITG_C128 ? 16 : 8;
+  std::unique_ptr<itg_batch_plan> p(new itg_batch_plan);
+  p->device = h->device;
+  p->dtype = dtype;
+  // base pointer of each operand = lowest item address; items become
+  // element offsets from it
+  auto base_of = [&](const void* const* ptrs) {
+    uintptr_t b = UINTPTR_MAX;
+    for (int64_t i = 0; i < nbatch; ++i) b = std::min(b, (uintptr_t)ptrs[i]);
+    return b;
+  };
+  const uintptr_t b1 = base_of(T1), b2 = base_of(T2), bR = base_of((const void* const*)R);
+  p->base1 = (const void*)b1;
+  p->base2 = (const void*)b2;
+  p->baseR = (void*)bR;
+  BlockList L1, L2, LR;
+  L1.N = N1;
+  L2.N = N2;
+  LR.N = NR;
+  std::vector<PairSpec> pairs;
+  for (int64_t i = 0; i < nbatch; ++i) {
+    const int64_t* d1 = dims1 + i * N1;
+    const int64_t* d2 = dims2 + i * N2;
+    const int64_t* dR = dimsR + i * NR;
+    int64_t n1 = 1, n2 = 1, nR = 1;
+    for (int k = 0; k < N1; ++k) {
+      if (d1[k] < 0) return fail(h, ITG_ERR_INVALID, "negative dimension");
+      n1 *= d1[k];
+      L1.dims.push_back((int32_t)d1[k]);
+    }
+    for (int k = 0; k < N2; ++k) {
+      if (d2[k] < 0) return fail(h, ITG_ERR_INVALID, "negative dimension");
+      n2 *= d2[k];
+      L2.dims.push_back((int32_t)d2[k]);
+    }
+    for (int r = 0; r < NR; ++r) {
+      const int64_t want = st.r_src[r] == 1 ? d1[st.r_pos[r]] : d2[st.r_pos[r]];
+      if (dR[r] != want) return fail(h, ITG_ERR_INVALID, "dimension mismatch between R and inputs");
+      nR *= dR[r];
+      LR.dims.push_back((int32_t)dR[r]);
+    }
+    for (int k = 0; k < st.nk; ++k)
+      if (d1[st.k1[k]] != d2[st.k2[k]])
+        return fail(h, ITG_ERR_INVALID, "contracted dimensions do not match");
+    if (n1 > INT32_MAX || n2 > INT32_MAX || nR > INT32_MAX)
+      return fail(h, ITG_ERR_UNSUPPORTED, "item with more than 2^31-1 elements");
+    const uintptr_t a1 = (uintptr_t)T1[i], a2 = (uintptr_t)T2[i], aR = (uintptr_t)R[i];
+    if ((a1 - b1) % es || (a2 - b2) % es || (aR - bR) % es)
+      return fail(h, ITG_ERR_INVALID, "item pointers are not element-aligned");
+    if (N1 == 0) L1.dims.push_back(1);
+    if (N2 == 0) L2.dims.push_back(1);
+    if (NR == 0) LR.dims.push_back(1);
+    L1.offs.push_back((int64_t)((a1 - b1) / es));
+    L2.offs.push_back((int64_t)((a2 - b2) / es));
+    LR.offs.push_back((int64_t)((aR - bR) / es));
+    L1.nnz = std::max(L1.nnz, L1.offs.back() + n1);
+    L2.nnz = std::max(L2.nnz, L2.offs.back() + n2);
+    LR.nnz = std::max(LR.nnz, LR.offs.back() + nR);
+    pairs.push_back(PairSpec{i, i, i});
+  }
+  DeviceGuard g(h->device);
+  p->ep.reset(new ExecPlan);
+  m = build_exec_plan(st, L1, L2, LR, pairs, nullptr, h->stream, p->ep.get());
+  if (!m.empty()) return fail(h, ITG_ERR_HIP, m);
+  *out = p.release();
+  return ITG_OK;
+}
+
+itg_status itg_batch_execute(itg_handle* h, itg_batch_plan* p, const double* alpha,
+                             itg_prec prec) {
+  if (!h || !p) return ITG_ERR_INVALID;
+  if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
+    return fail(h, ITG_ERR_INVALID, "unknown precision mode");
+  if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
+  DeviceGuard g(h->device);
+  return run_plan(h, *p->ep, p->dtype == ITG_C128, (int)prec, p->base1, p->base2, p->baseR, 0u,
+                  alpha, nullptr);
+}
+
+itg_status itg_batch_plan_destroy(itg_batch_plan* p) {
+  if (!p) return ITG_OK;
+  DeviceGuard g(p->device);
+  (void)hipDeviceSynchronize();
+  delete p;
   return ITG_OK;
 }
 

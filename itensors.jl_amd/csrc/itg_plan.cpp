@@ -421,20 +421,35 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
   ep->permB = ch.permB;
   ep->ta = ch.permA ? 0 : ch.ta;
   ep->tb = ch.permB ? 0 : ch.tb;
-  ep->nnzA = P.nnz;
-  ep->nnzB = Q.nnz;
-  ep->nnzC = bR.nnz;
   (void)NP;
   (void)NQ;
+  // Workspace copies (permuted operands, the pre-permutation C) are packed:
+  // block b at the running sum of the sizes before it, whatever the source
+  // offsets are (they may be far apart when a batch spans allocations).
+  auto packed = [](const BlockList& T, std::vector<int64_t>* offs) {
+    offs->resize(T.nblk());
+    int64_t acc = 0;
+    for (int64_t b = 0; b < T.nblk(); ++b) {
+      (*offs)[b] = acc;
+      int64_t n = 1;
+      for (int i = 0; i < T.N; ++i) n *= T.d(b)[i];
+      acc += n;
+    }
+    return acc;
+  };
+  std::vector<int64_t> offA, offB, offC;
+  ep->nnzA = packed(P, &offA);
+  ep->nnzB = packed(Q, &offB);
+  ep->nnzC = packed(bR, &offC);
   hipError_t e;
   if (ch.permA) {
     std::vector<int> perm = seq_concat(Mp, ch.Xp);  // new mode i = old mode perm[i]
-    if ((e = build_permute(&ep->pA, (int)perm.size(), perm.data(), P, nullptr, s)) != hipSuccess)
+    if ((e = build_permute(&ep->pA, (int)perm.size(), perm.data(), P, &offA, s)) != hipSuccess)
       return std::string("device table upload failed: ") + hipGetErrorString(e);
   }
   if (ch.permB) {
     std::vector<int> perm = seq_concat(ch.Xq, Nq);
-    if ((e = build_permute(&ep->pB, (int)perm.size(), perm.data(), Q, nullptr, s)) != hipSuccess)
+    if ((e = build_permute(&ep->pB, (int)perm.size(), perm.data(), Q, &offB, s)) != hipSuccess)
       return std::string("device table upload failed: ") + hipGetErrorString(e);
   }
   if (permC) {
@@ -446,15 +461,15 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
     // dims of the temp blocks
     BlockList tmp;
     tmp.N = NR;
-    tmp.offs = bR.offs;
-    tmp.nnz = bR.nnz;
+    tmp.offs = offC;
+    tmp.nnz = ep->nnzC;
     tmp.dims.resize(bR.dims.size());
     for (int64_t blk = 0; blk < bR.nblk(); ++blk) {
       const int32_t* d = bR.d(blk);
       int32_t* t = tmp.dims.data() + blk * NR;
       for (int r = 0; r < NR; ++r) t[perm[r]] = d[r];
     }
-    if ((e = build_permute(&ep->pC, NR, perm.data(), tmp, nullptr, s)) != hipSuccess)
+    if ((e = build_permute(&ep->pC, NR, perm.data(), tmp, &bR.offs, s)) != hipSuccess)
       return std::string("device table upload failed: ") + hipGetErrorString(e);
   }
   // GEMM pairs and groups
@@ -472,8 +487,8 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
     for (int x : ch.Xp) K *= dP[x];
     GemmPair& g = ep->h_pairs[p];
     std::memset(&g, 0, sizeof(g));
-    g.off_a = P.offs[bp];
-    g.off_b = Q.offs[bq];
+    g.off_a = ch.permA ? offA[bp] : P.offs[bp];
+    g.off_b = ch.permB ? offB[bq] : Q.offs[bq];
     g.K = (int32_t)K;
     g.lda = (int32_t)((ep->ta && !ch.permA) ? std::max<int64_t>(K, 1) : std::max<int64_t>(M, 1));
     g.ldb = (int32_t)((ep->tb && !ch.permB) ? std::max<int64_t>(N, 1) : std::max<int64_t>(K, 1));
@@ -489,7 +504,7 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
     for (int x : Nq) N *= Q.d(bq)[x];
     GemmGroup& g = ep->h_groups[gi];
     std::memset(&g, 0, sizeof(g));
-    g.off_c = bR.offs[br];
+    g.off_c = permC ? offC[br] : bR.offs[br];
     g.M = (int32_t)M;
     g.N = (int32_t)N;
     g.ldc = (int32_t)std::max<int64_t>(M, 1);

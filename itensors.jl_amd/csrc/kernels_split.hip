@@ -79,8 +79,13 @@ __global__ void __launch_bounds__(kThreads)
   const int tid = threadIdx.x;
   const double* x = src + d.off;
 
-  // phase 1: per-row max |x| over K (4 threads per row)
-  {
+  // The source is contiguous either along the rows (A stored M-fastest) or
+  // along K (A stored K-fastest, B in N form); threads are laid along the
+  // contiguous direction so that loads coalesce.
+  const bool kfast = d.nkm == 1 && d.kstr[0] == 1;
+
+  // phase 1: per-row max |x| over K
+  if (!kfast) {
     const int row = tid % kRowsPerWG, part = tid / kRowsPerWG;
     double m = 0.0;
     if (r0 + row < d.R) {
@@ -92,6 +97,26 @@ __global__ void __launch_bounds__(kThreads)
       }
     }
     red[part][row] = m;
+  } else {
+    // one wave per row at a time, lanes along K
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int row = wave; row < kRowsPerWG; row += 4) {
+      double m = 0.0;
+      if (r0 + row < d.R) {
+        const double* xr = x + mode_offset((uint32_t)(r0 + row), d.nrm, d.rdim, d.rstr);
+        for (int k = lane; k < d.K; k += 64) {
+          const double v = xr[k];
+          m = fmax(m, fabs(v));
+          if (!isfinite(v)) m = INFINITY;
+        }
+      }
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s));
+      if (lane == 0) {
+        red[0][row] = m;
+        red[1][row] = red[2][row] = red[3][row] = 0.0;
+      }
+    }
   }
   __syncthreads();
   if (tid < kRowsPerWG) {
@@ -111,11 +136,32 @@ __global__ void __launch_bounds__(kThreads)
   const int items = kRowsPerWG * nchunk;
   const int64_t pstride = (int64_t)d.R * d.Kpad;
   for (int it = tid; it < items; it += kThreads) {
-    const int row = it % kRowsPerWG, ch = it / kRowsPerWG;
+    // consecutive threads: consecutive rows (row-contiguous source) or
+    // consecutive K chunks of one row (K-contiguous source)
+    const int row = kfast ? it / nchunk : it % kRowsPerWG;
+    const int ch = kfast ? it % nchunk : it / kRowsPerWG;
     const int gr = r0 + row;
     if (gr >= d.R) continue;
     const int e = sexp[row];
     const int64_t ro = mode_offset((uint32_t)gr, d.nrm, d.rdim, d.rstr);
+    // K offsets of this chunk: step through the K modes instead of dividing
+    int64_t ko = 0;
+    int32_t kc[kSliceModes] = {0, 0, 0, 0};
+    {
+      uint32_t idx = (uint32_t)(ch * 16);
+#pragma unroll
+      for (int i = 0; i < kSliceModes; ++i) {
+        if (i >= d.nkm) break;
+        if (i + 1 < d.nkm) {
+          const uint32_t q = idx / (uint32_t)d.kdim[i];
+          kc[i] = (int32_t)(idx - q * (uint32_t)d.kdim[i]);
+          idx = q;
+        } else {
+          kc[i] = (int32_t)idx;
+        }
+        ko += (int64_t)kc[i] * d.kstr[i];
+      }
+    }
     uint32_t packed[S][4];
 #pragma unroll
     for (int i = 0; i < S; ++i)
@@ -125,8 +171,21 @@ __global__ void __launch_bounds__(kThreads)
     for (int j = 0; j < 16; ++j) {
       const int k = ch * 16 + j;
       double y = 0.0;
-      if (k < d.K && e != kNonFinite)
-        y = ldexp(x[ro + mode_offset((uint32_t)k, d.nkm, d.kdim, d.kstr)], 6 - e);
+      if (k < d.K && e != kNonFinite) y = ldexp(x[ro + ko], 6 - e);
+      // advance the K coordinate by one (mode 0 fastest)
+      ko += d.kstr[0];
+      if (++kc[0] == d.kdim[0]) {
+        kc[0] = 0;
+        ko -= (int64_t)d.kdim[0] * d.kstr[0];
+#pragma unroll
+        for (int i = 1; i < kSliceModes; ++i) {
+          if (i >= d.nkm) break;
+          ko += d.kstr[i];
+          if (++kc[i] < d.kdim[i]) break;
+          kc[i] = 0;
+          ko -= (int64_t)d.kdim[i] * d.kstr[i];
+        }
+      }
 #pragma unroll
       for (int i = 0; i < S; ++i) {
         const double v = rint(y);
@@ -145,7 +204,7 @@ __global__ void __launch_bounds__(kThreads)
 constexpr int BM = 64, BN = 64, BKB = 32;  // BKB: K bytes per LDS stage
 
 template <int S>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, 2)
     split_gemm_kernel(const SplitPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
                       const int2* __restrict__ tiles, const int8_t* __restrict__ planesA,
                       const int8_t* __restrict__ planesB, const int32_t* __restrict__ expA,
@@ -183,47 +242,37 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int L = 0; L < NL; ++L) lev[L] = v16i{0};
 
-    v4i ra[(CH_A + kThreads - 1) / kThreads], rb[(CH_B + kThreads - 1) / kThreads];
+    // one 16-byte chunk = (slice plane, row, K half) of the A or B tile;
+    // the CH_A + CH_B chunks of a stage are spread over the threads
+    constexpr int NQ = (CH_A + CH_B + kThreads - 1) / kThreads;
+    v4i rq[NQ];
     auto load = [&](int k0) {
 #pragma unroll
-      for (int q = 0; q < (CH_A + kThreads - 1) / kThreads; ++q) {
-        const int c = tid + q * kThreads;
-        if (c < CH_A) {
-          const int half = c & 1, row = (c >> 1) % BM, pl = c / (2 * BM);
-          const int g = m0 + row;
-          ra[q] = (g < grp.M) ? *reinterpret_cast<const v4i*>(pa + pl * pr.a_pstride +
-                                                              (int64_t)g * kpad + k0 + half * 16)
-                              : v4i{0, 0, 0, 0};
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < (CH_B + kThreads - 1) / kThreads; ++q) {
-        const int c = tid + q * kThreads;
-        if (c < CH_B) {
-          const int half = c & 1, row = (c >> 1) % BN, pl = c / (2 * BN);
-          const int g = n0 + row;
-          rb[q] = (g < grp.N) ? *reinterpret_cast<const v4i*>(pb + pl * pr.b_pstride +
-                                                              (int64_t)g * kpad + k0 + half * 16)
-                              : v4i{0, 0, 0, 0};
-        }
+      for (int q = 0; q < NQ; ++q) {
+        int c = tid + q * kThreads;
+        if (c >= CH_A + CH_B) continue;
+        const bool isa = c < CH_A;
+        if (!isa) c -= CH_A;
+        const int half = c & 1, row = (c >> 1) % 64, pl = c / 128;
+        const int g = (isa ? m0 : n0) + row;
+        const bool in = g < (isa ? grp.M : grp.N);
+        const int8_t* src = (isa ? pa + pl * pr.a_pstride : pb + pl * pr.b_pstride) +
+                            (int64_t)g * kpad + k0 + half * 16;
+        rq[q] = in ? *reinterpret_cast<const v4i*>(src) : v4i{0, 0, 0, 0};
       }
     };
     auto store = [&](int buf) {
 #pragma unroll
-      for (int q = 0; q < (CH_A + kThreads - 1) / kThreads; ++q) {
-        const int c = tid + q * kThreads;
-        if (c < CH_A) {
-          const int half = c & 1, row = (c >> 1) % BM, pl = c / (2 * BM);
-          As[buf][pl][row][half ^ ((row >> 3) & 1)] = ra[q];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < (CH_B + kThreads - 1) / kThreads; ++q) {
-        const int c = tid + q * kThreads;
-        if (c < CH_B) {
-          const int half = c & 1, row = (c >> 1) % BN, pl = c / (2 * BN);
-          Bs[buf][pl][row][half ^ ((row >> 3) & 1)] = rb[q];
-        }
+      for (int q = 0; q < NQ; ++q) {
+        int c = tid + q * kThreads;
+        if (c >= CH_A + CH_B) continue;
+        const bool isa = c < CH_A;
+        if (!isa) c -= CH_A;
+        const int half = c & 1, row = (c >> 1) % 64, pl = c / 128;
+        if (isa)
+          As[buf][pl][row][half ^ ((row >> 3) & 1)] = rq[q];
+        else
+          Bs[buf][pl][row][half ^ ((row >> 3) & 1)] = rq[q];
       }
     };
 
@@ -233,20 +282,25 @@ __global__ void __launch_bounds__(kThreads)
     int buf = 0;
     for (int k0 = 0; k0 < kpad; k0 += BKB) {
       const bool more = k0 + BKB < kpad;
-      if (more) load(k0 + BKB);
       const int ra_row = wm0 + l32, rb_row = wn0 + l32;
-      v4i av[S], bv[S];
+      // all B slices stay in registers; A slices stream through one register
+      // quad (keeps the kernel at two waves per SIMD)
+      v4i bv[S];
+#pragma unroll
+      for (int j = 0; j < S; ++j) bv[j] = Bs[buf][j][rb_row][lh ^ ((rb_row >> 3) & 1)];
 #pragma unroll
       for (int i = 0; i < S; ++i) {
-        av[i] = As[buf][i][ra_row][lh ^ ((ra_row >> 3) & 1)];
-        bv[i] = Bs[buf][i][rb_row][lh ^ ((rb_row >> 3) & 1)];
-      }
-#pragma unroll
-      for (int i = 0; i < S; ++i)
+        const v4i av = As[buf][i][ra_row][lh ^ ((ra_row >> 3) & 1)];
 #pragma unroll
         for (int j = 0; j < S - i; ++j)  // slice i of A with slice j of B, level i+j
-          lev[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av[i], lev[i + j], 0, 0, 0);
-      if (more) store(buf ^ 1);
+          lev[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av, lev[i + j], 0, 0, 0);
+      }
+      // next stage: fetched after the MFMA block so the staging registers are
+      // not live across it; the other wave on this SIMD covers the latency
+      if (more) {
+        load(k0 + BKB);
+        store(buf ^ 1);
+      }
       __syncthreads();
       buf ^= 1;
     }

@@ -36,6 +36,8 @@ EXPORTS = (
     "itg_version", "itg_permute", "itg_contract_dense", "itg_contract_dense_ex",
     "itg_bs_plan_create", "itg_bs_plan_sizes", "itg_bs_plan_blocks", "itg_bs_plan_pairs",
     "itg_bs_execute", "itg_bs_plan_destroy", "itg_bs_permute", "itg_bs_autotune",
+    "itg_batch_plan_create", "itg_batch_execute", "itg_batch_plan_destroy",
+    "itg_malloc", "itg_free", "itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod",
 )
 
 
@@ -94,6 +96,16 @@ def lib():
         L.itg_bs_plan_destroy.argtypes = [P]
         L.itg_bs_permute.argtypes = [P, I, I, I64, I64P, I64P, I64P, I32P, P, P]
         L.itg_bs_autotune.argtypes = [P, P, I, P, P, P, I]
+        PP = ctypes.POINTER(P)
+        L.itg_batch_plan_create.argtypes = [P, I, I64, I, I64P, I64P, PP, I, I64P, I64P, PP,
+                                            I, I64P, I64P, PP, ctypes.POINTER(P)]
+        L.itg_batch_execute.argtypes = [P, P, DP, I]
+        L.itg_batch_plan_destroy.argtypes = [P]
+        SZ = ctypes.c_size_t
+        L.itg_malloc.argtypes = [P, SZ, ctypes.POINTER(P)]
+        L.itg_free.argtypes = [P, P]
+        for n in ("itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod"):
+            getattr(L, n).argtypes = [P, P, P, SZ]
         for name in EXPORTS:
             if name not in ("itg_last_error", "itg_version"):
                 getattr(L, name).restype = I

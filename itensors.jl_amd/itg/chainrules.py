@@ -41,6 +41,117 @@ def permute_like(x, inds):
     return ITensor(ND.permutedims(x.tensor, perm), inds)
 
 
+def replaceinds(x, inds):
+    """The same storage under other indices (each new index must have the
+    space of the one it replaces)."""
+    inds = tuple(inds)
+    for a, b in zip(x.inds, inds):
+        if a.space != b.space:
+            raise ValueError(f"index {b} does not have the space of {a}")
+    return ITensor(x.tensor, inds)
+
+
+def _layout_map(src, dst):
+    """Element positions in dst's storage of every element of src's storage
+    (same index set; src's blocks must be blocks of dst). Returns a device
+    int64 tensor, or None when the layouts are identical."""
+    import math
+
+    import torch
+
+    if tuple(src.inds) == tuple(dst.inds):
+        if not src.hasqns() or list(src.tensor.blockoffsets.items()) == list(dst.tensor.blockoffsets.items()):
+            return None
+    perm = [src.inds.index(i) for i in dst.inds]  # dst mode k = src mode perm[k]
+    chunks = []
+    if not src.hasqns():
+        dims = src.tensor.dims
+        idx = torch.arange(math.prod(dims), device=src.tensor.data.device).reshape(tuple(reversed(dims)))
+        # idx[c_{n-1},...,c_0] = linear src index; dst position of each src element
+        ddims = tuple(dims[p] for p in perm)
+        dst_lin = torch.arange(math.prod(ddims), device=src.tensor.data.device).reshape(tuple(reversed(ddims)))
+        # dst coordinate k = src coordinate perm[k]; permute dst_lin's axes to src order
+        inv = [perm.index(m) for m in range(len(dims))]
+        rev = len(dims) - 1
+        src_view = dst_lin.permute(*[rev - inv[rev - a] for a in range(len(dims))])
+        return src_view.reshape(-1).contiguous()
+    dev = src.tensor.data.device
+    for b, o in src.tensor.blockoffsets.items():
+        db = tuple(b[p] for p in perm)
+        if db not in dst.tensor.blockoffsets:
+            raise ValueError("gradient block outside the tensor's block structure")
+        od = dst.tensor.blockoffsets[db]
+        dims = src.tensor.blockdims_of(b)
+        n = math.prod(dims)
+        ddims = tuple(dims[p] for p in perm)
+        dst_lin = torch.arange(n, device=dev).reshape(tuple(reversed(ddims)))
+        inv = [perm.index(m) for m in range(len(dims))]
+        rev = len(dims) - 1
+        src_view = dst_lin.permute(*[rev - inv[rev - a] for a in range(len(dims))])
+        chunks.append(src_view.reshape(-1) + od)
+    if not chunks:
+        return torch.empty(0, dtype=torch.int64, device=dev)
+    return torch.cat(chunks).contiguous()
+
+
+def axpy_(y, a, x):
+    """y += a*x for ITensors over the same index set (x may store its modes
+    in another order and hold a subset of y's blocks). Returns y."""
+    m = _layout_map(x, y)
+    if m is None:
+        y.tensor.data.add_(x.tensor.data, alpha=a)
+    else:
+        y.tensor.data.index_add_(0, m, x.tensor.data, alpha=a)
+    return y
+
+
+def inner(x, y):
+    """<x|y> = sum conj(x)*y over matching layouts (returns a Python number)."""
+    m = _layout_map(y, x)
+    yd = y.tensor.data if m is None else y.tensor.data.new_zeros(x.tensor.data.numel()).index_add_(0, m, y.tensor.data)
+    v = (x.tensor.data.conj() * yd).sum().item()
+    return v
+
+
+def heff_energy_and_grad(L, psi, W1, W2, R, out_inds, *, prec=None):
+    """E(psi) = Re<psi|H psi>/<psi|psi> for H psi = ((((L*psi)*W1)*W2)*R) with
+    H psi carrying `out_inds` (psi's spaces), and dE/dpsi by reverse mode
+    through rrule_contract -- every backward step is a device contraction
+    (the adjoints conjugate inside the kernels). For real psi the result is
+    ((H + H^T) psi - 2 E psi) / <psi|psi>. Returns (E, grad ITensor in psi's
+    layout)."""
+    import torch
+
+    y1, pb1 = rrule_contract(L, psi, prec=prec)
+    y2, pb2 = rrule_contract(y1, W1, prec=prec)
+    y3, pb3 = rrule_contract(y2, W2, prec=prec)
+    y4, pb4 = rrule_contract(y3, R, prec=prec)
+    bra = replaceinds(psi, out_inds)
+    bra = ITensor(bra.tensor, tuple(i.dag() for i in bra.inds))  # dag, data conj in-kernel
+    num_t, pbn = rrule_contract(bra, y4, prec=prec)
+    num = complex(num_t.tensor.data.reshape(-1)[0].item()) if num_t.tensor.data.numel() else 0j
+    den = float(torch.linalg.vector_norm(psi.tensor.data).item() ** 2)
+    E = num.real / den
+    # cotangent of num (a 0-index ITensor) for E = Re(num)/den
+    seed = ITensor(ND._with_data(num_t.tensor, torch.full_like(num_t.tensor.data, 1.0 / den)), num_t.inds)
+    _, bra_bar, y4_bar = pbn(seed)
+    _, y3_bar, _ = pb4(y4_bar)
+    _, y2_bar, _ = pb3(y3_bar)
+    _, y1_bar, _ = pb2(y2_bar)
+    _, _, psi_bar = pb1(y1_bar)
+    grad = ITensor(ND._with_data(psi.tensor, psi.tensor.data.clone()), psi.inds)
+    grad.tensor.data.mul_(-2.0 * E / den)
+    axpy_(grad, 1.0, psi_bar)
+    # the bra holds conj(psi): its cotangent maps back to psi conjugated
+    bb = replaceinds(ITensor(bra_bar.tensor, tuple(i.dag() for i in bra_bar.inds)),
+                     tuple(psi.inds[out_inds.index(i)] if i in out_inds else i
+                           for i in (j.dag() for j in bra_bar.inds)))
+    if bb.eltype.is_complex:
+        bb = ITensor(ND._with_data(bb.tensor, bb.tensor.data.conj().resolve_conj()), bb.inds)
+    axpy_(grad, 1.0, bb)
+    return E, grad
+
+
 def rrule_contract(x1, x2, *, prec=None):
     y = contract(x1, x2, prec=prec)
     cplx1 = x1.eltype.is_complex

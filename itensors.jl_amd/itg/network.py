@@ -66,11 +66,111 @@ class ContractionSequence:
 
     def capture(self, prec=None):
         """Record one run() in a graph; replay with graph.replay()."""
+        return _capture(self.run, prec)
+
+
+def _capture(run, prec):
+    import torch
+
+    run(prec)  # warm: workspace sized, device tables uploaded
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        run(prec)
+    return g
+
+
+class BatchPlan:
+    """libitg batch plan: nbatch dense contractions with one label pattern,
+    bound to their device buffers, executed as grouped launches."""
+
+    def __init__(self, A_list, lA, B_list, lB, R_list, lR):
+        import ctypes
+        from . import _lib
+
+        n = len(A_list)
+        dev = R_list[0].data.device
+        self.h = ND._handle(dev)
+        keep = []
+
+        def arr64(v):
+            a = _lib.i64(v)
+            keep.append(a)
+            return _lib.ptr64(a)[0]
+
+        def ptrs(ts):
+            a = (ctypes.c_void_p * n)(*[t.data.data_ptr() for t in ts])
+            keep.append(a)
+            return a
+
+        plan = ctypes.c_void_p()
+        st = _lib.lib().itg_batch_plan_create(
+            self.h.h, ND._dtype_code(R_list[0].eltype), n,
+            len(lR), arr64([d for t in R_list for d in t.dims]), arr64(lR), ptrs(R_list),
+            len(lA), arr64([d for t in A_list for d in t.dims]), arr64(lA), ptrs(A_list),
+            len(lB), arr64([d for t in B_list for d in t.dims]), arr64(lB), ptrs(B_list),
+            ctypes.byref(plan))
+        self.h.check(st)
+        self.plan = plan
+        self.dev = dev
+
+    def execute(self, prec=None, alpha=1.0):
+        from . import _lib
+
+        h = ND._handle(self.dev)
+        a, _k = _lib.scalar2(alpha)
+        h.check(_lib.lib().itg_batch_execute(h.h, self.plan, a, _lib.PREC[prec or ND.precision()]))
+
+    def __del__(self):
+        from . import _lib
+
+        try:
+            if getattr(self, "plan", None) and _lib._lib is not None:
+                _lib._lib.itg_batch_plan_destroy(self.plan)
+        except Exception:
+            pass
+
+
+class BatchedSequence:
+    """Many independent dense networks with the same index pattern (each with
+    its own dimensions) contracted left to right, one grouped launch per
+    kernel of each step for the whole batch. The intermediates of a step
+    share one allocation."""
+
+    def __init__(self, networks, *, prec=None):
         import torch
 
-        self.run(prec)  # warm: workspace sized, device tables uploaded
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self.run(prec)
-        return g
+        self.prec = prec
+        self.plans = []
+        cur = [net[0] for net in networks]
+        nsteps = len(networks[0]) - 1
+        for s in range(nsteps):
+            Bs = [net[s + 1] for net in networks]
+            lA, lB = compute_contraction_labels(cur[0].inds, Bs[0].inds)
+            lR = ND.contract_labels(lA, lB)
+            outs, sizes, dims_all, inds_all = [], [], [], []
+            for A, B in zip(cur, Bs):
+                la, lb = compute_contraction_labels(A.inds, B.inds)
+                if (la, lb) != (lA, lB):
+                    raise ValueError("networks of a batch must share one index pattern")
+                inds = tuple(A.inds[la.index(x)] if x in la else B.inds[lb.index(x)] for x in lR)
+                dims = tuple(i.dim for i in inds)
+                dims_all.append(dims)
+                inds_all.append(inds)
+                sizes.append(math.prod(dims))
+            arena = torch.empty(sum(sizes), dtype=cur[0].eltype, device=cur[0].tensor.data.device)
+            o = 0
+            for dims, n in zip(dims_all, sizes):
+                outs.append(ND.DenseTensor(arena[o:o + n], dims))
+                o += n
+            self.plans.append(BatchPlan([c.tensor for c in cur], lA, [b.tensor for b in Bs], lB, outs, lR))
+            cur = [ITensor(R, inds) for R, inds in zip(outs, inds_all)]
+        self.results = cur
+
+    def run(self, prec=None):
+        for p in self.plans:
+            p.execute(prec or self.prec)
+        return self.results
+
+    def capture(self, prec=None):
+        return _capture(self.run, prec)

@@ -1,0 +1,205 @@
+# NDTensorsMI355XExt -- package-extension glue that routes NDTensors'
+# contraction to libitg.so (the MI355X backend) through `ccall`.
+#
+# Loaded as an NDTensors extension (NDTensors/Project.toml [weakdeps] +
+# [extensions] pattern, as the cuTENSOR/AMDGPU extensions are). Storage on
+# the device is `ItgVector{T}`, a vector of device memory owned by libitg
+# (itg_malloc / itg_free); tensors are moved there with `mi355x(T)` and back
+# with `cpu(T)`. No AMDGPU.jl or CUDA.jl is involved.
+#
+# Seams (ITensors.jl v0.9.27 / NDTensors v0.4.27):
+#   dense        NDTensors.contract!(::Exposed{<:ItgVector,<:DenseTensor}, ...)
+#                -- the method NDTensors/ext/NDTensorscuTENSORExt/contract.jl:15-24
+#                overrides for CuArray
+#   block-sparse NDTensors.contract(::BlockSparseTensor{..,<:ItgVector}, ...)
+#                -- NDTensors/src/blocksparse/contract.jl:3-17, whole plan in one
+#                grouped launch
+#   AD           ChainRulesCore.rrule(::typeof(contract), ::ITensor, ::ITensor)
+#                -- ext/ITensorsChainRulesCoreExt/itensor.jl:94-103, conj in-kernel
+# On ITG_ERR_UNSUPPORTED the methods fall back to the stock path after moving
+# the operands to the host, as the cuTENSOR extension falls back on
+# CUTENSORError (NDTensorscuTENSORExt/contract.jl:45-53).
+module NDTensorsMI355XExt
+
+using NDTensors
+using NDTensors: DenseTensor, BlockSparseTensor, BlockOffsets, Block, Dense, BlockSparse,
+    blockoffsets, inds, nnzblocks, nblocks, blockdim, data, tensor, contract_labels
+using NDTensors.Expose: Exposed, unexpose
+using Dictionaries: Dictionary
+
+const libitg = get(ENV, "ITG_LIBRARY", "libitg")
+
+# ------------------------------------------------------------------ handle
+const ITG_OK = Cint(0)
+const ITG_ERR_UNSUPPORTED = Cint(2)
+const ITG_F64 = Cint(1)
+const ITG_C128 = Cint(3)
+const PRECISION = Ref{Cint}(0)  # 0 FP64 MFMA, 2 split (int8 slices)
+const HANDLE = Ref{Ptr{Cvoid}}(C_NULL)
+
+struct ItgError <: Exception
+    status::Cint
+    msg::String
+end
+
+function handle()
+    if HANDLE[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:itg_init, libitg), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, h)
+        st == ITG_OK || throw(ItgError(st, "itg_init failed"))
+        HANDLE[] = h[]
+    end
+    return HANDLE[]
+end
+
+function check(st::Cint)
+    st == ITG_OK && return nothing
+    msg = unsafe_string(ccall((:itg_last_error, libitg), Cstring, (Ptr{Cvoid},), handle()))
+    throw(ItgError(st, msg))
+end
+
+using_mi355x_split_precision(on::Bool = true) = (PRECISION[] = on ? Cint(2) : Cint(0))
+
+eltcode(::Type{Float64}) = ITG_F64
+eltcode(::Type{ComplexF64}) = ITG_C128
+eltcode(T) = nothing
+
+# --------------------------------------------------------- device storage
+mutable struct ItgVector{T} <: AbstractVector{T}
+    ptr::Ptr{T}
+    len::Int
+    function ItgVector{T}(::UndefInitializer, n::Integer) where {T}
+        p = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:itg_malloc, libitg), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}),
+                    handle(), n * sizeof(T), p))
+        v = new{T}(Ptr{T}(p[]), n)
+        finalizer(v) do x
+            ccall((:itg_free, libitg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), handle(), x.ptr)
+        end
+        return v
+    end
+end
+Base.size(v::ItgVector) = (v.len,)
+Base.similar(::ItgVector{T}, ::Type{S}, dims::Dims{1}) where {T, S} = ItgVector{S}(undef, dims[1])
+function Base.getindex(v::ItgVector{T}, i::Int) where {T}  # scalar access: slow, host copy
+    r = Ref{T}()
+    check(ccall((:itg_memcpy_dtoh, libitg), Cint, (Ptr{Cvoid}, Ref{T}, Ptr{T}, Csize_t),
+                handle(), r, v.ptr + (i - 1) * sizeof(T), sizeof(T)))
+    return r[]
+end
+function Base.Array(v::ItgVector{T}) where {T}
+    a = Vector{T}(undef, v.len)
+    check(ccall((:itg_memcpy_dtoh, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Csize_t),
+                handle(), a, v.ptr, v.len * sizeof(T)))
+    return a
+end
+function ItgVector(a::Vector{T}) where {T}
+    v = ItgVector{T}(undef, length(a))
+    check(ccall((:itg_memcpy_htod, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Csize_t),
+                handle(), v.ptr, a, length(a) * sizeof(T)))
+    return v
+end
+
+mi355x(T::DenseTensor) = tensor(Dense(ItgVector(Vector(data(T)))), inds(T))
+mi355x(T::BlockSparseTensor) = tensor(BlockSparse(ItgVector(Vector(data(T))), blockoffsets(T)), inds(T))
+cpu(T::DenseTensor{<:Any, <:Any, <:Dense{<:Any, <:ItgVector}}) = tensor(Dense(Array(data(T))), inds(T))
+cpu(T::BlockSparseTensor) = tensor(BlockSparse(Array(data(T)), blockoffsets(T)), inds(T))
+
+# --------------------------------------------------------------- dense
+function NDTensors.contract!(
+        exposedR::Exposed{<:ItgVector, <:DenseTensor}, labelsR,
+        exposedT1::Exposed{<:ItgVector, <:DenseTensor}, labelsT1,
+        exposedT2::Exposed{<:ItgVector, <:DenseTensor}, labelsT2,
+        α::Number = one(Bool), β::Number = zero(Bool)
+    )
+    R, T1, T2 = unexpose.((exposedR, exposedT1, exposedT2))
+    elt = eltype(R)
+    code = eltcode(elt)
+    if code === nothing || eltype(T1) != elt || eltype(T2) != elt
+        error("MI355X contraction needs Float64/ComplexF64 operands of one element type")
+    end
+    a = ComplexF64(α)
+    b = ComplexF64(β)
+    st = ccall((:itg_contract_dense, libitg), Cint,
+               (Ptr{Cvoid}, Cint,
+                Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
+                Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
+                Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
+                Ref{ComplexF64}, Ref{ComplexF64}, Cint),
+               handle(), code,
+               ndims(R), collect(Int64, size(R)), collect(Int64, labelsR), data(R).ptr,
+               ndims(T1), collect(Int64, size(T1)), collect(Int64, labelsT1), data(T1).ptr,
+               ndims(T2), collect(Int64, size(T2)), collect(Int64, labelsT2), data(T2).ptr,
+               a, b, PRECISION[])
+    if st == ITG_ERR_UNSUPPORTED
+        Rh = cpu(R)
+        NDTensors.contract!(Rh, labelsR, cpu(T1), labelsT1, cpu(T2), labelsT2, α, β)
+        copyto!(R, mi355x(Rh))
+        return R
+    end
+    check(st)
+    return R
+end
+
+# -------------------------------------------------------- block sparse
+function blockstructure(T::BlockSparseTensor)
+    N = ndims(T)
+    bo = blockoffsets(T)
+    blocks = Int64[]
+    offs = Int64[]
+    for (b, o) in pairs(bo)
+        append!(blocks, Int64.(Tuple(b)))
+        push!(offs, o)
+    end
+    nsec = Int64[nblocks(inds(T), i) for i in 1:N]
+    secd = Int64[blockdim(inds(T)[i], j) for i in 1:N for j in 1:nsec[i]]
+    return length(offs), blocks, offs, nsec, secd
+end
+
+function NDTensors.contract(
+        T1::BlockSparseTensor{<:Any, N1, <:Any, <:BlockSparse{<:Any, <:ItgVector}}, labels1,
+        T2::BlockSparseTensor{<:Any, N2, <:Any, <:BlockSparse{<:Any, <:ItgVector}}, labels2,
+        labelsR = contract_labels(labels1, labels2)
+    ) where {N1, N2}
+    elt = promote_type(eltype(T1), eltype(T2))
+    eltype(T1) == eltype(T2) == elt || error("promote both operands to one element type first")
+    n1, b1, o1, ns1, sd1 = blockstructure(T1)
+    n2, b2, o2, ns2, sd2 = blockstructure(T2)
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:itg_bs_plan_create, libitg), Cint,
+                (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Cint, Ptr{Int64}, Ref{Ptr{Cvoid}}),
+                handle(), N1, n1, b1, o1, ns1, sd1, collect(Int64, labels1),
+                N2, n2, b2, o2, ns2, sd2, collect(Int64, labels2),
+                length(labelsR), collect(Int64, labelsR), plan))
+    try
+        nblk, nnz, npairs = Ref{Int64}(), Ref{Int64}(), Ref{Int64}()
+        check(ccall((:itg_bs_plan_sizes, libitg), Cint,
+                    (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}), plan[], nblk, nnz, npairs))
+        NR = length(labelsR)
+        blocksR = Vector{Int64}(undef, NR * nblk[])
+        offsR = Vector{Int64}(undef, nblk[])
+        check(ccall((:itg_bs_plan_blocks, libitg), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}),
+                    plan[], blocksR, offsR))
+        boffsR = BlockOffsets{NR}()
+        for k in 1:nblk[]
+            blk = Block(ntuple(i -> UInt(blocksR[(k - 1) * NR + i]), NR))
+            insert!(boffsR, blk, Int(offsR[k]))
+        end
+        indsR = NDTensors.contract_inds(inds(T1), labels1, inds(T2), labels2, labelsR)
+        R = tensor(BlockSparse(ItgVector{elt}(undef, nnz[]), boffsR), indsR)
+        if npairs[] > 0
+            check(ccall((:itg_bs_execute, libitg), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                         Ref{ComplexF64}, Ptr{Cvoid}, Cint, Cuint),
+                        handle(), plan[], eltcode(elt), data(R).ptr, data(T1).ptr, data(T2).ptr,
+                        ComplexF64(1), C_NULL, PRECISION[], 0))
+        end
+        return R
+    finally
+        ccall((:itg_bs_plan_destroy, libitg), Cint, (Ptr{Cvoid},), plan[])
+    end
+end
+
+end # module

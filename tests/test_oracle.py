@@ -1,0 +1,264 @@
+"""Pins the CPU oracle (oracle/ndtensors_oracle.py) before it is trusted.
+
+Each check is independent of the formula it tests: known-answer tests taken
+from the reference's own test-suite, brute force over all block pairs,
+invariants (flux conservation), densify-and-compare against numpy's einsum,
+explicit loops, and central finite differences. A planted mistake in the
+oracle (a dropped pair, a wrong label sign, a transposed operand) fails at
+least one of them.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from helpers import relerr
+from oracle import ndtensors_oracle as O
+
+
+def qidx(pairs, tag="", dir=1):
+    return O.Index([(O.QN(q), d) for q, d in pairs], tags=tag, dir=dir)
+
+
+# ----------------------------------------------------------------- QN / labels
+def test_qn_arithmetic():
+    assert O.qn_eq(O.qn_add(O.QN(0), O.QN(1)), O.QN(1))
+    assert O.qn_eq(O.QN(), O.QN(("Sz", 0)))  # missing names are zero
+    assert O.qn_eq(O.qn_add(O.QN(("Sz", 1)), O.QN(("N", 2))), O.QN(("N", 2), ("Sz", 1)))
+    assert O.qn_eq(O.qn_add(O.QN(("P", 1, 2)), O.QN(("P", 1, 2))), O.QN(("P", 0, 2)))
+    assert not O.qn_eq(O.QN(1), O.QN(-1))
+    assert O.qn_eq(O.qn_scale(-1, O.QN(("Sz", 3))), O.QN(("Sz", -3)))
+
+
+def test_contraction_labels_closed_form():
+    i, j, k, l = (O.Index(2, t) for t in "ijkl")
+    la, lb = O.compute_contraction_labels((i, j, k), (k, l, i))
+    # matches are numbered A-position-major: (i: A0~B2) -> -1, (k: A2~B0) -> -2
+    assert la == (-1, 3, -2) and lb == (-2, 4, -1)
+    assert O.contract_labels(la, lb) == (3, 4)
+    la, lb = O.compute_contraction_labels((i, j), (k, l))
+    assert la == (1, 2) and lb == (3, 4)  # outer product
+    la, lb = O.compute_contraction_labels((i, j), (j, i))
+    assert la == (-1, -2) and lb == (-2, -1)
+    assert O.contract_labels(la, lb) == ()
+    assert O.contract_inds((i, j, k), (-1, 3, -2), (k, l, i), (-2, 4, -1), (4, 3)) == (l, j)
+
+
+def test_kat_arrow_mismatch():
+    """test/base/test_qnitensor.jl:1799-1805: matching QN indices must have
+    opposite arrows."""
+    i = qidx([(0, 2), (1, 2)], "i")
+    with pytest.raises(ValueError):
+        O.compute_contraction_labels((i, i.dag().prime()), (i, i.dag().prime(2)))
+    O.compute_contraction_labels((i, i.dag().prime()), (i.dag(), i.prime(3)))
+
+
+# ----------------------------------------------------------- block structure
+def test_kat_flux_and_nnzblocks():
+    """test/base/test_qnitensor.jl:564-583."""
+    rng = np.random.default_rng(0)
+    i = qidx([(0, 1), (1, 2)], "i")
+    j = qidx([(0, 3), (1, 4)], "j")
+    A = O.random_blocksparse(rng, O.QN(0), (i, j.dag()))
+    assert len(A.offsets) == 2
+    B = O.random_blocksparse(rng, O.QN(1), (j, i.dag().prime()))
+    assert len(B.offsets) == 1
+    la, lb = O.compute_contraction_labels(A.inds, B.inds)
+    C, plan = O.blocksparse_contract(A, la, B, lb)
+    assert set(C.inds) == {i, i.prime()}
+    assert len(C.offsets) == 1
+    for b in C.offsets:
+        assert O.qn_eq(O.flux_of_block(C.inds, b), O.QN(1))
+
+
+def test_kat_contract_to_scalar_is_norm2():
+    """test/base/test_qnitensor.jl:937-947: A * dag(A) has one block equal to
+    norm(A)^2."""
+    rng = np.random.default_rng(1)
+    i = qidx([(0, 1), (1, 1)], "i")
+    A = O.random_blocksparse(rng, O.QN(0), (i, i.dag().prime()))
+    Ad = O.BlockSparse(np.conj(A.data), A.offsets, tuple(x.dag() for x in A.inds))
+    la, lb = O.compute_contraction_labels(A.inds, Ad.inds)
+    C, _ = O.blocksparse_contract(A, la, Ad, lb)
+    assert len(C.offsets) == 1 and C.data.size == 1
+    assert abs(C.data[0] - np.sum(A.data ** 2)) < 1e-14 * np.sum(A.data ** 2)
+
+
+def test_zero_block_contraction():
+    """test/threading/test_threading.jl:60-78: no pair matches -> 0 blocks."""
+    i = qidx([(0, 2), (1, 3)], "i")
+    j = qidx([(0, 2), (1, 3)], "j")
+    offA, _ = O.blockoffsets([(0, 0)], (i, j.dag()))
+    offB, _ = O.blockoffsets([(1, 1)], (j, i.dag().prime()))
+    A = O.BlockSparse(np.arange(4.0), offA, (i, j.dag()))
+    B = O.BlockSparse(np.arange(9.0), offB, (j, i.dag().prime()))
+    la, lb = O.compute_contraction_labels(A.inds, B.inds)
+    C, plan = O.blocksparse_contract(A, la, B, lb)
+    assert plan == [] and len(C.offsets) == 0 and C.data.size == 0
+
+
+def _random_case(rng, nA=3, nB=3, ncont=1):
+    idx = []
+    for t in range(nA + nB - ncont):
+        n = int(rng.integers(1, 4))
+        idx.append(qidx([(q, int(rng.integers(1, 4))) for q in range(-(n // 2), n - n // 2)], f"t{t}"))
+    a_inds = idx[:nA]
+    b_inds = [x.dag() for x in idx[:ncont]] + idx[nA:]
+    rng.shuffle(b_inds)
+    A = O.random_blocksparse(rng, O.QN(int(rng.integers(-1, 2))), tuple(a_inds))
+    B = O.random_blocksparse(rng, O.QN(int(rng.integers(-1, 2))), tuple(b_inds))
+    return A, B
+
+
+def test_plan_brute_force_and_flux_invariant():
+    rng = np.random.default_rng(2)
+    done = 0
+    while done < 25:
+        try:
+            A, B = _random_case(rng, 3, 3, int(rng.integers(1, 3)))
+        except ValueError:
+            continue
+        la, lb = O.compute_contraction_labels(A.inds, B.inds)
+        lR = O.contract_labels(la, lb)
+        offs, nnz, plan, indsR = O.contract_blockoffsets_sequential(A, la, B, lb, lR)
+        # brute force: every (b1, b2) whose contracted coordinates agree, in
+        # (b1 outer, b2 inner) order; R blocks in first-appearance order
+        brute = []
+        for b1 in A.offsets:
+            for b2 in B.offsets:
+                if all(b1[i] == b2[lb.index(x)] for i, x in enumerate(la) if x < 0):
+                    bR = tuple(b1[la.index(x)] if x in la else b2[lb.index(x)] for x in lR)
+                    brute.append((b1, b2, bR))
+        assert plan == brute
+        seen = []
+        for _, _, bR in brute:
+            if bR not in seen:
+                seen.append(bR)
+        assert list(offs) == seen
+        fA = O.flux_of_block(A.inds, next(iter(A.offsets)))
+        fB = O.flux_of_block(B.inds, next(iter(B.offsets)))
+        for bR in offs:
+            assert O.qn_eq(O.flux_of_block(indsR, bR), O.qn_add(fA, fB))
+        done += 1
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_blocksparse_values_vs_dense(cplx):
+    rng = np.random.default_rng(3)
+    done = 0
+    while done < 20:
+        try:
+            A, B = _random_case(rng, 3, 3, int(rng.integers(1, 3)))
+        except ValueError:
+            continue
+        if cplx:
+            A.data = A.data + 1j * rng.standard_normal(A.data.size)
+        la, lb = O.compute_contraction_labels(A.inds, B.inds)
+        C, plan = O.blocksparse_contract(A, la, B, lb)
+        ref = np.einsum(O._einsum_spec(la, lb, O.contract_labels(la, lb)), A.dense(), B.dense())
+        assert np.allclose(C.dense(), ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+        done += 1
+
+
+# ------------------------------------------------------------ dense contract
+def test_dense_contract_vs_loops():
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((3, 4, 5))
+    B = rng.standard_normal((5, 2, 3))
+    # contract A's mode 0 with B's mode 2 and A's mode 2 with B's mode 0
+    la, lb = (-1, 1, -2), (-2, 2, -1)
+    R = O.dense_contract(None, (2, 1), A, la, B, lb, exact=True)
+    loop = np.zeros((2, 4))
+    for a, b, c, d in itertools.product(range(3), range(4), range(5), range(2)):
+        loop[d, b] += A[a, b, c] * B[c, d, a]
+    assert np.allclose(R.astype(np.float64), loop, atol=1e-13)
+
+
+def test_dense_alpha_beta_nan():
+    """beta == 0 overwrites NaN; otherwise R = alpha*AB + beta*R
+    (NDTensors/test/test_dense.jl:237-287)."""
+    rng = np.random.default_rng(5)
+    A, B = rng.standard_normal((4, 3)), rng.standard_normal((3, 5))
+    R = np.full((4, 5), np.nan)
+    out = O.dense_contract(R, (1, 2), A, (1, -1), B, (-1, 2), 2.0, 0.0)
+    assert np.allclose(out, 2 * A @ B)
+    R = rng.standard_normal((4, 5))
+    out = O.dense_contract(R, (1, 2), A, (1, -1), B, (-1, 2), -1.0, 0.5)
+    assert np.allclose(out, -A @ B + 0.5 * R)
+
+
+def test_exact_mode_is_tighter_than_float64():
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((60, 500))
+    B = rng.standard_normal((500, 40))
+    ex = O.dense_contract(None, (1, 2), A, (1, -1), B, (-1, 2), exact=True)
+    fp = O.dense_contract(None, (1, 2), A, (1, -1), B, (-1, 2))
+    # both agree to float64 rounding, and the exact one is a longdouble result
+    assert ex.dtype == np.longdouble
+    assert relerr(fp, ex.astype(np.float64)) < 1e-14
+
+
+# ------------------------------------------------------------------ adjoint
+def test_rrule_against_finite_differences():
+    """ext/ITensorsChainRulesCoreExt/itensor.jl:94-103 checked with central
+    differences of f(x1, x2) = Re<ybar, x1*x2>."""
+    rng = np.random.default_rng(7)
+    x1 = rng.standard_normal((3, 4)) + 1j * rng.standard_normal((3, 4))
+    x2 = rng.standard_normal((4, 5))
+    l1, l2, lR = (1, -1), (-1, 2), (1, 2)
+    ybar = rng.standard_normal((3, 5)) + 1j * rng.standard_normal((3, 5))
+
+    def f(a, b):
+        y = O.dense_contract(None, lR, a, l1, b, l2)
+        return np.real(np.sum(np.conj(ybar) * y))
+
+    g1, g2 = O.rrule_contract(x1, l1, x2, l2, ybar, lR)
+    h = 1e-6
+    for idx in [(0, 0), (2, 3), (1, 2)]:
+        e = np.zeros_like(x1)
+        e[idx] = 1
+        d_re = (f(x1 + h * e, x2) - f(x1 - h * e, x2)) / (2 * h)
+        d_im = (f(x1 + 1j * h * e, x2) - f(x1 - 1j * h * e, x2)) / (2 * h)
+        # ChainRules convention: the cotangent is d_re + i d_im
+        assert abs(g1[idx] - (d_re + 1j * d_im)) < 1e-7
+    for idx in [(0, 0), (3, 4)]:
+        e = np.zeros_like(x2)
+        e[idx] = 1
+        d = (f(x1, x2 + h * e) - f(x1, x2 - h * e)) / (2 * h)
+        assert np.isreal(g2).all() and abs(g2[idx] - d) < 1e-7
+
+
+# -------------------------------------------------------- workload structure
+def test_heff_sector_sizes_and_counts():
+    """The U(1) chi = 2048 configuration: sector sizes, stored blocks of L,
+    psi, W and R, and block pairs per contraction step."""
+    qs = np.arange(-12, 13)
+    w = np.exp(-qs ** 2 / 32.0)
+    n = np.rint(2048 * w / w.sum()).astype(int)
+    n[12] = 2048 - (n.sum() - n[12])
+    assert list(n) == [2, 5, 9, 16, 28, 44, 66, 94, 124, 154, 181, 198, 206, 198, 181, 154,
+                       124, 94, 66, 44, 28, 16, 9, 5, 2]
+    bond = [(O.QN(("Sz", int(q))), int(x)) for q, x in zip(qs, n)]
+    phys = [(O.QN(("Sz", 1)), 1), (O.QN(("Sz", -1)), 1)]
+    link = [(O.QN(("Sz", 0)), 1), (O.QN(("Sz", 2)), 1), (O.QN(("Sz", -2)), 1), (O.QN(("Sz", 0)), 2)]
+    a, b = O.Index(bond, "a", dir=1), O.Index(bond, "b", dir=-1)
+    s1, s2 = O.Index(phys, "s1", dir=1), O.Index(phys, "s2", dir=1)
+    ap, bp = O.Index(bond, "a'", dir=1), O.Index(bond, "b'", dir=-1)
+    w0, w1, w2 = (O.Index(link, t, dir=-1) for t in ("w", "w'", "w''"))
+    s1p, s2p = O.Index(phys, "s1'", dir=1), O.Index(phys, "s2'", dir=1)
+    z = O.QN(("Sz", 0))
+    rng = np.random.default_rng(0)
+    L = O.random_blocksparse(rng, z, (ap, w0, a.dag()))
+    psi = O.random_blocksparse(rng, z, (a, s1, s2, b))
+    W1 = O.random_blocksparse(rng, z, (w0.dag(), s1.dag(), s1p, w1))
+    W2 = O.random_blocksparse(rng, z, (w1.dag(), s2.dag(), s2p, w2))
+    R = O.random_blocksparse(rng, z, (bp, w2.dag(), b.dag()))
+    assert [len(t.offsets) for t in (L, psi, W1, W2, R)] == [96, 96, 20, 20, 96]
+    cur, pairs = L, []
+    for T in (psi, W1, W2, R):
+        la, lb = O.compute_contraction_labels(cur.inds, T.inds)
+        offs, nnz, plan, indsR = O.contract_blockoffsets_sequential(cur, la, T, lb, O.contract_labels(la, lb))
+        pairs.append(len(plan))
+        # stand-in with the right structure for the next step
+        cur = O.BlockSparse(np.zeros(nnz), offs, indsR)
+    assert pairs == [372, 940, 952, 372]
