@@ -306,8 +306,6 @@ itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype, int NR, const i
   if (nR == 0) return ITG_OK;
   if (!R || (n1 > 0 && !T1) || (n2 > 0 && !T2)) return fail(h, ITG_ERR_INVALID, "null data pointer");
   const int cplx = dtype == ITG_C128;
-  if (cplx && prec != ITG_PREC_FP64)
-    return fail(h, ITG_ERR_UNSUPPORTED, "split precision is implemented for Float64 only");
   DeviceGuard g(h->device);
 
   // plan cache key
@@ -545,8 +543,6 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* 
   if (!R || !T1 || !T2) return fail(h, ITG_ERR_INVALID, "null data pointer");
   if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
   const int cplx = dtype == ITG_C128;
-  if (cplx && prec != ITG_PREC_FP64)
-    return fail(h, ITG_ERR_UNSUPPORTED, "split precision is implemented for Float64 only");
   DeviceGuard g(h->device);
   ExecPlan* ep = p->ep.get();
   if (pair_alpha) {
@@ -651,8 +647,11 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   int best = -1;
   float best_ms = 1e30f;
   std::string err;
-  for (int cfg = 0; cfg < gemm_num_cfgs(); ++cfg) {
-    if (!gemm_cfg_ok(cfg, cplx)) continue;
+  std::vector<int> cands;
+  for (int c = 0; c < gemm_num_cfgs(); ++c)
+    if (gemm_cfg_ok(c, cplx)) cands.push_back(c);
+  cands.push_back(kCfgMixed);
+  for (int cfg : cands) {
     // warm-up, then 3 timed runs
     e = exec_run(ep, cplx, (int)prec, cfg, h->stream, h->ws, T1, T2, R, 0u, 1.0, 0.0, 0.0, 0.0, &err);
     if (e != hipSuccess) break;

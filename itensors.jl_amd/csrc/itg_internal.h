@@ -40,7 +40,7 @@ struct DirectGroup {
   int32_t cdim[kMaxModes];
   int32_t nelem;
   int32_t p0, p1;  // pair range
-  int32_t pad;
+  int32_t tiled;   // 1: 2-D tiles (lanes along R mode 0, one column per wave)
 };
 
 // Grouped GEMM: C (M x N, column-major, ldc = M) of a group is
@@ -83,6 +83,9 @@ hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pair
                          const void* A, const void* B, void* C, unsigned flags,
                          double ga_re, double ga_im, double beta_re, double beta_im);
 constexpr int kDirectElemsPerWG = 256 * 2;
+// Workgroups a direct-contraction group needs; sets g->tiled (host helper
+// that must agree with the kernel's tiling).
+int64_t direct_group_tiles(DirectGroup* g, int nr);
 
 // Grouped GEMM (FP64 MFMA). cfg selects the tile shape; ta/tb: A stored
 // K x M / B stored N x K. `tiles` holds one {group, tile index in group} per
@@ -126,12 +129,22 @@ struct SplitPair {
 
 int split_slices(int prec);  // slices per value for a precision mode (0: none)
 int64_t slice_block_wgs(int32_t rows);
-hipError_t launch_slice(hipStream_t s, int prec, const SliceBlock* blocks, const int64_t* prefix,
-                        int64_t nblk, int64_t grid, const void* src, void* planes, void* exps);
+// Complex operands: nsets = 2 slice sets (re, im) for the A side, 3 (re, im,
+// -im) for the B side; plane buffers hold nsets x the per-set bytes.
+hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
+                        const int64_t* prefix, int64_t nblk, int64_t grid, const void* src,
+                        void* planes, void* exps);
 // Tile shape of the split GEMM: 64 x 64 (the tile table uses gemm cfg 0).
-hipError_t launch_split_gemm(hipStream_t s, int prec, const SplitPair* pairs,
-                             const GemmGroup* groups, const int2* tiles, int64_t grid,
-                             const void* planesA, const void* planesB, const void* expA,
-                             const void* expB, void* C, double ga, double beta);
+// Complex: one launch per output component comp (0 = re, 1 = im); `sets`
+// packs the slice sets (0 re, 1 im, 2 -im) of A and B for the two K halves:
+// bits 0-1 A half 0, 2-3 B half 0, 4-5 A half 1, 6-7 B half 1. Both complex
+// operands are sliced with 3 sets.
+hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int comp, int sets,
+                             const SplitPair* pairs, const GemmGroup* groups, const int2* tiles,
+                             int64_t grid, const void* planesA, const void* planesB,
+                             const void* expA, const void* expB, void* C, double ga,
+                             double beta);
+// x[i] = beta * x[i] for i < n (complex when cplx).
+hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re, double b_im);
 
 }  // namespace itg

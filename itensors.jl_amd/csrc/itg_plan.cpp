@@ -324,7 +324,7 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
       y.nelem = (int32_t)ne;
       y.p0 = (int32_t)granges[g].first;
       y.p1 = (int32_t)granges[g].second;
-      prefix[g + 1] = prefix[g] + (ne + kDirectElemsPerWG - 1) / kDirectElemsPerWG;
+      prefix[g + 1] = prefix[g] + direct_group_tiles(&y, NR);
     }
     if (koffs.empty()) koffs.push_back(0), koffs.push_back(0);
     ep->d_ngroups = (int64_t)granges.size();
@@ -614,6 +614,8 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
     x.b_exp = (int32_t)dB[bq].exp_off;
     x.kpad = dA[bp].K == 0 ? 0 : dA[bp].Kpad;
     x.alpha = ep->h_pairs[p].alpha_re;
+    if (ep->h_pairs[p].alpha_im != 0.0) ep->cplx_pair_alpha = true;
+    ep->split_kmax = std::max<int64_t>(ep->split_kmax, dA[bp].K);
   }
   hipError_t e;
   if ((e = ep->sA_desc.upload(dA.data(), dA.size() * sizeof(SliceBlock), s)) != hipSuccess ||
@@ -642,16 +644,31 @@ int default_gemm_cfg(const ExecPlan& ep, int cplx) {
   return (tot > 0 && big >= 0.5 * tot) ? 1 : 0;
 }
 
+int mixed_big_cfg(int cplx) { return cplx ? 2 : 1; }
+
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
-  if (!gemm_cfg_ok(cfg, cplx)) cfg = 0;
+  if (cfg != kCfgMixed && !gemm_cfg_ok(cfg, cplx)) cfg = 0;
   auto it = ep.launches.find(cfg);
   if (it != ep.launches.end()) return hipSuccess;
   std::unique_ptr<GemmLaunch> gl(new GemmLaunch);
   gl->cfg = cfg;
-  std::vector<int2> table = xcd_tile_table(ep, gemm_cfg(cfg));
-  gl->grid = (int64_t)table.size();
-  hipError_t e = gl->tiles.upload(table.data(), table.size() * sizeof(int2), s);
-  if (e != hipSuccess) return e;
+  hipError_t e;
+  if (cfg == kCfgMixed) {
+    const int big = mixed_big_cfg(cplx);
+    std::vector<int2> t1 = xcd_tile_table(ep, gemm_cfg(big), 1);
+    std::vector<int2> t2 = xcd_tile_table(ep, gemm_cfg(0), 2);
+    gl->cfg = big;
+    gl->grid = (int64_t)t1.size();
+    gl->cfg2 = 0;
+    gl->grid2 = (int64_t)t2.size();
+    if ((e = gl->tiles.upload(t1.data(), t1.size() * sizeof(int2), s)) != hipSuccess) return e;
+    if ((e = gl->tiles2.upload(t2.data(), t2.size() * sizeof(int2), s)) != hipSuccess) return e;
+  } else {
+    std::vector<int2> table = xcd_tile_table(ep, gemm_cfg(cfg));
+    gl->grid = (int64_t)table.size();
+    if ((e = gl->tiles.upload(table.data(), table.size() * sizeof(int2), s)) != hipSuccess)
+      return e;
+  }
   ep.launches[cfg] = std::move(gl);
   return hipSuccess;
 }
@@ -662,7 +679,7 @@ hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
 // largest-work-first to the least loaded one, and interleave the per-XCD
 // lists so each XCD runs its own chunks back to back: the tiles of a chunk
 // share their A and B panels through that XCD's L2.
-std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc) {
+std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc, int filter) {
   constexpr int kXcds = 8;
   struct Chunk {
     int32_t g, t0, t1;
@@ -673,7 +690,11 @@ std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc) {
   std::vector<int32_t> gt(ep.h_groups.size());
   for (size_t g = 0; g < ep.h_groups.size(); ++g) {
     const GemmGroup& G = ep.h_groups[g];
-    gt[g] = (G.M <= 0 || G.N <= 0) ? 0 : ((G.M + tc.bm - 1) / tc.bm) * ((G.N + tc.bn - 1) / tc.bn);
+    const bool big = G.M >= tc.bm && G.N >= tc.bn;
+    const bool keep = filter == 0 || (filter == 1 && big) || (filter == 2 && !big);
+    gt[g] = (!keep || G.M <= 0 || G.N <= 0)
+                ? 0
+                : ((G.M + tc.bm - 1) / tc.bm) * ((G.N + tc.bn - 1) / tc.bn);
     total += gt[g];
   }
   const int64_t unit = std::max<int64_t>(1, (total + 8 * kXcds - 1) / (8 * kXcds));
@@ -712,8 +733,9 @@ size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec) {
   if (ep.strategy != ExecPlan::GEMM) return 0;
   const size_t es = cplx ? 16 : 8;
   size_t b = 0;
-  if (prec != 0 && ep.split_ok && !cplx) {
-    b += align256((size_t)ep.sA_plane_bytes) + align256((size_t)ep.sB_plane_bytes);
+  if (prec != 0 && ep.split_ok) {
+    const size_t ns = cplx ? 3 : 1;
+    b += align256((size_t)ep.sA_plane_bytes * ns) + align256((size_t)ep.sB_plane_bytes * ns);
     b += align256((size_t)ep.sA_rows * 4) + align256((size_t)ep.sB_rows * 4);
   } else {
     if (ep.permA) b += align256((size_t)ep.nnzA * es);
@@ -725,8 +747,23 @@ size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec) {
 
 namespace {
 
-hipError_t exec_split(ExecPlan& ep, int prec, hipStream_t s, void* ws, const void* T1,
-                      const void* T2, void* R, double a_re, double b_re, std::string* err) {
+// Slice sets of the two K halves of a complex split product (see
+// launch_split_gemm): Re C = Re A Re B - Im A Im B, Im C = Re A Im B + Im A Re B,
+// with Im A -> -Im A under conj(A) and Im B -> -Im B under conj(B).
+int split_sets(int comp, bool conja, bool conjb) {
+  const int ima = conja ? 2 : 1, imb = conjb ? 2 : 1, nimb = conjb ? 1 : 2;
+  int a0, b0, a1, b1;
+  if (comp == 0) {
+    a0 = 0, b0 = 0, a1 = ima, b1 = nimb;
+  } else {
+    a0 = 0, b0 = imb, a1 = ima, b1 = 0;
+  }
+  return a0 | (b0 << 2) | (a1 << 4) | (b1 << 6);
+}
+
+hipError_t exec_split(ExecPlan& ep, int cplx, int prec, hipStream_t s, void* ws, const void* T1,
+                      const void* T2, void* R, unsigned flags, double a_re, double a_im,
+                      double b_re, double b_im, std::string* err) {
   if (!ep.split_ok) {
     *err = "split precision is unavailable for this contraction: " + ep.split_why;
     return hipErrorNotSupported;
@@ -735,23 +772,34 @@ hipError_t exec_split(ExecPlan& ep, int prec, hipStream_t s, void* ws, const voi
     *err = "this split-precision form is not available";
     return hipErrorNotSupported;
   }
+  if (cplx && (a_im != 0.0 || ep.cplx_pair_alpha)) {
+    *err = "complex split precision takes a real alpha";
+    return hipErrorNotSupported;
+  }
+  if (cplx && ep.split_kmax > 32768) {  // the two K halves share one INT32 sum
+    *err = "complex split precision needs a contracted extent <= 32768 per pair";
+    return hipErrorNotSupported;
+  }
+  const int nsets = cplx ? 3 : 1;
   const void* A = ep.swap ? T2 : T1;
   const void* B = ep.swap ? T1 : T2;
+  const bool conja = (flags & (ep.swap ? 2u : 1u)) != 0;
+  const bool conjb = (flags & (ep.swap ? 1u : 2u)) != 0;
   char* w = (char*)ws;
   void* planesA = w;
-  w += align256((size_t)ep.sA_plane_bytes);
+  w += align256((size_t)ep.sA_plane_bytes * nsets);
   void* planesB = w;
-  w += align256((size_t)ep.sB_plane_bytes);
+  w += align256((size_t)ep.sB_plane_bytes * nsets);
   void* expA = w;
   w += align256((size_t)ep.sA_rows * 4);
   void* expB = w;
   w += align256((size_t)ep.sB_rows * 4);
-  hipError_t e = launch_slice(s, prec, (const SliceBlock*)ep.sA_desc.p,
+  hipError_t e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.sA_desc.p,
                               (const int64_t*)ep.sA_prefix.p, ep.sA_blocks, ep.sA_grid, A,
                               planesA, expA);
   if (e == hipSuccess)
-    e = launch_slice(s, prec, (const SliceBlock*)ep.sB_desc.p, (const int64_t*)ep.sB_prefix.p,
-                     ep.sB_blocks, ep.sB_grid, B, planesB, expB);
+    e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.sB_desc.p,
+                     (const int64_t*)ep.sB_prefix.p, ep.sB_blocks, ep.sB_grid, B, planesB, expB);
   if (e != hipSuccess) {
     *err = std::string("slice launch: ") + hipGetErrorString(e);
     return e;
@@ -762,17 +810,29 @@ hipError_t exec_split(ExecPlan& ep, int prec, hipStream_t s, void* ws, const voi
   }
   GemmLaunch& gl = *ep.launches[0];
   void* C = ep.permC ? (void*)w : R;
-  e = launch_split_gemm(s, prec, (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.g_groups.p,
-                        (const int2*)gl.tiles.p, gl.grid, planesA, planesB, expA, expB, C, a_re,
-                        ep.permC ? 0.0 : b_re);
+  double beta_k = ep.permC ? 0.0 : b_re;
+  if (cplx && !ep.permC && b_im != 0.0) {
+    // complex beta mixes the components: scale R first, then accumulate
+    e = launch_scale(s, 1, R, ep.nnzC, b_re, b_im);
+    if (e != hipSuccess) {
+      *err = std::string("scale launch: ") + hipGetErrorString(e);
+      return e;
+    }
+    beta_k = 1.0;
+  }
+  for (int comp = 0; comp < (cplx ? 2 : 1) && e == hipSuccess; ++comp)
+    e = launch_split_gemm(s, prec, cplx, comp, split_sets(comp, conja, conjb),
+                          (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.g_groups.p,
+                          (const int2*)gl.tiles.p, gl.grid, planesA, planesB, expA, expB, C, a_re,
+                          beta_k);
   if (e != hipSuccess) {
     *err = std::string("split gemm launch: ") + hipGetErrorString(e);
     return e;
   }
   if (ep.permC) {
-    e = launch_permute(s, 0, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,
+    e = launch_permute(s, cplx, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,
                        (const int64_t*)ep.pC.prefix.p, ep.pC.nblk, ep.pC.tiles, ep.pC.kind, C, R,
-                       1.0, 0.0, b_re, 0.0);
+                       1.0, 0.0, b_re, b_im);
     if (e != hipSuccess) *err = std::string("permute launch: ") + hipGetErrorString(e);
   }
   return e;
@@ -792,13 +852,8 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     if (e != hipSuccess) *err = std::string("direct kernel launch: ") + hipGetErrorString(e);
     return e;
   }
-  if (prec != 0) {
-    if (cplx) {
-      *err = "split precision is implemented for Float64 only";
-      return hipErrorNotSupported;
-    }
-    return exec_split(ep, prec, s, ws, T1, T2, R, a_re, b_re, err);
-  }
+  if (prec != 0)
+    return exec_split(ep, cplx, prec, s, ws, T1, T2, R, flags, a_re, a_im, b_re, b_im, err);
   const size_t es = cplx ? 16 : 8;
   char* w = (char*)ws;
   const void* A = ep.swap ? T2 : T1;
@@ -837,11 +892,15 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     *err = std::string("gemm table upload: ") + hipGetErrorString(e);
     return e;
   }
-  int ecfg = gemm_cfg_ok(cfg, cplx) ? cfg : 0;
+  int ecfg = (cfg == kCfgMixed || gemm_cfg_ok(cfg, cplx)) ? cfg : 0;
   GemmLaunch& gl = *ep.launches[ecfg];
-  e = launch_gemm_f64(s, ecfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
+  e = launch_gemm_f64(s, gl.cfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
                       (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B, C,
                       gflags, a_re, a_im, gbr, gbi);
+  if (e == hipSuccess && gl.cfg2 >= 0)
+    e = launch_gemm_f64(s, gl.cfg2, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
+                        (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles2.p, gl.grid2, A, B,
+                        C, gflags, a_re, a_im, gbr, gbi);
   if (e != hipSuccess) {
     *err = std::string("gemm launch: ") + hipGetErrorString(e);
     return e;

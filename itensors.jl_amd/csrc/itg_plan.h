@@ -74,10 +74,17 @@ struct PermuteLaunch {
   DevBuf blocks, prefix;
 };
 
+// Tile-shape choice "mixed": groups whose output block fills large tiles
+// use the large shape, the others 64x64, as two grouped launches.
+constexpr int kCfgMixed = 16;
+
 struct GemmLaunch {
   int cfg = -1;
   int64_t grid = 0;  // workgroups, including idle padding slots
   DevBuf tiles;      // int2 {group, tile} per workgroup
+  int cfg2 = -1;     // second launch of a mixed choice (64x64 part)
+  int64_t grid2 = 0;
+  DevBuf tiles2;
 };
 
 struct ExecPlan {
@@ -101,7 +108,9 @@ struct ExecPlan {
   // split precision (real data): slice tables of the A-side operand P and
   // the B-side operand Q, read in place (no permutation), and the pair table
   bool split_ok = false;
-  std::string split_why;  // reason when !split_ok
+  bool cplx_pair_alpha = false;  // some per-pair alpha has an imaginary part
+  int64_t split_kmax = 0;        // largest contracted extent of a pair
+  std::string split_why;         // reason when !split_ok
   int64_t sA_blocks = 0, sA_grid = 0, sB_blocks = 0, sB_grid = 0;
   int64_t sA_plane_bytes = 0, sB_plane_bytes = 0, sA_rows = 0, sB_rows = 0;
   DevBuf sA_desc, sA_prefix, sB_desc, sB_prefix, s_pairs;
@@ -134,6 +143,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
 int default_gemm_cfg(const ExecPlan& ep, int cplx);
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s);
 // XCD-balanced workgroup -> {group, tile} table for tile shape tc.
-std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc);
+// filter: 0 = every group, 1 = only groups with M, N >= the tile, 2 = the rest.
+std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc, int filter = 0);
 
 }  // namespace itg

@@ -90,23 +90,28 @@ __global__ void __launch_bounds__(kThreads)
     const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
     const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
 
+    // Loads are unconditional from clamped (in-block) addresses so that they
+    // issue back to back and their latency overlaps the MFMAs; out-of-range
+    // elements are zeroed, and alpha / conj applied, when the registers are
+    // written to LDS.
+    unsigned ina = 0u, inb = 0u;
     auto load_regs = [&](int k0) {
+      ina = inb = 0u;
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
         const int idx = tid + r * kThreads;
         const int m = TA ? idx / BK : idx % BM;
         const int k = TA ? idx % BK : idx / BM;
         const int gm = m0 + m, gk = k0 + k;
-        const bool in = gm < M && gk < K;
-        const int64_t off = TA ? ((int64_t)gm * pr.lda + gk) : ((int64_t)gk * pr.lda + gm);
+        if (gm < M && gk < K) ina |= 1u << r;
+        const int gmc = min(gm, M - 1), gkc = min(gk, K - 1);
+        const int64_t off = TA ? ((int64_t)gmc * pr.lda + gkc) : ((int64_t)gkc * pr.lda + gmc);
         if (CPLX) {
-          double2 v = in ? reinterpret_cast<const double2*>(Ag)[pr.off_a + off]
-                         : make_double2(0.0, 0.0);
-          v.y *= sa;
-          ra[0][r] = al_re * v.x - al_im * v.y;
-          ra[NPLANE - 1][r] = al_re * v.y + al_im * v.x;
+          const double2 v = reinterpret_cast<const double2*>(Ag)[pr.off_a + off];
+          ra[0][r] = v.x;
+          ra[NPLANE - 1][r] = v.y;
         } else {
-          ra[0][r] = in ? al_re * Ag[pr.off_a + off] : 0.0;
+          ra[0][r] = Ag[pr.off_a + off];
         }
       }
 #pragma unroll
@@ -115,15 +120,15 @@ __global__ void __launch_bounds__(kThreads)
         const int n = TB ? idx % BN : idx / BK;
         const int k = TB ? idx / BN : idx % BK;
         const int gn = n0 + n, gk = k0 + k;
-        const bool in = gn < N && gk < K;
-        const int64_t off = TB ? ((int64_t)gk * pr.ldb + gn) : ((int64_t)gn * pr.ldb + gk);
+        if (gn < N && gk < K) inb |= 1u << r;
+        const int gnc = min(gn, N - 1), gkc = min(gk, K - 1);
+        const int64_t off = TB ? ((int64_t)gkc * pr.ldb + gnc) : ((int64_t)gnc * pr.ldb + gkc);
         if (CPLX) {
-          double2 v = in ? reinterpret_cast<const double2*>(Bg)[pr.off_b + off]
-                         : make_double2(0.0, 0.0);
+          const double2 v = reinterpret_cast<const double2*>(Bg)[pr.off_b + off];
           rb[0][r] = v.x;
-          rb[NPLANE - 1][r] = sb * v.y;
+          rb[NPLANE - 1][r] = v.y;
         } else {
-          rb[0][r] = in ? Bg[pr.off_b + off] : 0.0;
+          rb[0][r] = Bg[pr.off_b + off];
         }
       }
     };
@@ -133,16 +138,24 @@ __global__ void __launch_bounds__(kThreads)
         const int idx = tid + r * kThreads;
         const int m = TA ? idx / BK : idx % BM;
         const int k = TA ? idx % BK : idx / BM;
-#pragma unroll
-        for (int c = 0; c < NPLANE; ++c) As[buf][c][k][m] = ra[c][r];
+        const bool in = (ina >> r) & 1u;
+        if (CPLX) {
+          const double x = in ? ra[0][r] : 0.0;
+          const double y = in ? sa * ra[NPLANE - 1][r] : 0.0;
+          As[buf][0][k][m] = al_re * x - al_im * y;
+          As[buf][NPLANE - 1][k][m] = al_re * y + al_im * x;
+        } else {
+          As[buf][0][k][m] = in ? al_re * ra[0][r] : 0.0;
+        }
       }
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
         const int idx = tid + r * kThreads;
         const int n = TB ? idx % BN : idx / BK;
         const int k = TB ? idx / BN : idx % BK;
-#pragma unroll
-        for (int c = 0; c < NPLANE; ++c) Bs[buf][c][k][n] = rb[c][r];
+        const bool in = (inb >> r) & 1u;
+        Bs[buf][0][k][n] = in ? rb[0][r] : 0.0;
+        if (CPLX) Bs[buf][NPLANE - 1][k][n] = in ? sb * rb[NPLANE - 1][r] : 0.0;
       }
     };
 

@@ -188,7 +188,26 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+template <int CPLX>
+__global__ void __launch_bounds__(kThreads)
+    scale_kernel(typename Elem<CPLX>::T* __restrict__ x, int64_t n, double b_re, double b_im) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  typename Elem<CPLX>::T y{};
+  x[i] = axpby(b_re, b_im, x[i], 0.0, 0.0, y, false);
+}
+
 }  // namespace
+
+hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re, double b_im) {
+  if (n <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + kThreads - 1) / kThreads)), block(kThreads);
+  if (cplx)
+    hipLaunchKernelGGL(scale_kernel<1>, grid, block, 0, s, (double2*)x, n, b_re, b_im);
+  else
+    hipLaunchKernelGGL(scale_kernel<0>, grid, block, 0, s, (double*)x, n, b_re, b_im);
+  return hipGetLastError();
+}
 
 int64_t permute_block_tiles(int nm, const int32_t* perm, const int32_t* dims, int mode_kind) {
   int64_t nelem = 1;

@@ -65,11 +65,28 @@ __device__ inline int64_t mode_offset(uint32_t idx, int n, const int32_t* dim,
   return o;
 }
 
-template <int S>
+// Magnitude used for the row scale: |x| (real) or max(|re|, |im|) (complex,
+// whose two parts share one scale per row so that their slice products land
+// on the same INT32 grid).
+template <int CPLX>
+__device__ inline double elem_mag(const double* x, int64_t o) {
+  if (CPLX) {
+    const double2 v = reinterpret_cast<const double2*>(x)[o];
+    const double m = fmax(fabs(v.x), fabs(v.y));
+    return (isfinite(v.x) && isfinite(v.y)) ? m : INFINITY;
+  }
+  const double v = x[o];
+  return isfinite(v) ? fabs(v) : INFINITY;
+}
+
+// nsets: slice sets written per element -- 1 (real), 2 (re, im: A side of a
+// complex product), 3 (re, im, -im: B side); set s of block b starts at
+// planes + nsets*plane_off + s*S*R*Kpad.
+template <int S, int CPLX>
 __global__ void __launch_bounds__(kThreads)
     slice_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
                  int64_t nblk, const double* __restrict__ src, int8_t* __restrict__ planes,
-                 int32_t* __restrict__ exps) {
+                 int32_t* __restrict__ exps, int nsets) {
   __shared__ double red[4][kRowsPerWG];
   __shared__ int32_t sexp[kRowsPerWG];
   const int64_t wg = blockIdx.x;
@@ -77,7 +94,7 @@ __global__ void __launch_bounds__(kThreads)
   const SliceBlock& d = blocks[b];
   const int r0 = (int)(wg - prefix[b]) * kRowsPerWG;
   const int tid = threadIdx.x;
-  const double* x = src + d.off;
+  const double* x = src + (CPLX ? 2 : 1) * d.off;
 
   // The source is contiguous either along the rows (A stored M-fastest) or
   // along K (A stored K-fastest, B in N form); threads are laid along the
@@ -90,11 +107,8 @@ __global__ void __launch_bounds__(kThreads)
     double m = 0.0;
     if (r0 + row < d.R) {
       const int64_t ro = mode_offset((uint32_t)(r0 + row), d.nrm, d.rdim, d.rstr);
-      for (int k = part; k < d.K; k += 4) {
-        const double v = x[ro + mode_offset((uint32_t)k, d.nkm, d.kdim, d.kstr)];
-        m = fmax(m, fabs(v));
-        if (!isfinite(v)) m = INFINITY;
-      }
+      for (int k = part; k < d.K; k += 4)
+        m = fmax(m, elem_mag<CPLX>(x, ro + mode_offset((uint32_t)k, d.nkm, d.kdim, d.kstr)));
     }
     red[part][row] = m;
   } else {
@@ -103,12 +117,8 @@ __global__ void __launch_bounds__(kThreads)
     for (int row = wave; row < kRowsPerWG; row += 4) {
       double m = 0.0;
       if (r0 + row < d.R) {
-        const double* xr = x + mode_offset((uint32_t)(r0 + row), d.nrm, d.rdim, d.rstr);
-        for (int k = lane; k < d.K; k += 64) {
-          const double v = xr[k];
-          m = fmax(m, fabs(v));
-          if (!isfinite(v)) m = INFINITY;
-        }
+        const int64_t ro = mode_offset((uint32_t)(r0 + row), d.nrm, d.rdim, d.rstr);
+        for (int k = lane; k < d.K; k += 64) m = fmax(m, elem_mag<CPLX>(x, ro + k));
       }
 #pragma unroll
       for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s));
@@ -162,16 +172,27 @@ __global__ void __launch_bounds__(kThreads)
         ko += (int64_t)kc[i] * d.kstr[i];
       }
     }
-    uint32_t packed[S][4];
+    constexpr int NC = CPLX ? 3 : 1;  // re, im, -im
+    uint32_t packed[NC][S][4];
 #pragma unroll
-    for (int i = 0; i < S; ++i)
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int w = 0; w < 4; ++w) packed[i][w] = 0u;
+      for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) packed[c][i][w] = 0u;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int k = ch * 16 + j;
-      double y = 0.0;
-      if (k < d.K && e != kNonFinite) y = ldexp(x[ro + ko], 6 - e);
+      double y = 0.0, yi = 0.0;
+      if (k < d.K && e != kNonFinite) {
+        if (CPLX) {
+          const double2 v = reinterpret_cast<const double2*>(x)[ro + ko];
+          y = ldexp(v.x, 6 - e);
+          yi = ldexp(v.y, 6 - e);
+        } else {
+          y = ldexp(x[ro + ko], 6 - e);
+        }
+      }
       // advance the K coordinate by one (mode 0 fastest)
       ko += d.kstr[0];
       if (++kc[0] == d.kdim[0]) {
@@ -190,26 +211,43 @@ __global__ void __launch_bounds__(kThreads)
       for (int i = 0; i < S; ++i) {
         const double v = rint(y);
         y = (y - v) * 128.0;
-        packed[i][j >> 2] |= ((uint32_t)((int32_t)v & 0xff)) << (8 * (j & 3));
+        packed[0][i][j >> 2] |= ((uint32_t)((int32_t)v & 0xff)) << (8 * (j & 3));
+        if (CPLX) {
+          const double w = rint(yi);
+          yi = (yi - w) * 128.0;
+          packed[1][i][j >> 2] |= ((uint32_t)((int32_t)w & 0xff)) << (8 * (j & 3));
+          packed[NC - 1][i][j >> 2] |= ((uint32_t)((int32_t)(-w) & 0xff)) << (8 * (j & 3));
+        }
       }
     }
-    int8_t* dst = planes + d.plane_off + (int64_t)gr * d.Kpad + ch * 16;
+    int8_t* dst = planes + (int64_t)nsets * d.plane_off + (int64_t)gr * d.Kpad + ch * 16;
 #pragma unroll
-    for (int i = 0; i < S; ++i)
-      *reinterpret_cast<v4i*>(dst + i * pstride) =
-          v4i{(int)packed[i][0], (int)packed[i][1], (int)packed[i][2], (int)packed[i][3]};
+    for (int c = 0; c < NC; ++c) {
+      if (c >= nsets) break;
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+        *reinterpret_cast<v4i*>(dst + (c * S + i) * pstride) = v4i{
+            (int)packed[c][i][0], (int)packed[c][i][1], (int)packed[c][i][2], (int)packed[c][i][3]};
+    }
   }
 }
 
 constexpr int BM = 64, BN = 64, BKB = 32;  // BKB: K bytes per LDS stage
 
-template <int S>
-__global__ void __launch_bounds__(kThreads, 2)
+// Complex data (CPLX = 1): one launch per output component `comp`; the
+// component is a K-stacked real product over two halves, e.g.
+//   Re C = [Re A | Im A] . [Re B ; -Im B],  Im C = [Re A | Im A] . [Im B ; Re B],
+// with the slice sets of each half packed in `sets` (bits 0-1 / 2-3: A / B
+// set of half 0, bits 4-5 / 6-7: half 1), accumulated into the same INT32
+// levels (both halves share the row and column scales), and written to the
+// interleaved complex output.
+template <int S, int CPLX>
+__global__ void __launch_bounds__(kThreads, CPLX ? 1 : 2)
     split_gemm_kernel(const SplitPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
                       const int2* __restrict__ tiles, const int8_t* __restrict__ planesA,
                       const int8_t* __restrict__ planesB, const int32_t* __restrict__ expA,
                       const int32_t* __restrict__ expB, double* __restrict__ Cg, double ga,
-                      double beta) {
+                      double beta, int comp, int sets, int nsetsA, int nsetsB) {
   constexpr int NL = S;  // levels 2..S+1
   constexpr int CH_A = S * BM * 2;  // 16-byte chunks per stage
   constexpr int CH_B = S * BN * 2;
@@ -236,8 +274,8 @@ __global__ void __launch_bounds__(kThreads, 2)
     const SplitPair pr = pairs[p];
     const int kpad = pr.kpad;
     if (kpad <= 0) continue;
-    const int8_t* pa = planesA + pr.a_planes;
-    const int8_t* pb = planesB + pr.b_planes;
+    const int8_t* pa = planesA;
+    const int8_t* pb = planesB;
     v16i lev[NL];
 #pragma unroll
     for (int L = 0; L < NL; ++L) lev[L] = v16i{0};
@@ -276,6 +314,19 @@ __global__ void __launch_bounds__(kThreads, 2)
       }
     };
 
+    for (int h = 0; h < (CPLX ? 2 : 1); ++h) {
+    // slice sets (0 = re, 1 = im, 2 = -im) of this half, chosen by the host
+    // for the output component and the conj flags
+    if (CPLX) {
+      const int seta = h == 0 ? (sets & 3) : ((sets >> 4) & 3);
+      const int setb = h == 0 ? ((sets >> 2) & 3) : ((sets >> 6) & 3);
+      pa = planesA + (int64_t)nsetsA * pr.a_planes + (int64_t)seta * S * pr.a_pstride;
+      pb = planesB + (int64_t)nsetsB * pr.b_planes + (int64_t)setb * S * pr.b_pstride;
+    } else {
+      pa = planesA + pr.a_planes;
+      pb = planesB + pr.b_planes;
+    }
+    // every wave is past the previous half's last barrier: buffer 0 is free
     load(0);
     store(0);
     __syncthreads();
@@ -304,6 +355,7 @@ __global__ void __launch_bounds__(kThreads, 2)
       __syncthreads();
       buf ^= 1;
     }
+    }  // halves
 
     // FP64 recombination of the levels, scaled per row/column
     const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
@@ -325,7 +377,8 @@ __global__ void __launch_bounds__(kThreads, 2)
   for (int r = 0; r < 16; ++r) {
     const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
     if (gn >= grp.N) continue;
-    const int64_t o = grp.off_c + gm + (int64_t)gn * grp.ldc;
+    const int64_t o0 = grp.off_c + gm + (int64_t)gn * grp.ldc;
+    const int64_t o = CPLX ? 2 * o0 + comp : o0;
     double out = acc[r];
     if (beta != 0.0) out = fma(beta, Cg[o], out);
     Cg[o] = out;
@@ -336,26 +389,41 @@ __global__ void __launch_bounds__(kThreads, 2)
 
 int split_slices(int prec) { return prec == 2 ? kSliceCount : 0; }
 
-hipError_t launch_slice(hipStream_t s, int prec, const SliceBlock* blocks, const int64_t* prefix,
-                        int64_t nblk, int64_t grid, const void* src, void* planes, void* exps) {
+hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
+                        const int64_t* prefix, int64_t nblk, int64_t grid, const void* src,
+                        void* planes, void* exps) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
-  hipLaunchKernelGGL(slice_kernel<kSliceCount>, dim3((unsigned)grid), dim3(kThreads), 0, s, blocks,
-                     prefix, nblk, (const double*)src, (int8_t*)planes, (int32_t*)exps);
+  if (cplx)
+    hipLaunchKernelGGL((slice_kernel<kSliceCount, 1>), dim3((unsigned)grid), dim3(kThreads), 0, s,
+                       blocks, prefix, nblk, (const double*)src, (int8_t*)planes, (int32_t*)exps,
+                       nsets);
+  else
+    hipLaunchKernelGGL((slice_kernel<kSliceCount, 0>), dim3((unsigned)grid), dim3(kThreads), 0, s,
+                       blocks, prefix, nblk, (const double*)src, (int8_t*)planes, (int32_t*)exps,
+                       1);
   return hipGetLastError();
 }
 
 int64_t slice_block_wgs(int32_t rows) { return (rows + kRowsPerWG - 1) / kRowsPerWG; }
 
-hipError_t launch_split_gemm(hipStream_t s, int prec, const SplitPair* pairs,
-                             const GemmGroup* groups, const int2* tiles, int64_t grid,
-                             const void* planesA, const void* planesB, const void* expA,
-                             const void* expB, void* C, double ga, double beta) {
+hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int comp, int sets,
+                             const SplitPair* pairs, const GemmGroup* groups, const int2* tiles,
+                             int64_t grid, const void* planesA, const void* planesB,
+                             const void* expA, const void* expB, void* C, double ga,
+                             double beta) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
-  hipLaunchKernelGGL(split_gemm_kernel<kSliceCount>, dim3((unsigned)grid), dim3(kThreads), 0, s,
-                     pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
-                     (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta);
+  if (cplx)
+    hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, 1>), dim3((unsigned)grid), dim3(kThreads),
+                       0, s, pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
+                       (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta, comp,
+                       sets, 3, 3);
+  else
+    hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, 0>), dim3((unsigned)grid), dim3(kThreads),
+                       0, s, pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
+                       (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta, 0, 0, 1,
+                       1);
   return hipGetLastError();
 }
 

@@ -92,6 +92,56 @@ def test_split_row_scales_zero_rows_alpha_beta(env):
     assert err < SPLIT_RTOL, err
 
 
+def test_split_complex_orientations_conj_and_beta(env):
+    """ComplexF64: Re/Im slices share a per-row scale; both output components
+    come from K-stacked real slice products (conj folded into the slice-set
+    choice)."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(15)
+    M, N, K = 45, 70, 130
+    for ta, tb in itertools.product([0, 1], repeat=2):
+        A = rand(rng, (K, M) if ta else (M, K), True)
+        B = rand(rng, (N, K) if tb else (K, N), True)
+        lA = (-1, 1) if ta else (1, -1)
+        lB = (2, -1) if tb else (-1, 2)
+        got = run(env, A, lA, B, lB, (1, 2))
+        ref = O.dense_contract(None, (1, 2), A, lA, B, lB, exact=True)
+        err = relerr(got, ref)
+        assert err < SPLIT_RTOL, (err, ta, tb)
+        for c1, c2 in [(True, False), (False, True), (True, True)]:
+            TA, TB = ND.DenseTensor.from_numpy(A), ND.DenseTensor.from_numpy(B)
+            T = ND.contract(TA, lA, TB, lB, (1, 2), prec=PREC, conj1=c1, conj2=c2)
+            torch.cuda.synchronize()
+            ref = O.dense_contract(None, (1, 2), np.conj(A) if c1 else A, lA,
+                                   np.conj(B) if c2 else B, lB, exact=True)
+            err = relerr(T.to_numpy(), ref)
+            assert err < SPLIT_RTOL, (err, ta, tb, c1, c2)
+    A = rand(rng, (40, 96), True)
+    B = rand(rng, (96, 33), True)
+    R1 = rand(rng, (40, 33), True)
+    for be in (0.5, 0.25 - 0.75j):
+        got = run(env, A, (1, -1), B, (-1, 2), (1, 2), alpha=2.0, beta=be, R0=R1)
+        ref = O.dense_contract(R1, (1, 2), A, (1, -1), B, (-1, 2), 2.0, be, exact=True)
+        err = relerr(got, ref)
+        assert err < SPLIT_RTOL, (err, be)
+
+
+def test_split_complex_heff_network(env):
+    torch, itg, ND, O = env
+    from itg.configs import heff_network
+    for qns, chi in [(False, 96), (True, 256)]:
+        L, psi, W1, W2, R, ix = heff_network(chi, qns=qns, seed=2, dtype=np.complex128)
+        exact = L
+        for T in (psi, W1, W2, R):
+            exact = itg.contract(exact, T, prec="fp64")
+        cur = L
+        for T in (psi, W1, W2, R):
+            cur = itg.contract(cur, T, prec=PREC)
+        torch.cuda.synchronize()
+        err = relerr(cur.tensor.data.cpu().numpy(), exact.tensor.data.cpu().numpy())
+        assert err < SPLIT_RTOL, (qns, err)
+
+
 def test_split_nonfinite_propagates(env):
     rng = np.random.default_rng(14)
     A = rand(rng, (20, 64))
