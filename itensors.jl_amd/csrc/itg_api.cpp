@@ -24,6 +24,7 @@ struct itg_handle {
   std::string err;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  std::vector<void*> retired;  // earlier workspaces, freed by itg_destroy
   // dense plan cache (LRU)
   std::list<std::pair<std::string, std::unique_ptr<ExecPlan>>> dense_lru;
   std::unordered_map<std::string, decltype(dense_lru)::iterator> dense_map;
@@ -68,18 +69,16 @@ bool dtype_ok(itg_dtype dt) { return dt == ITG_F64 || dt == ITG_C128; }
 
 hipError_t ensure_ws(itg_handle* h, size_t bytes) {
   if (bytes <= h->ws_bytes) return hipSuccess;
-  if (h->ws) {
-    // the old workspace may still be read by queued kernels
-    hipError_t e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return e;
-    e = hipFree(h->ws);
-    if (e != hipSuccess) return e;
-    h->ws = nullptr;
-    h->ws_bytes = 0;
-  }
-  size_t b = std::max(bytes, (size_t)1 << 20);
-  hipError_t e = hipMalloc(&h->ws, b);
+  // A captured graph keeps the address of the workspace it was recorded
+  // with, so a workspace that has been handed to kernels is never freed
+  // before itg_destroy: it is retired and the new one at least doubles, which
+  // keeps the retired total below the live size.
+  size_t b = std::max({bytes, (size_t)1 << 20, 2 * h->ws_bytes});
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, b);
   if (e != hipSuccess) return e;
+  if (h->ws) h->retired.push_back(h->ws);
+  h->ws = p;
   h->ws_bytes = b;
   return hipSuccess;
 }
@@ -150,10 +149,9 @@ itg_status itg_destroy(itg_handle* h) {
   if (!h) return ITG_OK;
   {
     DeviceGuard g(h->device);
-    if (h->ws) {
-      (void)hipStreamSynchronize(h->stream);
-      (void)hipFree(h->ws);
-    }
+    if (h->ws || !h->retired.empty()) (void)hipStreamSynchronize(h->stream);
+    if (h->ws) (void)hipFree(h->ws);
+    for (void* p : h->retired) (void)hipFree(p);
     h->dense_map.clear();
     h->dense_lru.clear();
   }

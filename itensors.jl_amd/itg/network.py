@@ -113,6 +113,9 @@ class BatchPlan:
         self.h.check(st)
         self.plan = plan
         self.dev = dev
+        # the plan holds raw device addresses: the tensors must outlive it
+        # (a graph captured from execute() replays against these addresses)
+        self.tensors = (list(A_list), list(B_list), list(R_list))
 
     def execute(self, prec=None, alpha=1.0):
         from . import _lib
@@ -127,6 +130,7 @@ class BatchPlan:
         try:
             if getattr(self, "plan", None) and _lib._lib is not None:
                 _lib._lib.itg_batch_plan_destroy(self.plan)
+                self.plan = None
         except Exception:
             pass
 
@@ -142,6 +146,7 @@ class BatchedSequence:
 
         self.prec = prec
         self.plans = []
+        self.arenas = []  # one per step; every plan points into these
         cur = [net[0] for net in networks]
         nsteps = len(networks[0]) - 1
         for s in range(nsteps):
@@ -159,6 +164,7 @@ class BatchedSequence:
                 inds_all.append(inds)
                 sizes.append(math.prod(dims))
             arena = torch.empty(sum(sizes), dtype=cur[0].eltype, device=cur[0].tensor.data.device)
+            self.arenas.append(arena)
             o = 0
             for dims, n in zip(dims_all, sizes):
                 outs.append(ND.DenseTensor(arena[o:o + n], dims))
