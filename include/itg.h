@@ -54,14 +54,13 @@ typedef enum {
 /* Arithmetic form of the GEMM-shaped work. */
 typedef enum {
   ITG_PREC_FP64 = 0,       /* FP64 MFMA (v_mfma_f64_16x16x4_f64), FP64 accumulate */
-  ITG_PREC_SPLIT_BF16 = 1, /* FP64 operands split into BF16 slices (per-row /   */
-                           /* per-column power-of-two scaling), slice products on */
-                           /* BF16 MFMA with exact FP32 accumulation, slices      */
-                           /* recombined and summed in FP64                       */
-  ITG_PREC_SPLIT_INT8 = 2  /* FP64 operands scaled to integers and reduced modulo */
-                           /* pairwise-coprime moduli; residue products on INT8   */
-                           /* MFMA (exact INT32 accumulation), CRT recombination  */
-                           /* in FP64                                             */
+  ITG_PREC_SPLIT_BF16 = 1, /* reserved for BF16 slices; returns                 */
+                           /* ITG_ERR_UNSUPPORTED in this version                 */
+  ITG_PREC_SPLIT_INT8 = 2  /* FP64 operands split into 7 int8 slices of 7 bits    */
+                           /* each under a power-of-two scale per row (A) / per   */
+                           /* column (B); the 28 slice products of weight >=      */
+                           /* 2^-42 run on INT8 MFMA with exact INT32 sums per    */
+                           /* weight level, recombined and summed in FP64         */
 } itg_prec;
 
 /* Execution flags for itg_bs_execute / itg_contract_dense_ex. */
@@ -85,6 +84,21 @@ itg_status itg_set_stream(itg_handle* h, void* stream);
 itg_status itg_synchronize(itg_handle* h);
 /* Library version, "major.minor.patch". */
 const char* itg_version(void);
+
+/* Per-kernel timing: the device counterpart of NDTensors' global
+ * `timer::TimerOutput` (NDTensors/src/NDTensors.jl:96-99) and its
+ * `@timeit_debug` sections around BLAS.gemm! and the permutations
+ * (NDTensors/src/abstractarray/tensoralgebra/contract.jl:61-63,125-184).
+ * While enabled, every kernel launch the handle makes outside stream capture
+ * is bracketed by HIP events on the handle's stream. itg_timer_enable(h, 1)
+ * clears the totals and starts; (h, 0) stops. itg_timer_read synchronizes
+ * the stream and reports entry i (0 <= i < itg_timer_count) as a kernel
+ * name, its launch count and its summed duration in milliseconds; `name`
+ * stays valid until the next timer call on the handle. */
+itg_status itg_timer_enable(itg_handle* h, int enable);
+int64_t itg_timer_count(itg_handle* h);
+itg_status itg_timer_read(itg_handle* h, int64_t i, const char** name, int64_t* calls,
+                          double* total_ms);
 
 /* ---------------------------------------------------------- device memory */
 /* Device buffers for callers without their own GPU array package (the Julia

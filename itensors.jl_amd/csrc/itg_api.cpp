@@ -25,6 +25,9 @@ struct itg_handle {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   std::vector<void*> retired;  // earlier workspaces, freed by itg_destroy
+  bool timing = false;         // itg_timer_enable
+  std::unique_ptr<KernelTimer> timer;
+  std::vector<std::string> timer_names;  // snapshot behind itg_timer_read's `name`
   // dense plan cache (LRU)
   std::list<std::pair<std::string, std::unique_ptr<ExecPlan>>> dense_lru;
   std::unordered_map<std::string, decltype(dense_lru)::iterator> dense_map;
@@ -104,6 +107,15 @@ std::string tune_key(const ExecPlan& ep, int cplx, int prec) {
   return buf;
 }
 
+// The handle's timer while timing is on and the stream is not being captured.
+KernelTimer* active_timer(itg_handle* h) {
+  if (!h->timing || !h->timer) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return nullptr;
+  return h->timer.get();
+}
+
 int pick_cfg(itg_handle* h, const ExecPlan& ep, int cplx, int prec) {
   auto it = h->tuned.find(tune_key(ep, cplx, prec));
   if (it != h->tuned.end()) return it->second;
@@ -120,7 +132,7 @@ itg_status run_plan(itg_handle* h, ExecPlan& ep, int cplx, int prec, const void*
   std::string err;
   const int cfg = pick_cfg(h, ep, cplx, prec);
   e = exec_run(ep, cplx, prec, cfg, h->stream, h->ws, T1, T2, R, flags, a_re, a_im, b_re, b_im,
-               &err);
+               &err, active_timer(h));
   if (e == hipErrorNotSupported) return fail(h, ITG_ERR_UNSUPPORTED, err);
   if (e != hipSuccess) return fail(h, ITG_ERR_HIP, err);
   return ITG_OK;
@@ -172,6 +184,42 @@ itg_status itg_synchronize(itg_handle* h) {
   DeviceGuard g(h->device);
   hipError_t e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
+  return ITG_OK;
+}
+
+itg_status itg_timer_enable(itg_handle* h, int enable) {
+  if (!h) return ITG_ERR_INVALID;
+  DeviceGuard g(h->device);
+  if (enable) {
+    if (!h->timer) h->timer.reset(new KernelTimer);
+    h->timer->clear();
+  }
+  h->timing = enable != 0;
+  return ITG_OK;
+}
+
+int64_t itg_timer_count(itg_handle* h) {
+  if (!h || !h->timer) return 0;
+  DeviceGuard g(h->device);
+  (void)h->timer->fold();
+  return (int64_t)h->timer->totals.size();
+}
+
+itg_status itg_timer_read(itg_handle* h, int64_t i, const char** name, int64_t* calls,
+                          double* total_ms) {
+  if (!h || !name || !calls || !total_ms) return ITG_ERR_INVALID;
+  if (!h->timer) return fail(h, ITG_ERR_INVALID, "timer entry out of range");
+  DeviceGuard g(h->device);
+  hipError_t e = h->timer->fold();
+  if (e != hipSuccess) return hip_fail(h, e, "timer events");
+  if (i < 0 || i >= (int64_t)h->timer->totals.size())
+    return fail(h, ITG_ERR_INVALID, "timer entry out of range");
+  h->timer_names.clear();
+  for (const auto& kv : h->timer->totals) h->timer_names.push_back(kv.first);
+  auto it = h->timer->totals.find(h->timer_names[(size_t)i]);
+  *name = h->timer_names[(size_t)i].c_str();
+  *calls = it->second.first;
+  *total_ms = it->second.second;
   return ITG_OK;
 }
 

@@ -110,12 +110,17 @@ constexpr int kSliceModes = 4;  // row / K modes of a sliced operand after mergi
 constexpr int kSliceCount = 7;  // int8 slices per FP64 value (7 bits each)
 
 // One block of an operand to slice: rows x K read through mode strides.
+constexpr int kSliceRows = 64;    // rows per slicing workgroup
+constexpr int kSliceKPart = 128;  // K elements per slicing workgroup
 struct SliceBlock {
   int64_t off;        // element offset of the block in the FP64 operand
-  int64_t plane_off;  // byte offset of slice plane 0 (planes are R x Kpad, row-major)
+  int64_t plane_off;  // byte offset of slice plane 0 (R x Kpad bytes per plane,
+                      // chunk-major: 16-byte K chunk c of row r at (c*R + r)*16)
   int64_t exp_off;    // index of the block's first row exponent
+  int64_t pmax_off;   // first partial exponent (kparts x R ints, part-major)
   int32_t R, K, Kpad;
-  int32_t nrm, nkm;  // number of row / K modes
+  int32_t kparts;     // ceil(Kpad / kSliceKPart): workgroups along K
+  int32_t nrm, nkm;   // number of row / K modes
   int32_t rdim[kSliceModes], rstr[kSliceModes];
   int32_t kdim[kSliceModes], kstr[kSliceModes];
 };
@@ -128,18 +133,22 @@ struct SplitPair {
 };
 
 int split_slices(int prec);  // slices per value for a precision mode (0: none)
-int64_t slice_block_wgs(int32_t rows);
-// Complex operands: nsets = 2 slice sets (re, im) for the A side, 3 (re, im,
-// -im) for the B side; plane buffers hold nsets x the per-set bytes.
+// Workgroups of one block: ceil(rows / kSliceRows) x kparts.
+int64_t slice_block_wgs(int32_t rows, int32_t kparts);
+// Two launches over the same grid: partial row maxima (as exponents, into
+// pmax), then the slices (planes) and the row exponents (exps).
+// Complex operands: nsets = 3 slice sets (re, im, -im); plane buffers hold
+// nsets x the per-set bytes.
 hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
                         const int64_t* prefix, int64_t nblk, int64_t grid, const void* src,
-                        void* planes, void* exps);
-// Tile shape of the split GEMM: 64 x 64 (the tile table uses gemm cfg 0).
+                        void* planes, void* exps, void* pmax);
+// Tile shape of the split GEMM: 64 x 64 (wmb = 1, tile table of gemm cfg 0);
+// other wmb values return hipErrorInvalidValue.
 // Complex: one launch per output component comp (0 = re, 1 = im); `sets`
 // packs the slice sets (0 re, 1 im, 2 -im) of A and B for the two K halves:
 // bits 0-1 A half 0, 2-3 B half 0, 4-5 A half 1, 6-7 B half 1. Both complex
 // operands are sliced with 3 sets.
-hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int comp, int sets,
+hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int comp, int sets,
                              const SplitPair* pairs, const GemmGroup* groups, const int2* tiles,
                              int64_t grid, const void* planesA, const void* planesB,
                              const void* expA, const void* expB, void* C, double ga,

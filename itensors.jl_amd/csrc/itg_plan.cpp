@@ -565,10 +565,10 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
   ep->split_ok = false;
   auto side = [&](const BlockList& T, const std::vector<int>& rows, const std::vector<int>& ks,
                   std::vector<SliceBlock>* out, std::vector<int64_t>* prefix, int64_t* plane_bytes,
-                  int64_t* nrows) -> std::string {
+                  int64_t* nrows, int64_t* npmax) -> std::string {
     out->resize(T.nblk());
     prefix->assign(T.nblk() + 1, 0);
-    int64_t pb = 0, er = 0;
+    int64_t pb = 0, er = 0, pm = 0;
     for (int64_t b = 0; b < T.nblk(); ++b) {
       SliceBlock& d = (*out)[b];
       std::memset(&d, 0, sizeof(d));
@@ -582,20 +582,26 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
       d.R = (int32_t)R;
       d.K = (int32_t)K;
       d.Kpad = (int32_t)((K + 31) / 32 * 32);
+      d.kparts = (d.Kpad + kSliceKPart - 1) / kSliceKPart;
       d.plane_off = pb;
       d.exp_off = er;
+      d.pmax_off = pm;
       pb += (int64_t)kSliceCount * R * d.Kpad;
       er += R;
-      (*prefix)[b + 1] = (*prefix)[b] + slice_block_wgs(d.R);
+      pm += (int64_t)d.kparts * R;
+      (*prefix)[b + 1] = (*prefix)[b] + slice_block_wgs(d.R, d.kparts);
     }
     *plane_bytes = pb;
     *nrows = er;
+    *npmax = pm;
     return "";
   };
   std::vector<SliceBlock> dA, dB;
   std::vector<int64_t> pA, pB;
-  std::string err = side(P, Mp, Xp, &dA, &pA, &ep->sA_plane_bytes, &ep->sA_rows);
-  if (err.empty()) err = side(Q, Nq, Xq, &dB, &pB, &ep->sB_plane_bytes, &ep->sB_rows);
+  std::string err =
+      side(P, Mp, Xp, &dA, &pA, &ep->sA_plane_bytes, &ep->sA_rows, &ep->sA_pmax);
+  if (err.empty())
+    err = side(Q, Nq, Xq, &dB, &pB, &ep->sB_plane_bytes, &ep->sB_rows, &ep->sB_pmax);
   if (!err.empty()) {
     ep->split_why = err;
     return "";  // the FP64 path still works
@@ -737,6 +743,7 @@ size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec) {
     const size_t ns = cplx ? 3 : 1;
     b += align256((size_t)ep.sA_plane_bytes * ns) + align256((size_t)ep.sB_plane_bytes * ns);
     b += align256((size_t)ep.sA_rows * 4) + align256((size_t)ep.sB_rows * 4);
+    b += align256((size_t)ep.sA_pmax * 4) + align256((size_t)ep.sB_pmax * 4);
   } else {
     if (ep.permA) b += align256((size_t)ep.nnzA * es);
     if (ep.permB) b += align256((size_t)ep.nnzB * es);
@@ -761,9 +768,23 @@ int split_sets(int comp, bool conja, bool conjb) {
   return a0 | (b0 << 2) | (a1 << 4) | (b1 << 6);
 }
 
-hipError_t exec_split(ExecPlan& ep, int cplx, int prec, hipStream_t s, void* ws, const void* T1,
-                      const void* T2, void* R, unsigned flags, double a_re, double a_im,
-                      double b_re, double b_im, std::string* err) {
+// Brackets the launches made in its lifetime with timer events (no-op when
+// the timer is null).
+struct TimeScope {
+  KernelTimer* kt;
+  hipStream_t s;
+  TimeScope(KernelTimer* k, const char* name, hipStream_t st) : kt(k), s(st) {
+    if (kt) kt->begin(name, s);
+  }
+  ~TimeScope() {
+    if (kt) kt->end(s);
+  }
+};
+
+hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
+                      const void* T1, const void* T2, void* R, unsigned flags, double a_re,
+                      double a_im, double b_re, double b_im, std::string* err,
+                      KernelTimer* kt) {
   if (!ep.split_ok) {
     *err = "split precision is unavailable for this contraction: " + ep.split_why;
     return hipErrorNotSupported;
@@ -794,42 +815,62 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, hipStream_t s, void* ws,
   w += align256((size_t)ep.sA_rows * 4);
   void* expB = w;
   w += align256((size_t)ep.sB_rows * 4);
-  hipError_t e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.sA_desc.p,
-                              (const int64_t*)ep.sA_prefix.p, ep.sA_blocks, ep.sA_grid, A,
-                              planesA, expA);
-  if (e == hipSuccess)
+  void* pmaxA = w;
+  w += align256((size_t)ep.sA_pmax * 4);
+  void* pmaxB = w;
+  w += align256((size_t)ep.sB_pmax * 4);
+  hipError_t e;
+  {
+    TimeScope ts(kt, "slice_max_kernel+slice_kernel", s);
+    e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.sA_desc.p,
+                     (const int64_t*)ep.sA_prefix.p, ep.sA_blocks, ep.sA_grid, A, planesA, expA,
+                     pmaxA);
+  }
+  if (e == hipSuccess) {
+    TimeScope ts(kt, "slice_max_kernel+slice_kernel", s);
     e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.sB_desc.p,
-                     (const int64_t*)ep.sB_prefix.p, ep.sB_blocks, ep.sB_grid, B, planesB, expB);
+                     (const int64_t*)ep.sB_prefix.p, ep.sB_blocks, ep.sB_grid, B, planesB, expB,
+                     pmaxB);
+  }
   if (e != hipSuccess) {
     *err = std::string("slice launch: ") + hipGetErrorString(e);
     return e;
   }
-  if ((e = ensure_gemm_launch(ep, 0, 0, s)) != hipSuccess) {
+  (void)cfg;            // one split tile shape: 64 x 64
+  const int wmb = 1;
+  const int tcfg = 0;   // the FP64 tile table of the same shape
+  if ((e = ensure_gemm_launch(ep, tcfg, 0, s)) != hipSuccess) {
     *err = std::string("gemm table upload: ") + hipGetErrorString(e);
     return e;
   }
-  GemmLaunch& gl = *ep.launches[0];
+  GemmLaunch& gl = *ep.launches[tcfg];
   void* C = ep.permC ? (void*)w : R;
   double beta_k = ep.permC ? 0.0 : b_re;
   if (cplx && !ep.permC && b_im != 0.0) {
     // complex beta mixes the components: scale R first, then accumulate
-    e = launch_scale(s, 1, R, ep.nnzC, b_re, b_im);
+    {
+      TimeScope ts(kt, "scale_kernel", s);
+      e = launch_scale(s, 1, R, ep.nnzC, b_re, b_im);
+    }
     if (e != hipSuccess) {
       *err = std::string("scale launch: ") + hipGetErrorString(e);
       return e;
     }
     beta_k = 1.0;
   }
-  for (int comp = 0; comp < (cplx ? 2 : 1) && e == hipSuccess; ++comp)
-    e = launch_split_gemm(s, prec, cplx, comp, split_sets(comp, conja, conjb),
+  for (int comp = 0; comp < (cplx ? 2 : 1) && e == hipSuccess; ++comp) {
+    TimeScope ts(kt, "split_gemm_kernel", s);
+    e = launch_split_gemm(s, prec, cplx, wmb, comp, split_sets(comp, conja, conjb),
                           (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.g_groups.p,
                           (const int2*)gl.tiles.p, gl.grid, planesA, planesB, expA, expB, C, a_re,
                           beta_k);
+  }
   if (e != hipSuccess) {
     *err = std::string("split gemm launch: ") + hipGetErrorString(e);
     return e;
   }
   if (ep.permC) {
+    TimeScope ts(kt, "permute_kernel", s);
     e = launch_permute(s, cplx, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,
                        (const int64_t*)ep.pC.prefix.p, ep.pC.nblk, ep.pC.tiles, ep.pC.kind, C, R,
                        1.0, 0.0, b_re, b_im);
@@ -842,27 +883,34 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, hipStream_t s, void* ws,
 
 hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                     const void* T1, const void* T2, void* R, unsigned flags, double a_re,
-                    double a_im, double b_re, double b_im, std::string* err) {
+                    double a_im, double b_re, double b_im, std::string* err, KernelTimer* kt) {
   hipError_t e;
   if (ep.strategy == ExecPlan::DIRECT) {
-    e = launch_direct(s, cplx, ep.st.NR, (const DirectPair*)ep.d_pairs.p,
-                      (const DirectGroup*)ep.d_groups.p, (const int32_t*)ep.d_koffs.p,
-                      (const int64_t*)ep.d_prefix.p, ep.d_ngroups, ep.d_tiles, T1, T2, R, flags,
-                      a_re, a_im, b_re, b_im);
+    {
+      TimeScope ts(kt, "direct_kernel", s);
+      e = launch_direct(s, cplx, ep.st.NR, (const DirectPair*)ep.d_pairs.p,
+                        (const DirectGroup*)ep.d_groups.p, (const int32_t*)ep.d_koffs.p,
+                        (const int64_t*)ep.d_prefix.p, ep.d_ngroups, ep.d_tiles, T1, T2, R,
+                        flags, a_re, a_im, b_re, b_im);
+    }
     if (e != hipSuccess) *err = std::string("direct kernel launch: ") + hipGetErrorString(e);
     return e;
   }
   if (prec != 0)
-    return exec_split(ep, cplx, prec, s, ws, T1, T2, R, flags, a_re, a_im, b_re, b_im, err);
+    return exec_split(ep, cplx, prec, cfg, s, ws, T1, T2, R, flags, a_re, a_im, b_re, b_im, err,
+                      kt);
   const size_t es = cplx ? 16 : 8;
   char* w = (char*)ws;
   const void* A = ep.swap ? T2 : T1;
   const void* B = ep.swap ? T1 : T2;
   unsigned gflags = ep.swap ? (((flags & 1u) << 1) | ((flags & 2u) >> 1)) : flags;
   if (ep.permA) {
-    e = launch_permute(s, cplx, ep.pA.nm, ep.pA.perm, (const PermBlock*)ep.pA.blocks.p,
-                       (const int64_t*)ep.pA.prefix.p, ep.pA.nblk, ep.pA.tiles, ep.pA.kind, A, w,
-                       1.0, 0.0, 0.0, 0.0);
+    {
+      TimeScope ts(kt, "permute_kernel", s);
+      e = launch_permute(s, cplx, ep.pA.nm, ep.pA.perm, (const PermBlock*)ep.pA.blocks.p,
+                         (const int64_t*)ep.pA.prefix.p, ep.pA.nblk, ep.pA.tiles, ep.pA.kind, A,
+                         w, 1.0, 0.0, 0.0, 0.0);
+    }
     if (e != hipSuccess) {
       *err = std::string("permute launch: ") + hipGetErrorString(e);
       return e;
@@ -871,9 +919,12 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     w += (size_t)ep.nnzA * es;
   }
   if (ep.permB) {
-    e = launch_permute(s, cplx, ep.pB.nm, ep.pB.perm, (const PermBlock*)ep.pB.blocks.p,
-                       (const int64_t*)ep.pB.prefix.p, ep.pB.nblk, ep.pB.tiles, ep.pB.kind, B, w,
-                       1.0, 0.0, 0.0, 0.0);
+    {
+      TimeScope ts(kt, "permute_kernel", s);
+      e = launch_permute(s, cplx, ep.pB.nm, ep.pB.perm, (const PermBlock*)ep.pB.blocks.p,
+                         (const int64_t*)ep.pB.prefix.p, ep.pB.nblk, ep.pB.tiles, ep.pB.kind, B,
+                         w, 1.0, 0.0, 0.0, 0.0);
+    }
     if (e != hipSuccess) {
       *err = std::string("permute launch: ") + hipGetErrorString(e);
       return e;
@@ -894,24 +945,95 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
   }
   int ecfg = (cfg == kCfgMixed || gemm_cfg_ok(cfg, cplx)) ? cfg : 0;
   GemmLaunch& gl = *ep.launches[ecfg];
-  e = launch_gemm_f64(s, gl.cfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
-                      (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B, C,
-                      gflags, a_re, a_im, gbr, gbi);
-  if (e == hipSuccess && gl.cfg2 >= 0)
+  {
+    TimeScope ts(kt, "gemm_f64_kernel", s);
+    e = launch_gemm_f64(s, gl.cfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
+                        (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles.p, gl.grid, A, B,
+                        C, gflags, a_re, a_im, gbr, gbi);
+  }
+  if (e == hipSuccess && gl.cfg2 >= 0) {
+    TimeScope ts(kt, "gemm_f64_kernel", s);
     e = launch_gemm_f64(s, gl.cfg2, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
                         (const GemmGroup*)ep.g_groups.p, (const int2*)gl.tiles2.p, gl.grid2, A, B,
                         C, gflags, a_re, a_im, gbr, gbi);
+  }
   if (e != hipSuccess) {
     *err = std::string("gemm launch: ") + hipGetErrorString(e);
     return e;
   }
   if (ep.permC) {
+    TimeScope ts(kt, "permute_kernel", s);
     e = launch_permute(s, cplx, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,
                        (const int64_t*)ep.pC.prefix.p, ep.pC.nblk, ep.pC.tiles, ep.pC.kind, C, R,
                        1.0, 0.0, b_re, b_im);
     if (e != hipSuccess) *err = std::string("permute launch: ") + hipGetErrorString(e);
   }
   return e;
+}
+
+// ---------------------------------------------------------------- timer
+hipEvent_t KernelTimer::take() {
+  if (!pool.empty()) {
+    hipEvent_t ev = pool.back();
+    pool.pop_back();
+    return ev;
+  }
+  hipEvent_t ev = nullptr;
+  if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+  return ev;
+}
+
+void KernelTimer::begin(const char* name, hipStream_t s) {
+  open_name = name;
+  open_a = take();
+  if (open_a) (void)hipEventRecord(open_a, s);
+}
+
+void KernelTimer::end(hipStream_t s) {
+  hipEvent_t b = take();
+  if (!open_a || !b) {
+    if (open_a) pool.push_back(open_a);
+    if (b) pool.push_back(b);
+    open_a = nullptr;
+    return;
+  }
+  (void)hipEventRecord(b, s);
+  pending.push_back(Pending{open_name, open_a, b});
+  open_a = nullptr;
+}
+
+hipError_t KernelTimer::fold() {
+  hipError_t first = hipSuccess;
+  for (Pending& p : pending) {
+    hipError_t e = hipEventSynchronize(p.b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
+    if (e == hipSuccess) {
+      auto& t = totals[p.name];
+      t.first += 1;
+      t.second += ms;
+    } else if (first == hipSuccess) {
+      first = e;
+    }
+    pool.push_back(p.a);
+    pool.push_back(p.b);
+  }
+  pending.clear();
+  return first;
+}
+
+void KernelTimer::clear() {
+  (void)fold();
+  totals.clear();
+}
+
+KernelTimer::~KernelTimer() {
+  for (Pending& p : pending) {
+    (void)hipEventSynchronize(p.b);
+    pool.push_back(p.a);
+    pool.push_back(p.b);
+  }
+  for (hipEvent_t ev : pool) (void)hipEventDestroy(ev);
 }
 
 }  // namespace itg

@@ -27,6 +27,29 @@
 
 namespace itg {
 
+// Per-kernel HIP-event timing (enabled per handle). begin()/end() bracket one
+// launch on stream s; fold() waits for the stream and adds the elapsed times
+// to the per-name totals.
+struct KernelTimer {
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<int64_t, double>> totals;  // calls, ms
+  hipStream_t stream = nullptr;
+  std::string open_name;
+  hipEvent_t open_a = nullptr;
+
+  hipEvent_t take();
+  void begin(const char* name, hipStream_t s);
+  void end(hipStream_t s);
+  hipError_t fold();
+  void clear();
+  ~KernelTimer();
+};
+
 struct Structure {
   int NR = 0, N1 = 0, N2 = 0;
   int r_src[kMaxModes];  // 1 or 2
@@ -113,6 +136,7 @@ struct ExecPlan {
   std::string split_why;         // reason when !split_ok
   int64_t sA_blocks = 0, sA_grid = 0, sB_blocks = 0, sB_grid = 0;
   int64_t sA_plane_bytes = 0, sB_plane_bytes = 0, sA_rows = 0, sB_rows = 0;
+  int64_t sA_pmax = 0, sB_pmax = 0;  // partial row exponents (ints)
   DevBuf sA_desc, sA_prefix, sB_desc, sB_prefix, s_pairs;
 };
 
@@ -136,9 +160,11 @@ hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const
 
 // Launch the plan. ws: workspace of at least exec_workspace_bytes().
 size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec);
+// kt (optional) brackets each launch with timer events.
 hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                     const void* T1, const void* T2, void* R, unsigned flags, double a_re,
-                    double a_im, double b_re, double b_im, std::string* err);
+                    double a_im, double b_re, double b_im, std::string* err,
+                    KernelTimer* kt = nullptr);
 // Tile cfg picked by the built-in heuristic (the autotuner may override).
 int default_gemm_cfg(const ExecPlan& ep, int cplx);
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s);

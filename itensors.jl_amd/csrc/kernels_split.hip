@@ -8,8 +8,10 @@
 //    v_i in [-64, 64] (int8), |residue| <= 2^(E_r - 7S). The slices come
 //    from exact FP64 steps (y = x*2^(6-E); v = rint(y); y = (y-v)*128).
 //    The operand is read in place through its mode strides (no permutation
-//    pass) and each slice plane is written row-major with K padded to a
-//    multiple of 32 (zeros), so the GEMM reads 16-byte K chunks unchecked.
+//    pass). Each slice plane holds K padded to a multiple of 32 (zeros) in
+//    16-byte chunks, chunk-major: chunk c of row r at byte (c*R + r)*16, so
+//    that 64 consecutive rows of one chunk are one contiguous 1 KB run for
+//    both the writer and the GEMM's loads.
 // 2. split_gemm_kernel: C = sum_{pairs} alpha_p * A_p B_p, where for every
 //    pair the products v^A_i v^B_j with i + j <= S + 1 run on
 //    v_mfma_i32_32x32x32_i8, summed per level L = i + j in INT32 -- exact,
@@ -19,9 +21,14 @@
 //    The error is that of representing A and B with 7S bits per row/column
 //    scale (about 2^-7S relative to the row maximum); no rounding occurs in
 //    the products or the INT32 sums.
-// Tiles: 256 threads = 4 wave64s (2x2), 64x64 per workgroup, 32x32 per wave;
-// K steps of 32 double-buffered in LDS (slice planes, 16-byte chunks with a
-// bank swizzle). The MFMA is issued as D = B^T A^T so that each lane's
+// Tiles: 256 threads = 4 wave64s (2x2), (64*WMB) x 64 per workgroup,
+// (32*WMB) x 32 per wave; K steps of 32 (two chunks) double-buffered in LDS.
+// Stages are fetched with global_load_lds_dwordx4 (HBM/L2 -> LDS without
+// registers): one wave instruction moves the 64 rows of one (plane, chunk),
+// issued a full K step ahead of the MFMAs that consume it. In LDS a stage is
+// [plane][chunk][row] x 16 bytes, so a lane's MFMA operand (row l%32, chunk
+// l/32) is read with consecutive lanes on consecutive 16-byte slots (no bank
+// conflicts). The MFMA is issued as D = B^T A^T so that each lane's
 // accumulators run down a column of C (contiguous column-major stores).
 #include "itg_internal.h"
 
@@ -33,8 +40,9 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
-constexpr int kRowsPerWG = 64;  // slice kernel rows per workgroup
-constexpr int kNonFinite = 0x7fff;
+constexpr int kNonFinite = 0x7fff;   // row exponent of a row holding Inf / NaN
+constexpr int kZeroRow = -0x40000000;  // partial exponent of an all-zero K range
+static_assert(kSliceRows == 64 && kSliceKPart % 64 == 0, "slice tiling");
 
 __device__ inline int64_t find_slot(const int64_t* prefix, int64_t n, int64_t wg) {
   int64_t lo = 0, hi = n - 1;
@@ -79,46 +87,100 @@ __device__ inline double elem_mag(const double* x, int64_t o) {
   return isfinite(v) ? fabs(v) : INFINITY;
 }
 
-// nsets: slice sets written per element -- 1 (real), 2 (re, im: A side of a
-// complex product), 3 (re, im, -im: B side); set s of block b starts at
-// planes + nsets*plane_off + s*S*R*Kpad.
-template <int S, int CPLX>
+// Offset of K multi-index k (K mode 0 fastest), advanced one element at a
+// time without divisions.
+struct KWalk {
+  int64_t off;
+  int32_t c[kSliceModes];
+  __device__ void init(uint32_t k, const SliceBlock& d) {
+    off = 0;
+#pragma unroll
+    for (int i = 0; i < kSliceModes; ++i) {
+      c[i] = 0;
+      if (i >= d.nkm) continue;
+      if (i + 1 < d.nkm) {
+        const uint32_t q = k / (uint32_t)d.kdim[i];
+        c[i] = (int32_t)(k - q * (uint32_t)d.kdim[i]);
+        k = q;
+      } else {
+        c[i] = (int32_t)k;
+      }
+      off += (int64_t)c[i] * d.kstr[i];
+    }
+  }
+  __device__ void step(const SliceBlock& d) {
+    off += d.kstr[0];
+    if (++c[0] == d.kdim[0]) {
+      c[0] = 0;
+      off -= (int64_t)d.kdim[0] * d.kstr[0];
+#pragma unroll
+      for (int i = 1; i < kSliceModes; ++i) {
+        if (i >= d.nkm) break;
+        off += d.kstr[i];
+        if (++c[i] < d.kdim[i]) break;
+        c[i] = 0;
+        off -= (int64_t)d.kdim[i] * d.kstr[i];
+      }
+    }
+  }
+};
+
+// Workgroup -> (block, 64-row tile rt, K part kp) of the slicing grid.
+struct SliceTile {
+  int64_t b;
+  int r0, kp;
+};
+__device__ inline SliceTile slice_tile(const SliceBlock* blocks, const int64_t* prefix,
+                                       int64_t nblk, int64_t wg) {
+  SliceTile t;
+  t.b = find_slot(prefix, nblk, wg);
+  const int local = (int)(wg - prefix[t.b]);
+  const int kparts = blocks[t.b].kparts;
+  t.kp = local % kparts;
+  t.r0 = (local / kparts) * kSliceRows;
+  return t;
+}
+
+// Pass 1: for each row of a (64 rows) x (kSliceKPart) tile, the exponent of
+// the largest magnitude in the tile's K range (kZeroRow if all are zero,
+// kNonFinite if any is Inf/NaN), stored as partial kp of the row. The source
+// is contiguous either along the rows (operand stored rows-fastest) or along
+// K; threads run along the contiguous direction so that loads coalesce.
+template <int CPLX>
 __global__ void __launch_bounds__(kThreads)
-    slice_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
-                 int64_t nblk, const double* __restrict__ src, int8_t* __restrict__ planes,
-                 int32_t* __restrict__ exps, int nsets) {
-  __shared__ double red[4][kRowsPerWG];
-  __shared__ int32_t sexp[kRowsPerWG];
-  const int64_t wg = blockIdx.x;
-  const int64_t b = find_slot(prefix, nblk, wg);
-  const SliceBlock& d = blocks[b];
-  const int r0 = (int)(wg - prefix[b]) * kRowsPerWG;
+    slice_max_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
+                     int64_t nblk, const double* __restrict__ src, int32_t* __restrict__ pmax) {
+  __shared__ double red[4][kSliceRows];
+  const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
+  const SliceBlock& d = blocks[t.b];
   const int tid = threadIdx.x;
   const double* x = src + (CPLX ? 2 : 1) * d.off;
-
-  // The source is contiguous either along the rows (A stored M-fastest) or
-  // along K (A stored K-fastest, B in N form); threads are laid along the
-  // contiguous direction so that loads coalesce.
+  const int k_lo = t.kp * kSliceKPart, k_hi = min(d.K, k_lo + kSliceKPart);
   const bool kfast = d.nkm == 1 && d.kstr[0] == 1;
-
-  // phase 1: per-row max |x| over K
   if (!kfast) {
-    const int row = tid % kRowsPerWG, part = tid / kRowsPerWG;
+    // lanes along the rows; thread part p walks its quarter of the K range
+    constexpr int kQ = kSliceKPart / 4;
+    const int row = tid % kSliceRows, part = tid / kSliceRows;
     double m = 0.0;
-    if (r0 + row < d.R) {
-      const int64_t ro = mode_offset((uint32_t)(r0 + row), d.nrm, d.rdim, d.rstr);
-      for (int k = part; k < d.K; k += 4)
-        m = fmax(m, elem_mag<CPLX>(x, ro + mode_offset((uint32_t)k, d.nkm, d.kdim, d.kstr)));
+    const int ka = k_lo + part * kQ, kb = min(k_hi, ka + kQ);
+    if (t.r0 + row < d.R && ka < kb) {
+      const int64_t ro = mode_offset((uint32_t)(t.r0 + row), d.nrm, d.rdim, d.rstr);
+      KWalk w;
+      w.init((uint32_t)ka, d);
+      for (int k = ka; k < kb; ++k) {
+        m = fmax(m, elem_mag<CPLX>(x, ro + w.off));
+        w.step(d);
+      }
     }
     red[part][row] = m;
   } else {
     // one wave per row at a time, lanes along K
     const int lane = tid & 63, wave = tid >> 6;
-    for (int row = wave; row < kRowsPerWG; row += 4) {
+    for (int row = wave; row < kSliceRows; row += 4) {
       double m = 0.0;
-      if (r0 + row < d.R) {
-        const int64_t ro = mode_offset((uint32_t)(r0 + row), d.nrm, d.rdim, d.rstr);
-        for (int k = lane; k < d.K; k += 64) m = fmax(m, elem_mag<CPLX>(x, ro + k));
+      if (t.r0 + row < d.R) {
+        const int64_t ro = mode_offset((uint32_t)(t.r0 + row), d.nrm, d.rdim, d.rstr);
+        for (int k = k_lo + lane; k < k_hi; k += 64) m = fmax(m, elem_mag<CPLX>(x, ro + k));
       }
 #pragma unroll
       for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s));
@@ -129,49 +191,59 @@ __global__ void __launch_bounds__(kThreads)
     }
   }
   __syncthreads();
-  if (tid < kRowsPerWG) {
+  if (tid < kSliceRows && t.r0 + tid < d.R) {
     const double m = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
-    int e = 0;
+    int e = kZeroRow;
     if (!isfinite(m))
       e = kNonFinite;
     else if (m > 0.0)
       (void)frexp(m, &e);  // m = f*2^e, f in [0.5, 1): m < 2^e
+    pmax[d.pmax_off + (int64_t)t.kp * d.R + t.r0 + tid] = e;
+  }
+}
+
+// Pass 2: row exponent E = max of the row's partials (0 for an all-zero row),
+// then the S slices of every element of the tile.
+// nsets: slice sets written per element -- 1 (real) or 3 (re, im, -im:
+// complex); set s of block b starts at planes + nsets*plane_off + s*S*R*Kpad.
+template <int S, int CPLX>
+__global__ void __launch_bounds__(kThreads)
+    slice_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
+                 int64_t nblk, const double* __restrict__ src, int8_t* __restrict__ planes,
+                 int32_t* __restrict__ exps, const int32_t* __restrict__ pmax, int nsets) {
+  __shared__ int32_t sexp[kSliceRows];
+  const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
+  const SliceBlock& d = blocks[t.b];
+  const int r0 = t.r0;
+  const int tid = threadIdx.x;
+  const double* x = src + (CPLX ? 2 : 1) * d.off;
+  if (tid < kSliceRows) {
+    int e = kZeroRow;
+    if (r0 + tid < d.R)
+      for (int p = 0; p < d.kparts; ++p)
+        e = max(e, pmax[d.pmax_off + (int64_t)p * d.R + r0 + tid]);
+    if (e == kZeroRow) e = 0;
     sexp[tid] = e;
-    if (r0 + tid < d.R) exps[d.exp_off + r0 + tid] = e;
+    if (t.kp == 0 && r0 + tid < d.R) exps[d.exp_off + r0 + tid] = e;
   }
   __syncthreads();
 
-  // phase 2: slice 16 consecutive k of one row per item
-  const int nchunk = d.Kpad / 16;
-  const int items = kRowsPerWG * nchunk;
+  // slice 16 consecutive k of one row per item; consecutive threads take
+  // consecutive rows, so the 16-byte chunks they write are adjacent in the
+  // chunk-major planes (a K-contiguous source is then read 128 bytes per
+  // lane, which the L1 cache serves line by line)
+  constexpr int kChunks = kSliceKPart / 16;
+  const int ch_end = min(d.Kpad / 16, (t.kp + 1) * kChunks);
   const int64_t pstride = (int64_t)d.R * d.Kpad;
-  for (int it = tid; it < items; it += kThreads) {
-    // consecutive threads: consecutive rows (row-contiguous source) or
-    // consecutive K chunks of one row (K-contiguous source)
-    const int row = kfast ? it / nchunk : it % kRowsPerWG;
-    const int ch = kfast ? it % nchunk : it / kRowsPerWG;
+  for (int it = tid; it < kSliceRows * kChunks; it += kThreads) {
+    const int row = it % kSliceRows;
+    const int ch = t.kp * kChunks + it / kSliceRows;
     const int gr = r0 + row;
-    if (gr >= d.R) continue;
+    if (gr >= d.R || ch >= ch_end) continue;
     const int e = sexp[row];
     const int64_t ro = mode_offset((uint32_t)gr, d.nrm, d.rdim, d.rstr);
-    // K offsets of this chunk: step through the K modes instead of dividing
-    int64_t ko = 0;
-    int32_t kc[kSliceModes] = {0, 0, 0, 0};
-    {
-      uint32_t idx = (uint32_t)(ch * 16);
-#pragma unroll
-      for (int i = 0; i < kSliceModes; ++i) {
-        if (i >= d.nkm) break;
-        if (i + 1 < d.nkm) {
-          const uint32_t q = idx / (uint32_t)d.kdim[i];
-          kc[i] = (int32_t)(idx - q * (uint32_t)d.kdim[i]);
-          idx = q;
-        } else {
-          kc[i] = (int32_t)idx;
-        }
-        ko += (int64_t)kc[i] * d.kstr[i];
-      }
-    }
+    KWalk kw;
+    kw.init((uint32_t)(ch * 16), d);
     constexpr int NC = CPLX ? 3 : 1;  // re, im, -im
     uint32_t packed[NC][S][4];
 #pragma unroll
@@ -186,27 +258,14 @@ __global__ void __launch_bounds__(kThreads)
       double y = 0.0, yi = 0.0;
       if (k < d.K && e != kNonFinite) {
         if (CPLX) {
-          const double2 v = reinterpret_cast<const double2*>(x)[ro + ko];
+          const double2 v = reinterpret_cast<const double2*>(x)[ro + kw.off];
           y = ldexp(v.x, 6 - e);
           yi = ldexp(v.y, 6 - e);
         } else {
-          y = ldexp(x[ro + ko], 6 - e);
+          y = ldexp(x[ro + kw.off], 6 - e);
         }
       }
-      // advance the K coordinate by one (mode 0 fastest)
-      ko += d.kstr[0];
-      if (++kc[0] == d.kdim[0]) {
-        kc[0] = 0;
-        ko -= (int64_t)d.kdim[0] * d.kstr[0];
-#pragma unroll
-        for (int i = 1; i < kSliceModes; ++i) {
-          if (i >= d.nkm) break;
-          ko += d.kstr[i];
-          if (++kc[i] < d.kdim[i]) break;
-          kc[i] = 0;
-          ko -= (int64_t)d.kdim[i] * d.kstr[i];
-        }
-      }
+      kw.step(d);
 #pragma unroll
       for (int i = 0; i < S; ++i) {
         const double v = rint(y);
@@ -220,7 +279,7 @@ __global__ void __launch_bounds__(kThreads)
         }
       }
     }
-    int8_t* dst = planes + (int64_t)nsets * d.plane_off + (int64_t)gr * d.Kpad + ch * 16;
+    int8_t* dst = planes + (int64_t)nsets * d.plane_off + ((int64_t)ch * d.R + gr) * 16;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if (c >= nsets) break;
@@ -232,7 +291,16 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-constexpr int BM = 64, BN = 64, BKB = 32;  // BKB: K bytes per LDS stage
+constexpr int BN = 64, BKB = 32;  // BKB: K bytes per LDS stage (two 16-byte chunks)
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+typedef __attribute__((address_space(1))) void* gbl_ptr_t;
+
+// 16 bytes per lane from global memory straight into LDS; lane l's bytes land
+// at lds_base + 16*l (lds_base is wave-uniform, passed through M0).
+__device__ inline void load_lds16(const int8_t* g, v4i* lds_base) {
+  __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, (lds_ptr_t)lds_base, 16, 0, 0);
+}
 
 // Complex data (CPLX = 1): one launch per output component `comp`; the
 // component is a K-stacked real product over two halves, e.g.
@@ -241,18 +309,21 @@ constexpr int BM = 64, BN = 64, BKB = 32;  // BKB: K bytes per LDS stage
 // set of half 0, bits 4-5 / 6-7: half 1), accumulated into the same INT32
 // levels (both halves share the row and column scales), and written to the
 // interleaved complex output.
-template <int S, int CPLX>
-__global__ void __launch_bounds__(kThreads, CPLX ? 1 : 2)
+template <int S, int CPLX, int WMB>
+__global__ void __launch_bounds__(kThreads, WMB == 1 ? 2 : 1)
     split_gemm_kernel(const SplitPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
                       const int2* __restrict__ tiles, const int8_t* __restrict__ planesA,
                       const int8_t* __restrict__ planesB, const int32_t* __restrict__ expA,
                       const int32_t* __restrict__ expB, double* __restrict__ Cg, double ga,
                       double beta, int comp, int sets, int nsetsA, int nsetsB) {
-  constexpr int NL = S;  // levels 2..S+1
-  constexpr int CH_A = S * BM * 2;  // 16-byte chunks per stage
-  constexpr int CH_B = S * BN * 2;
-  __shared__ v4i As[2][S][BM][2];
-  __shared__ v4i Bs[2][S][BN][2];
+  constexpr int BM = 64 * WMB;
+  constexpr int NL = S;               // levels 2..S+1
+  constexpr int A_ST = S * 2 * BM;    // 16-byte LDS slots of one A stage
+  constexpr int B_ST = S * 2 * BN;
+  constexpr int NLD_A = S * 2 * WMB;  // wave loads (64 rows x 16 bytes) per A stage
+  constexpr int NLD = NLD_A + S * 2;
+  constexpr int NLD_W = (NLD + 3) / 4;
+  __shared__ v4i lds[2 * (A_ST + B_ST)];
 
   const int2 tt = tiles[blockIdx.x];
   if (tt.x < 0) return;
@@ -260,15 +331,20 @@ __global__ void __launch_bounds__(kThreads, CPLX ? 1 : 2)
   const int tiles_m = (grp.M + BM - 1) / BM;
   const int tm = tt.y % tiles_m, tn = tt.y / tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm0 = (wave & 1) * 32 * WMB, wn0 = (wave >> 1) * 32;
   const int l32 = lane & 31, lh = lane >> 5;
+  // B row fetched by this lane; rows past the block are clamped onto its
+  // last row (they only feed outputs that are never stored)
+  const int brow = min(n0 + lane, grp.N - 1);
 
-  // this lane's output rows/cols: m = m0+wm0+l32 ; n = n0+wn0+(r%4)+8*(r/4)+4*lh
-  const int gm = m0 + wm0 + l32;
-  double acc[16];
+  // this lane's outputs: m = m0+wm0+32b+l32 ; n = n0+wn0+(r%4)+8*(r/4)+4*lh
+  double acc[WMB][16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+  for (int b = 0; b < WMB; ++b)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[b][r] = 0.0;
 
   for (int p = grp.p0; p < grp.p1; ++p) {
     const SplitPair pr = pairs[p];
@@ -276,41 +352,32 @@ __global__ void __launch_bounds__(kThreads, CPLX ? 1 : 2)
     if (kpad <= 0) continue;
     const int8_t* pa = planesA;
     const int8_t* pb = planesB;
-    v16i lev[NL];
+    v16i lev[WMB][NL];
 #pragma unroll
-    for (int L = 0; L < NL; ++L) lev[L] = v16i{0};
+    for (int b = 0; b < WMB; ++b)
+#pragma unroll
+      for (int L = 0; L < NL; ++L) lev[b][L] = v16i{0};
 
-    // one 16-byte chunk = (slice plane, row, K half) of the A or B tile;
-    // the CH_A + CH_B chunks of a stage are spread over the threads
-    constexpr int NQ = (CH_A + CH_B + kThreads - 1) / kThreads;
-    v4i rq[NQ];
-    auto load = [&](int k0) {
+    // the NLD wave loads of the stage at K byte k0, into buffer buf; wave w
+    // issues loads w, w+4, ...
+    auto issue = [&](int k0, int buf) {
+      v4i* sA = lds + buf * (A_ST + B_ST);
+      v4i* sB = sA + A_ST;
+      const int c0 = k0 >> 4;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        int c = tid + q * kThreads;
-        if (c >= CH_A + CH_B) continue;
-        const bool isa = c < CH_A;
-        if (!isa) c -= CH_A;
-        const int half = c & 1, row = (c >> 1) % 64, pl = c / 128;
-        const int g = (isa ? m0 : n0) + row;
-        const bool in = g < (isa ? grp.M : grp.N);
-        const int8_t* src = (isa ? pa + pl * pr.a_pstride : pb + pl * pr.b_pstride) +
-                            (int64_t)g * kpad + k0 + half * 16;
-        rq[q] = in ? *reinterpret_cast<const v4i*>(src) : v4i{0, 0, 0, 0};
-      }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        int c = tid + q * kThreads;
-        if (c >= CH_A + CH_B) continue;
-        const bool isa = c < CH_A;
-        if (!isa) c -= CH_A;
-        const int half = c & 1, row = (c >> 1) % 64, pl = c / 128;
-        if (isa)
-          As[buf][pl][row][half ^ ((row >> 3) & 1)] = rq[q];
-        else
-          Bs[buf][pl][row][half ^ ((row >> 3) & 1)] = rq[q];
+      for (int t = 0; t < NLD_W; ++t) {
+        const int q = wave + 4 * t;
+        if (q >= NLD) break;
+        if (q < NLD_A) {
+          const int b = q % WMB, ch = (q / WMB) & 1, pl = q / (2 * WMB);
+          const int row = min(m0 + b * 64 + lane, grp.M - 1);
+          load_lds16(pa + pl * pr.a_pstride + ((int64_t)(c0 + ch) * grp.M + row) * 16,
+                     sA + (pl * 2 + ch) * BM + b * 64);
+        } else {
+          const int r = q - NLD_A, ch = r & 1, pl = r >> 1;
+          load_lds16(pb + pl * pr.b_pstride + ((int64_t)(c0 + ch) * grp.N + brow) * 16,
+                     sB + (pl * 2 + ch) * BN);
+        }
       }
     };
 
@@ -327,61 +394,74 @@ __global__ void __launch_bounds__(kThreads, CPLX ? 1 : 2)
       pb = planesB + pr.b_planes;
     }
     // every wave is past the previous half's last barrier: buffer 0 is free
-    load(0);
-    store(0);
+    issue(0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     int buf = 0;
     for (int k0 = 0; k0 < kpad; k0 += BKB) {
-      const bool more = k0 + BKB < kpad;
-      const int ra_row = wm0 + l32, rb_row = wn0 + l32;
-      // all B slices stay in registers; A slices stream through one register
-      // quad (keeps the kernel at two waves per SIMD)
-      v4i bv[S];
+      const v4i* sA = lds + buf * (A_ST + B_ST);
+      const v4i* sB = sA + A_ST;
+      v4i bv[S], av[S];
 #pragma unroll
-      for (int j = 0; j < S; ++j) bv[j] = Bs[buf][j][rb_row][lh ^ ((rb_row >> 3) & 1)];
+      for (int j = 0; j < S; ++j) bv[j] = sB[(j * 2 + lh) * BN + wn0 + l32];
 #pragma unroll
-      for (int i = 0; i < S; ++i) {
-        const v4i av = As[buf][i][ra_row][lh ^ ((ra_row >> 3) & 1)];
+      for (int i = 0; i < S; ++i) av[i] = sA[(i * 2 + lh) * BM + wm0 + l32];
+      // next stage into the other buffer (its last readers finished before
+      // the barrier that closed the previous step); it lands while the MFMAs
+      // below run
+      if (k0 + BKB < kpad) issue(k0 + BKB, buf ^ 1);
 #pragma unroll
-        for (int j = 0; j < S - i; ++j)  // slice i of A with slice j of B, level i+j
-          lev[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av, lev[i + j], 0, 0, 0);
+      for (int b = 0; b < WMB; ++b) {
+        if (b > 0) {  // A fragments of the next 32-row block (one set live at a time)
+#pragma unroll
+          for (int i = 0; i < S; ++i) av[i] = sA[(i * 2 + lh) * BM + wm0 + b * 32 + l32];
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+#pragma unroll
+          for (int j = 0; j < S - i; ++j)  // slice i of A with slice j of B, level i+j
+            lev[b][i + j] =
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av[i], lev[b][i + j], 0, 0, 0);
       }
-      // next stage: fetched after the MFMA block so the staging registers are
-      // not live across it; the other wave on this SIMD covers the latency
-      if (more) {
-        load(k0 + BKB);
-        store(buf ^ 1);
-      }
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
       buf ^= 1;
     }
     }  // halves
 
     // FP64 recombination of the levels, scaled per row/column
-    const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
-      const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
-      double h = (double)lev[NL - 1][r];
+    for (int b = 0; b < WMB; ++b) {
+      const int gm = m0 + wm0 + b * 32 + l32;
+      const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
 #pragma unroll
-      for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[L][r]);
-      double v = ldexp(h, ea + eb - 12);
-      if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
-      acc[r] = fma(ga * pr.alpha, v, acc[r]);
+      for (int r = 0; r < 16; ++r) {
+        const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
+        const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
+        double h = (double)lev[b][NL - 1][r];
+#pragma unroll
+        for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[b][L][r]);
+        double v = ldexp(h, ea + eb - 12);
+        if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
+        acc[b][r] = fma(ga * pr.alpha, v, acc[b][r]);
+      }
     }
   }
 
-  if (gm >= grp.M) return;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
-    if (gn >= grp.N) continue;
-    const int64_t o0 = grp.off_c + gm + (int64_t)gn * grp.ldc;
-    const int64_t o = CPLX ? 2 * o0 + comp : o0;
-    double out = acc[r];
-    if (beta != 0.0) out = fma(beta, Cg[o], out);
-    Cg[o] = out;
+  for (int b = 0; b < WMB; ++b) {
+    const int gm = m0 + wm0 + b * 32 + l32;
+    if (gm >= grp.M) continue;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
+      if (gn >= grp.N) continue;
+      const int64_t o0 = grp.off_c + gm + (int64_t)gn * grp.ldc;
+      const int64_t o = CPLX ? 2 * o0 + comp : o0;
+      double out = acc[b][r];
+      if (beta != 0.0) out = fma(beta, Cg[o], out);
+      Cg[o] = out;
+    }
   }
 }
 
@@ -391,40 +471,64 @@ int split_slices(int prec) { return prec == 2 ? kSliceCount : 0; }
 
 hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
                         const int64_t* prefix, int64_t nblk, int64_t grid, const void* src,
-                        void* planes, void* exps) {
+                        void* planes, void* exps, void* pmax) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
-  if (cplx)
-    hipLaunchKernelGGL((slice_kernel<kSliceCount, 1>), dim3((unsigned)grid), dim3(kThreads), 0, s,
-                       blocks, prefix, nblk, (const double*)src, (int8_t*)planes, (int32_t*)exps,
-                       nsets);
-  else
-    hipLaunchKernelGGL((slice_kernel<kSliceCount, 0>), dim3((unsigned)grid), dim3(kThreads), 0, s,
-                       blocks, prefix, nblk, (const double*)src, (int8_t*)planes, (int32_t*)exps,
-                       1);
+  const dim3 g((unsigned)grid), blk(kThreads);
+  if (cplx) {
+    hipLaunchKernelGGL((slice_max_kernel<1>), g, blk, 0, s, blocks, prefix, nblk,
+                       (const double*)src, (int32_t*)pmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((slice_kernel<kSliceCount, 1>), g, blk, 0, s, blocks, prefix, nblk,
+                       (const double*)src, (int8_t*)planes, (int32_t*)exps,
+                       (const int32_t*)pmax, nsets);
+  } else {
+    hipLaunchKernelGGL((slice_max_kernel<0>), g, blk, 0, s, blocks, prefix, nblk,
+                       (const double*)src, (int32_t*)pmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((slice_kernel<kSliceCount, 0>), g, blk, 0, s, blocks, prefix, nblk,
+                       (const double*)src, (int8_t*)planes, (int32_t*)exps,
+                       (const int32_t*)pmax, 1);
+  }
   return hipGetLastError();
 }
 
-int64_t slice_block_wgs(int32_t rows) { return (rows + kRowsPerWG - 1) / kRowsPerWG; }
+int64_t slice_block_wgs(int32_t rows, int32_t kparts) {
+  return (int64_t)((rows + kSliceRows - 1) / kSliceRows) * kparts;
+}
 
-hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int comp, int sets,
+namespace {
+template <int CPLX, int WMB>
+hipError_t launch_sg(hipStream_t s, int comp, int sets, const SplitPair* pairs,
+                     const GemmGroup* groups, const int2* tiles, int64_t grid,
+                     const void* planesA, const void* planesB, const void* expA,
+                     const void* expB, void* C, double ga, double beta) {
+  const int ns = CPLX ? 3 : 1;
+  hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, CPLX, WMB>), dim3((unsigned)grid),
+                     dim3(kThreads), 0, s, pairs, groups, tiles, (const int8_t*)planesA,
+                     (const int8_t*)planesB, (const int32_t*)expA, (const int32_t*)expB,
+                     (double*)C, ga, beta, CPLX ? comp : 0, CPLX ? sets : 0, ns, ns);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int comp, int sets,
                              const SplitPair* pairs, const GemmGroup* groups, const int2* tiles,
                              int64_t grid, const void* planesA, const void* planesB,
                              const void* expA, const void* expB, void* C, double ga,
                              double beta) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
+  // 128-row tiles (WMB = 2) need more than the 512 registers of a wave with
+  // seven levels per 32x32 block; only the 64 x 64 tile is built.
+  if (wmb != 1) return hipErrorInvalidValue;
   if (cplx)
-    hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, 1>), dim3((unsigned)grid), dim3(kThreads),
-                       0, s, pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
-                       (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta, comp,
-                       sets, 3, 3);
-  else
-    hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, 0>), dim3((unsigned)grid), dim3(kThreads),
-                       0, s, pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
-                       (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta, 0, 0, 1,
-                       1);
-  return hipGetLastError();
+    return launch_sg<1, 1>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB, expA,
+                           expB, C, ga, beta);
+  return launch_sg<0, 1>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB, expA, expB,
+                         C, ga, beta);
 }
 
 }  // namespace itg

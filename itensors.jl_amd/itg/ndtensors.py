@@ -41,8 +41,9 @@ _precision = ["fp64"]
 
 def set_precision(mode: str) -> str:
     """Select the arithmetic of GEMM-shaped contractions: "fp64" (FP64 MFMA),
-    "split_bf16" (BF16 slices on MFMA, FP64 recombination) or "split_int8"
-    (INT8 residues on MFMA, CRT recombination). Returns the previous mode."""
+    "split_int8" (FP64 operands as 7 int8 slices under per-row scales, slice
+    products on INT8 MFMA with exact INT32 sums, FP64 recombination) or
+    "split_bf16" (reserved; raises ItgUnsupported). Returns the previous mode."""
     if mode not in _lib.PREC:
         raise ValueError(f"unknown precision mode {mode!r}")
     old = _precision[0]
@@ -71,6 +72,45 @@ def _handle(dev):
     h = _lib.handle_for(dev.index if dev.index is not None else torch.cuda.current_device())
     h.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     return h
+
+
+class timer:
+    """Per-kernel device timing, the counterpart of NDTensors' global
+    `timer::TimerOutput` (NDTensors/src/NDTensors.jl:96-99): inside the block
+    every kernel launched by the library on `device` (outside graph capture)
+    is bracketed by HIP events on its stream; on exit `results` maps kernel
+    name -> (launches, total ms).
+
+        with itg.ndtensors.timer() as t:
+            seq.run_step(0)
+        t.results["split_gemm_kernel"]
+    """
+
+    def __init__(self, device=None):
+        self.dev = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        self.results = {}
+
+    def __enter__(self):
+        h = _handle(self.dev)
+        h.check(_lib.lib().itg_timer_enable(h.h, 1))
+        return self
+
+    def __exit__(self, *exc):
+        import ctypes
+
+        h = _handle(self.dev)
+        L = _lib.lib()
+        n = L.itg_timer_count(h.h)
+        for i in range(n):
+            name = ctypes.c_char_p()
+            calls = ctypes.c_int64()
+            ms = ctypes.c_double()
+            h.check(L.itg_timer_read(h.h, i, ctypes.byref(name), ctypes.byref(calls),
+                                     ctypes.byref(ms)))
+            self.results[name.value.decode()] = (calls.value, ms.value)
+        h.check(L.itg_timer_enable(h.h, 0))
+        return False
 
 
 class DenseTensor:

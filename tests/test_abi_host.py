@@ -74,6 +74,8 @@ def test_null_handle_is_rejected():
     L = _lib.lib()
     assert L.itg_synchronize(None) == _lib.ITG_ERR_INVALID
     assert L.itg_bs_plan_destroy(None) == _lib.ITG_OK
+    assert L.itg_timer_enable(None, 1) == _lib.ITG_ERR_INVALID
+    assert L.itg_timer_count(None) == 0
 
 
 # ---------------------------------------------------------------- host logic
