@@ -37,10 +37,11 @@ struct DirectPair {
 };
 struct DirectGroup {
   int64_t off_c;
+  int64_t wg0;     // first workgroup of the group
   int32_t cdim[kMaxModes];
   int32_t nelem;
   int32_t p0, p1;  // pair range
-  int32_t tiled;   // 1: 2-D tiles (lanes along R mode 0, one column per wave)
+  int32_t pad;
 };
 
 // Grouped GEMM: C (M x N, column-major, ldc = M) of a group is
@@ -77,15 +78,16 @@ hipError_t launch_permute(hipStream_t s, int cplx, int nm, const int32_t* perm_h
 // 1 = 2-D tile transpose of src mode 0 and src mode perm[0].
 int64_t permute_block_tiles(int nm, const int32_t* perm, const int32_t* dims, int mode_kind);
 
+// wg_group: the group of every workgroup (total_tiles entries).
 hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pairs,
                          const DirectGroup* groups, const int32_t* koffs,
-                         const int64_t* tile_prefix, int64_t ngroups, int64_t total_tiles,
-                         const void* A, const void* B, void* C, unsigned flags,
-                         double ga_re, double ga_im, double beta_re, double beta_im);
-constexpr int kDirectElemsPerWG = 256 * 2;
-// Workgroups a direct-contraction group needs; sets g->tiled (host helper
-// that must agree with the kernel's tiling).
-int64_t direct_group_tiles(DirectGroup* g, int nr);
+                         const int32_t* wg_group, int64_t total_tiles, const void* A,
+                         const void* B, void* C, unsigned flags, double ga_re, double ga_im,
+                         double beta_re, double beta_im);
+constexpr int kDirectElemsPerWG = 256 * 4;
+// Workgroups a direct-contraction group needs (kDirectElemsPerWG output
+// elements each; host helper that must agree with the kernel).
+int64_t direct_group_tiles(const DirectGroup* g, int nr);
 
 // Grouped GEMM (FP64 MFMA). cfg selects the tile shape; ta/tb: A stored
 // K x M / B stored N x K. `tiles` holds one {group, tile index in group} per

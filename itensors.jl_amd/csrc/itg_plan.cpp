@@ -324,16 +324,21 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
       y.nelem = (int32_t)ne;
       y.p0 = (int32_t)granges[g].first;
       y.p1 = (int32_t)granges[g].second;
+      y.wg0 = prefix[g];
       prefix[g + 1] = prefix[g] + direct_group_tiles(&y, NR);
     }
     if (koffs.empty()) koffs.push_back(0), koffs.push_back(0);
     ep->d_ngroups = (int64_t)granges.size();
     ep->d_tiles = prefix.back();
+    // workgroup -> group
+    std::vector<int32_t> wgg((size_t)std::max<int64_t>(ep->d_tiles, 1), 0);
+    for (size_t g = 0; g < granges.size(); ++g)
+      for (int64_t w = prefix[g]; w < prefix[g + 1]; ++w) wgg[(size_t)w] = (int32_t)g;
     hipError_t e;
     if ((e = ep->d_pairs.upload(dp.data(), dp.size() * sizeof(DirectPair), s)) != hipSuccess ||
         (e = ep->d_groups.upload(dg.data(), dg.size() * sizeof(DirectGroup), s)) != hipSuccess ||
         (e = ep->d_koffs.upload(koffs.data(), koffs.size() * sizeof(int32_t), s)) != hipSuccess ||
-        (e = ep->d_prefix.upload(prefix.data(), prefix.size() * sizeof(int64_t), s)) != hipSuccess)
+        (e = ep->d_wgg.upload(wgg.data(), wgg.size() * sizeof(int32_t), s)) != hipSuccess)
       return std::string("device table upload failed: ") + hipGetErrorString(e);
     return "";
   }
@@ -890,8 +895,8 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
       TimeScope ts(kt, "direct_kernel", s);
       e = launch_direct(s, cplx, ep.st.NR, (const DirectPair*)ep.d_pairs.p,
                         (const DirectGroup*)ep.d_groups.p, (const int32_t*)ep.d_koffs.p,
-                        (const int64_t*)ep.d_prefix.p, ep.d_ngroups, ep.d_tiles, T1, T2, R,
-                        flags, a_re, a_im, b_re, b_im);
+                        (const int32_t*)ep.d_wgg.p, ep.d_tiles, T1, T2, R, flags, a_re, a_im,
+                        b_re, b_im);
     }
     if (e != hipSuccess) *err = std::string("direct kernel launch: ") + hipGetErrorString(e);
     return e;

@@ -115,7 +115,7 @@ struct ExecPlan {
   Structure st;
   // DIRECT
   int64_t d_ngroups = 0, d_tiles = 0;
-  DevBuf d_pairs, d_groups, d_koffs, d_prefix;
+  DevBuf d_pairs, d_groups, d_koffs, d_wgg;  // d_wgg: workgroup -> group
   // GEMM
   bool swap = false, permA = false, permB = false, permC = false;
   int ta = 0, tb = 0;

@@ -27,149 +27,17 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kPerThread = kDirectElemsPerWG / kThreads;
 
-__device__ inline int64_t find_group(const int64_t* prefix, int64_t n, int64_t wg) {
-  int64_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int64_t mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= wg)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-// 2-D tiling of an output block viewed as (mode 0) x (all other modes): the
-// 64 lanes of a wave run along mode 0 (kRowPasses passes), each wave takes
-// kColsPerWave columns. Column coordinates and the per-pair column offsets
-// are wave-uniform (scalar unit), so a lane's work per element is one
-// multiply-add per operand plus the loads and FMAs.
-constexpr int kRowPasses = 2;
-constexpr int kTileRows = 64 * kRowPasses;
-constexpr int kColsPerWave = 2;
-constexpr int kTileCols = 4 * kColsPerWave;
-
-template <int CPLX, int NR>
-__device__ void direct_tile(const DirectPair* __restrict__ pairs, const DirectGroup& grp,
-                            int64_t t, const int32_t* __restrict__ koffs,
-                            const double* __restrict__ A, const double* __restrict__ B,
-                            double* __restrict__ C, unsigned flags, double ga_re, double ga_im,
-                            double beta_re, double beta_im) {
-  const int d0 = grp.cdim[0];
-  const int row_tiles = (d0 + kTileRows - 1) / kTileRows;
-  const int r0 = (int)(t % row_tiles) * kTileRows;
-  const int64_t c0 = (t / row_tiles) * kTileCols;
-  const uint32_t ncols = (uint32_t)grp.nelem / (uint32_t)d0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const bool conja = (flags & 1u) != 0, conjb = (flags & 2u) != 0;
-  const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
-  for (int cc = 0; cc < kColsPerWave; ++cc) {
-    const uint32_t col = (uint32_t)__builtin_amdgcn_readfirstlane(
-        (int)(c0 + wave * kColsPerWave + cc));
-    if (col >= ncols) break;
-    // coordinates of modes 1.. of this column (wave-uniform)
-    uint32_t cc_coord[NR > 1 ? NR : 1];
-    {
-      uint32_t rem = col;
-#pragma unroll
-      for (int i = 1; i < NR; ++i) {
-        if (i + 1 < NR) {
-          const uint32_t q = rem / (uint32_t)grp.cdim[i];
-          cc_coord[i] = rem - q * (uint32_t)grp.cdim[i];
-          rem = q;
-        } else {
-          cc_coord[i] = rem;
-        }
-      }
-    }
-    double acc_re[kRowPasses], acc_im[kRowPasses];
-#pragma unroll
-    for (int r = 0; r < kRowPasses; ++r) acc_re[r] = acc_im[r] = 0.0;
-    for (int p = grp.p0; p < grp.p1; ++p) {
-      const DirectPair& pr = pairs[p];
-      const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
-      const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
-      int64_t ca = pr.off_a, cb = pr.off_b;
-#pragma unroll
-      for (int i = 1; i < NR; ++i) {
-        ca += (int64_t)cc_coord[i] * pr.cstr_a[i];
-        cb += (int64_t)cc_coord[i] * pr.cstr_b[i];
-      }
-      const double* Ap = A + (CPLX ? 2 : 1) * ca;
-      const double* Bp = B + (CPLX ? 2 : 1) * cb;
-      const int32_t* ko = koffs + 2 * (int64_t)pr.kofs;
-      double s_re[kRowPasses], s_im[kRowPasses];
-#pragma unroll
-      for (int r = 0; r < kRowPasses; ++r) s_re[r] = s_im[r] = 0.0;
-      for (int k = 0; k < pr.nk; ++k) {
-        const int32_t ka = ko[2 * k], kb = ko[2 * k + 1];
-#pragma unroll
-        for (int r = 0; r < kRowPasses; ++r) {
-          const int row = r0 + r * 64 + lane;
-          if (row >= d0) continue;
-          const int64_t oa = (int64_t)row * pr.cstr_a[0] + ka;
-          const int64_t ob = (int64_t)row * pr.cstr_b[0] + kb;
-          if (CPLX) {
-            const double2 x = reinterpret_cast<const double2*>(Ap)[oa];
-            const double2 y = reinterpret_cast<const double2*>(Bp)[ob];
-            const double xi = conja ? -x.y : x.y;
-            const double yi = conjb ? -y.y : y.y;
-            s_re[r] = fma(x.x, y.x, fma(-xi, yi, s_re[r]));
-            s_im[r] = fma(x.x, yi, fma(xi, y.x, s_im[r]));
-          } else {
-            s_re[r] = fma(Ap[oa], Bp[ob], s_re[r]);
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < kRowPasses; ++r) {
-        if (CPLX) {
-          acc_re[r] += al_re * s_re[r] - al_im * s_im[r];
-          acc_im[r] += al_re * s_im[r] + al_im * s_re[r];
-        } else {
-          acc_re[r] = fma(al_re, s_re[r], acc_re[r]);
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < kRowPasses; ++r) {
-      const int row = r0 + r * 64 + lane;
-      if (row >= d0) continue;
-      const int64_t o = grp.off_c + (int64_t)col * d0 + row;
-      if (CPLX) {
-        double2 out = make_double2(acc_re[r], acc_im[r]);
-        if (use_beta) {
-          const double2 c = reinterpret_cast<double2*>(C)[o];
-          out.x += beta_re * c.x - beta_im * c.y;
-          out.y += beta_re * c.y + beta_im * c.x;
-        }
-        reinterpret_cast<double2*>(C)[o] = out;
-      } else {
-        double out = acc_re[r];
-        if (use_beta) out = fma(beta_re, C[o], out);
-        C[o] = out;
-      }
-    }
-  }
-}
-
 template <int CPLX, int NR>
 __global__ void __launch_bounds__(kThreads)
     direct_kernel(const DirectPair* __restrict__ pairs, const DirectGroup* __restrict__ groups,
-                  const int32_t* __restrict__ koffs, const int64_t* __restrict__ prefix,
-                  int64_t ngroups, const double* __restrict__ A, const double* __restrict__ B,
+                  const int32_t* __restrict__ koffs, const int32_t* __restrict__ wg_group,
+                  const double* __restrict__ A, const double* __restrict__ B,
                   double* __restrict__ C, unsigned flags, double ga_re, double ga_im,
                   double beta_re, double beta_im) {
   const int64_t wg = blockIdx.x;
-  const int64_t g = find_group(prefix, ngroups, wg);
-  const DirectGroup& grp = groups[g];
-  if (NR >= 1 && grp.tiled) {
-    direct_tile<CPLX, NR>(pairs, grp, wg - prefix[g], koffs, A, B, C, flags, ga_re, ga_im,
-                          beta_re, beta_im);
-    return;
-  }
+  const DirectGroup& grp = groups[wg_group[wg]];
   const uint32_t nelem = (uint32_t)grp.nelem;
-  const uint32_t e0 = (uint32_t)(wg - prefix[g]) * kDirectElemsPerWG;
+  const uint32_t e0 = (uint32_t)(wg - grp.wg0) * kDirectElemsPerWG;
   const bool conja = (flags & 1u) != 0, conjb = (flags & 2u) != 0;
   const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
   uint32_t cdim[NR > 0 ? NR : 1];
@@ -231,6 +99,9 @@ __global__ void __launch_bounds__(kThreads)
     }
     const int32_t* ko = koffs + 2 * (int64_t)pr.kofs;
     const int nk = pr.nk;
+    // two contracted indices per trip: the loads of both are in flight
+    // together
+#pragma unroll 2
     for (int k = 0; k < nk; ++k) {
       const uint32_t ka = (uint32_t)ko[2 * k], kb = (uint32_t)ko[2 * k + 1];
 #pragma unroll
@@ -281,24 +152,23 @@ __global__ void __launch_bounds__(kThreads)
 
 template <int CPLX, int NR>
 hipError_t launch_nr(hipStream_t s, dim3 grid, const DirectPair* pairs, const DirectGroup* groups,
-                     const int32_t* koffs, const int64_t* prefix, int64_t ngroups, const void* A,
-                     const void* B, void* C, unsigned flags, double ar, double ai, double br,
-                     double bi) {
+                     const int32_t* koffs, const int32_t* wg_group, const void* A, const void* B,
+                     void* C, unsigned flags, double ar, double ai, double br, double bi) {
   hipLaunchKernelGGL((direct_kernel<CPLX, NR>), grid, dim3(kThreads), 0, s, pairs, groups, koffs,
-                     prefix, ngroups, (const double*)A, (const double*)B, (double*)C, flags, ar,
-                     ai, br, bi);
+                     wg_group, (const double*)A, (const double*)B, (double*)C, flags, ar, ai, br,
+                     bi);
   return hipGetLastError();
 }
 
 template <int CPLX>
 hipError_t launch_c(hipStream_t s, int nr, dim3 grid, const DirectPair* pairs,
-                    const DirectGroup* groups, const int32_t* koffs, const int64_t* prefix,
-                    int64_t ngroups, const void* A, const void* B, void* C, unsigned flags,
-                    double ar, double ai, double br, double bi) {
-#define ITG_NR_CASE(n)                                                                        \
-  case n:                                                                                     \
-    return launch_nr<CPLX, n>(s, grid, pairs, groups, koffs, prefix, ngroups, A, B, C, flags, \
-                              ar, ai, br, bi);
+                    const DirectGroup* groups, const int32_t* koffs, const int32_t* wg_group,
+                    const void* A, const void* B, void* C, unsigned flags, double ar, double ai,
+                    double br, double bi) {
+#define ITG_NR_CASE(n)                                                                      \
+  case n:                                                                                   \
+    return launch_nr<CPLX, n>(s, grid, pairs, groups, koffs, wg_group, A, B, C, flags, ar, \
+                              ai, br, bi);
   switch (nr) {
     ITG_NR_CASE(0)
     ITG_NR_CASE(1)
@@ -317,32 +187,23 @@ hipError_t launch_c(hipStream_t s, int nr, dim3 grid, const DirectPair* pairs,
 
 }  // namespace
 
-int64_t direct_group_tiles(DirectGroup* g, int nr) {
-  const int64_t ne = g->nelem;
-  if (ne == 0) {
-    g->tiled = 0;
-    return 0;
-  }
-  // lanes along mode 0 pay off once mode 0 fills most of a wave
-  g->tiled = (nr >= 1 && g->cdim[0] >= 48) ? 1 : 0;
-  if (!g->tiled) return (ne + kDirectElemsPerWG - 1) / kDirectElemsPerWG;
-  const int64_t d0 = g->cdim[0];
-  const int64_t cols = ne / d0;
-  return ((d0 + kTileRows - 1) / kTileRows) * ((cols + kTileCols - 1) / kTileCols);
+int64_t direct_group_tiles(const DirectGroup* g, int nr) {
+  (void)nr;
+  return ((int64_t)g->nelem + kDirectElemsPerWG - 1) / kDirectElemsPerWG;
 }
 
 hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pairs,
                          const DirectGroup* groups, const int32_t* koffs,
-                         const int64_t* tile_prefix, int64_t ngroups, int64_t total_tiles,
-                         const void* A, const void* B, void* C, unsigned flags, double ga_re,
-                         double ga_im, double beta_re, double beta_im) {
+                         const int32_t* wg_group, int64_t total_tiles, const void* A,
+                         const void* B, void* C, unsigned flags, double ga_re, double ga_im,
+                         double beta_re, double beta_im) {
   if (total_tiles == 0) return hipSuccess;
   dim3 grid((unsigned)total_tiles);
   if (cplx)
-    return launch_c<1>(s, nr, grid, pairs, groups, koffs, tile_prefix, ngroups, A, B, C, flags,
-                       ga_re, ga_im, beta_re, beta_im);
-  return launch_c<0>(s, nr, grid, pairs, groups, koffs, tile_prefix, ngroups, A, B, C, flags,
-                     ga_re, ga_im, beta_re, beta_im);
+    return launch_c<1>(s, nr, grid, pairs, groups, koffs, wg_group, A, B, C, flags, ga_re,
+                       ga_im, beta_re, beta_im);
+  return launch_c<0>(s, nr, grid, pairs, groups, koffs, wg_group, A, B, C, flags, ga_re, ga_im,
+                     beta_re, beta_im);
 }
 
 }  // namespace itg

@@ -141,6 +141,13 @@ __device__ inline SliceTile slice_tile(const SliceBlock* blocks, const int64_t* 
   return t;
 }
 
+// srow[i] = element offset of row min(r0 + i, R - 1) of block d (threads
+// 0..kSliceRows-1 write; the caller synchronises).
+__device__ inline void load_row_offsets(const SliceBlock& d, int r0, int tid, int64_t* srow) {
+  if (tid < kSliceRows)
+    srow[tid] = mode_offset((uint32_t)min(r0 + tid, d.R - 1), d.nrm, d.rdim, d.rstr);
+}
+
 // Pass 1: for each row of a (64 rows) x (kSliceKPart) tile, the exponent of
 // the largest magnitude in the tile's K range (kZeroRow if all are zero,
 // kNonFinite if any is Inf/NaN), stored as partial kp of the row. The source
@@ -151,41 +158,71 @@ __global__ void __launch_bounds__(kThreads)
     slice_max_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
                      int64_t nblk, const double* __restrict__ src, int32_t* __restrict__ pmax) {
   __shared__ double red[4][kSliceRows];
+  __shared__ int64_t srow[kSliceRows];
   const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
   const SliceBlock& d = blocks[t.b];
   const int tid = threadIdx.x;
   const double* x = src + (CPLX ? 2 : 1) * d.off;
   const int k_lo = t.kp * kSliceKPart, k_hi = min(d.K, k_lo + kSliceKPart);
   const bool kfast = d.nkm == 1 && d.kstr[0] == 1;
+  // Every load below is unconditional, from an offset clamped into the block
+  // (rows past R onto row R-1, K past the range onto its first element), and
+  // the value is dropped afterwards: conditional loads end up in branches of
+  // their own and serialise.
+  load_row_offsets(d, t.r0, tid, srow);
+  __syncthreads();
   if (!kfast) {
-    // lanes along the rows; thread part p walks its quarter of the K range
-    constexpr int kQ = kSliceKPart / 4;
+    // lanes along the rows; thread part p walks its quarter of the K range,
+    // 8 elements at a time: the 8 offsets first, then 8 loads in flight
+    constexpr int kQ = kSliceKPart / 4, kU = 8;
     const int row = tid % kSliceRows, part = tid / kSliceRows;
     double m = 0.0;
     const int ka = k_lo + part * kQ, kb = min(k_hi, ka + kQ);
     if (t.r0 + row < d.R && ka < kb) {
-      const int64_t ro = mode_offset((uint32_t)(t.r0 + row), d.nrm, d.rdim, d.rstr);
+      const int64_t ro = srow[row];
       KWalk w;
       w.init((uint32_t)ka, d);
-      for (int k = ka; k < kb; ++k) {
-        m = fmax(m, elem_mag<CPLX>(x, ro + w.off));
-        w.step(d);
+      const int64_t o0 = w.off;
+      for (int k = ka; k < kb; k += kU) {
+        int64_t o[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          o[u] = (k + u < kb) ? w.off : o0;
+          w.step(d);
+        }
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) v[u] = elem_mag<CPLX>(x, ro + o[u]);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) m = fmax(m, (k + u < kb) ? v[u] : 0.0);
       }
     }
     red[part][row] = m;
   } else {
-    // one wave per row at a time, lanes along K
+    // lanes along K (two elements each: the part is 128 wide); a wave takes
+    // rows wave, wave+4, ... and issues the loads of all of them before
+    // reducing
+    static_assert(kSliceKPart == 128, "two K elements per lane");
     const int lane = tid & 63, wave = tid >> 6;
-    for (int row = wave; row < kSliceRows; row += 4) {
-      double m = 0.0;
-      if (t.r0 + row < d.R) {
-        const int64_t ro = mode_offset((uint32_t)(t.r0 + row), d.nrm, d.rdim, d.rstr);
-        for (int k = k_lo + lane; k < k_hi; k += 64) m = fmax(m, elem_mag<CPLX>(x, ro + k));
-      }
+    constexpr int kRowsW = kSliceRows / 4;
+    const int k0 = k_lo + lane, k1 = k0 + 64;
+    const int c0 = min(k0, k_hi - 1), c1 = min(k1, k_hi - 1);
+    double m[kRowsW];
 #pragma unroll
-      for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s));
+    for (int i = 0; i < kRowsW; ++i) {
+      const int64_t ro = srow[wave + 4 * i];
+      const double a = elem_mag<CPLX>(x, ro + c0);
+      const double b = elem_mag<CPLX>(x, ro + c1);
+      m[i] = fmax(k0 < k_hi ? a : 0.0, k1 < k_hi ? b : 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < kRowsW; ++i) {
+      double mm = m[i];
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1) mm = fmax(mm, __shfl_xor(mm, s));
       if (lane == 0) {
-        red[0][row] = m;
+        const int row = wave + 4 * i;
+        red[0][row] = t.r0 + row < d.R ? mm : 0.0;
         red[1][row] = red[2][row] = red[3][row] = 0.0;
       }
     }
@@ -212,11 +249,13 @@ __global__ void __launch_bounds__(kThreads)
                  int64_t nblk, const double* __restrict__ src, int8_t* __restrict__ planes,
                  int32_t* __restrict__ exps, const int32_t* __restrict__ pmax, int nsets) {
   __shared__ int32_t sexp[kSliceRows];
+  __shared__ int64_t srow[kSliceRows];
   const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
   const SliceBlock& d = blocks[t.b];
   const int r0 = t.r0;
   const int tid = threadIdx.x;
   const double* x = src + (CPLX ? 2 : 1) * d.off;
+  load_row_offsets(d, r0, tid, srow);
   if (tid < kSliceRows) {
     int e = kZeroRow;
     if (r0 + tid < d.R)
@@ -241,7 +280,7 @@ __global__ void __launch_bounds__(kThreads)
     const int gr = r0 + row;
     if (gr >= d.R || ch >= ch_end) continue;
     const int e = sexp[row];
-    const int64_t ro = mode_offset((uint32_t)gr, d.nrm, d.rdim, d.rstr);
+    const int64_t ro = srow[row];
     KWalk kw;
     kw.init((uint32_t)(ch * 16), d);
     constexpr int NC = CPLX ? 3 : 1;  // re, im, -im
@@ -252,20 +291,32 @@ __global__ void __launch_bounds__(kThreads)
       for (int i = 0; i < S; ++i)
 #pragma unroll
         for (int w = 0; w < 4; ++w) packed[c][i][w] = 0u;
+    // the 16 offsets first (padding k >= K clamped onto the row's k = 0),
+    // then the 16 loads back to back and unconditional, then the digits
+    int64_t off[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const int k = ch * 16 + j;
-      double y = 0.0, yi = 0.0;
-      if (k < d.K && e != kNonFinite) {
-        if (CPLX) {
-          const double2 v = reinterpret_cast<const double2*>(x)[ro + kw.off];
-          y = ldexp(v.x, 6 - e);
-          yi = ldexp(v.y, 6 - e);
-        } else {
-          y = ldexp(x[ro + kw.off], 6 - e);
-        }
-      }
+      off[j] = ro + (ch * 16 + j < d.K ? kw.off : 0);
       kw.step(d);
+    }
+    const bool ok = e != kNonFinite;
+    double xr[16], xi[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const bool in = ok && ch * 16 + j < d.K;
+      if (CPLX) {
+        const double2 v = reinterpret_cast<const double2*>(x)[off[j]];
+        xr[j] = in ? v.x : 0.0;
+        xi[j] = in ? v.y : 0.0;
+      } else {
+        const double v = x[off[j]];
+        xr[j] = in ? v : 0.0;
+        xi[j] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      double y = ldexp(xr[j], 6 - e), yi = ldexp(xi[j], 6 - e);
 #pragma unroll
       for (int i = 0; i < S; ++i) {
         const double v = rint(y);
@@ -423,6 +474,10 @@ __global__ void __launch_bounds__(kThreads, WMB == 1 ? 2 : 1)
             lev[b][i + j] =
                 __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av[i], lev[b][i + j], 0, 0, 0);
       }
+      // The MFMAs only touch registers, so without this fence the scheduler
+      // sinks them below the wait for the next stage, which then serialises
+      // the fetch with the math.
+      __builtin_amdgcn_sched_barrier(0);
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
       buf ^= 1;
