@@ -41,7 +41,101 @@ def parse():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB 5 times (a known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the dominant "
+                         "kernel's HBM traffic")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="(internal) run step 1 in every mode a few times plus a 256 MiB copy "
+                         "and exit; the workload of the counter passes")
     return ap.parse_args()
+
+
+CALIB_MIB = 256
+KERNELS = {"fp64": "gemm_f64_kernel", "split_int8": "split_gemm_kernel"}
+
+
+def traffic_probe(args):
+    """Step 1 (the dominant GEMM) three times in each mode, then a copy of
+    CALIB_MIB MiB three times (the calibration for the counters)."""
+    import torch
+    import itg
+    from itg.configs import heff_network
+    from itg.network import ContractionSequence
+
+    from itg import ndtensors as ND
+
+    dev = torch.device("cuda:0")
+    L, psi, W1, W2, R, ix = heff_network(args.chi, qns=True, seed=0, device=dev)
+    seq = ContractionSequence([L, psi, W1, W2, R])
+    counts = {}
+    for m, kname in KERNELS.items():
+        try:
+            seq.autotune(m)
+            with ND.timer(dev) as t:  # GEMM launches per step with the tuned tiles
+                seq.run_step(0, m)
+                torch.cuda.synchronize()
+            counts[m] = t.results.get(kname, (0, 0.0))[0]
+            for _ in range(3):
+                seq.run_step(0, m)
+        except itg.ItgUnsupported:
+            pass
+    torch.cuda.synchronize()
+    with open("probe_counts.json", "w") as f:  # in the counter pass's own directory
+        json.dump(counts, f)
+    n = CALIB_MIB * (1 << 20) // 8
+    src = torch.randn(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of each mode's step-1 GEMM from two rocprofv3
+    counter passes (FETCH_SIZE, then WRITE_SIZE) over --traffic-probe,
+    each scaled by what the counter reads for the CALIB_MIB copy (which
+    moves CALIB_MIB MiB each way). Runs before this process touches the
+    GPU; returns {mode: (read_bytes, write_bytes)} or {} when unavailable."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return {}
+    out = {m: [None, None] for m in KERNELS}
+    for j, counter in enumerate(("FETCH_SIZE", "WRITE_SIZE")):
+        with tempfile.TemporaryDirectory() as d:
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-probe",
+                   "--chi", str(args.chi)]
+            p = subprocess.Popen(cmd, cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                 start_new_session=True)
+            try:
+                p.wait(timeout=180)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.wait()
+                return {}
+            path = os.path.join(d, "t_counter_collection.csv")
+            cpath = os.path.join(d, "probe_counts.json")
+            if not os.path.exists(path) or not os.path.exists(cpath):
+                return {}
+            rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+            counts = json.load(open(cpath))
+        copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        if not copies or max(copies) <= 0:
+            return {}
+        scale = CALIB_MIB * 1024.0 / max(copies)  # counters read KiB
+        for m, kname in KERNELS.items():
+            n = int(counts.get(m, 0))
+            v = [float(r["Counter_Value"]) for r in rows if kname in r["Kernel_Name"]]
+            if n > 0 and len(v) >= 3 * n:
+                # the last 3 steps, after tuning: bytes per launch
+                out[m][j] = sum(v[-3 * n:]) / (3 * n) * scale * 1024.0
+    return {m: tuple(v) for m, v in out.items() if None not in v}
 
 
 def log(args, *a):
@@ -145,6 +239,13 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (the "
                           "multi-GPU sharding is described in DESIGN.md and not implemented)"}))
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
+    # counter passes first: they run as child processes, before this process
+    # initialises the GPU
+    traffic = {} if args.no_traffic else measure_traffic(args)
+    log(args, f"[bench] HBM bytes per step-1 GEMM launch (read, write): {traffic}")
     import torch
     import itg
     from itg.configs import heff_network, heff_flops
@@ -212,20 +313,26 @@ def main():
     kt1 = kernel_times(torch, lambda: seq.run_step(0, head), max(5, args.steps))
     launches, k_ms = kt1[kname]
     achieved = step_flops[0] / (launches * k_ms * 1e-3) / 1e12
+    tr = traffic.get(head)
+    traffic_bytes = (tr[0] + tr[1]) if tr else None
     if head == "fp64":
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes,
                 "kernel": f"gemm_f64_kernel, step 1 (L*psi), {launches:g} launch(es) per step, "
                           f"{k_ms:.4f} ms per launch"}
     else:
         # the slice products run on INT8 MFMA: 28 products per FP64 product
         int8_ops = achieved * 28.0
         roof = {"bound": "mfma", "achieved": int8_ops, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                "frac": int8_ops / INT8_PEAK_TOPS, "traffic": None,
+                "frac": int8_ops / INT8_PEAK_TOPS, "traffic": traffic_bytes,
                 "fp64_equivalent_tflops": achieved,
                 "kernel": f"split_gemm_kernel, step 1 (L*psi), {launches:g} launch(es) per step, "
                           f"{k_ms:.4f} ms per launch; achieved = 28 int8 slice products per "
                           f"algorithmic FP64 flop (padding not counted)"}
+    if tr:
+        roof["traffic_read_write_bytes"] = [tr[0], tr[1]]
+        roof["traffic_note"] = ("HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, "
+                                f"each scaled by its reading for a {CALIB_MIB} MiB copy")
     out = {
         "metric": METRIC,
         "value": r["gflops"],

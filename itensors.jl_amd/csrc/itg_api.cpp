@@ -119,7 +119,7 @@ KernelTimer* active_timer(itg_handle* h) {
 int pick_cfg(itg_handle* h, const ExecPlan& ep, int cplx, int prec) {
   auto it = h->tuned.find(tune_key(ep, cplx, prec));
   if (it != h->tuned.end()) return it->second;
-  return default_gemm_cfg(ep, cplx);
+  return prec != 0 ? default_split_cfg(ep) : default_gemm_cfg(ep, cplx);
 }
 
 itg_status run_plan(itg_handle* h, ExecPlan& ep, int cplx, int prec, const void* T1,
@@ -681,7 +681,7 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   const int cplx = dtype == ITG_C128;
   DeviceGuard g(h->device);
   ExecPlan& ep = *p->ep;
-  if (prec != ITG_PREC_FP64) return ITG_OK;  // the split GEMM has one tile shape
+  if (prec != ITG_PREC_FP64 && !ep.split_ok) return ITG_OK;
   hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx, (int)prec));
   if (e != hipSuccess) return hip_fail(h, e, "workspace allocation");
   hipEvent_t t0, t1;
@@ -694,9 +694,13 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   float best_ms = 1e30f;
   std::string err;
   std::vector<int> cands;
-  for (int c = 0; c < gemm_num_cfgs(); ++c)
-    if (gemm_cfg_ok(c, cplx)) cands.push_back(c);
-  cands.push_back(kCfgMixed);
+  if (prec != ITG_PREC_FP64) {
+    cands = {0, 2};  // split tiles 64 x 64 and 128 x 64
+  } else {
+    for (int c = 0; c < gemm_num_cfgs(); ++c)
+      if (gemm_cfg_ok(c, cplx)) cands.push_back(c);
+    cands.push_back(kCfgMixed);
+  }
   for (int cfg : cands) {
     // warm-up, then 3 timed runs
     e = exec_run(ep, cplx, (int)prec, cfg, h->stream, h->ws, T1, T2, R, 0u, 1.0, 0.0, 0.0, 0.0, &err);

@@ -85,6 +85,42 @@ hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pair
                          const void* B, void* C, unsigned flags, double ga_re, double ga_im,
                          double beta_re, double beta_im);
 constexpr int kDirectElemsPerWG = 256 * 4;
+
+// Fiber-stationary direct contraction (small contracted extent, one operand
+// much smaller than the other). "A" is the large operand, "B" the small one;
+// a fiber is one value of A's free (uncontracted) multi-index. A group is a
+// set of A blocks and output blocks that share their A-free sector: every
+// output element of the group is sum_p alpha_p sum_k A_p[fiber, k] B_p[k, g]
+// over the group's pairs p of that output block, g running over B's free
+// multi-index. One thread owns a fiber and writes all of its outputs, so A
+// is fetched from HBM once.
+constexpr int kFibersPerWG = 256 * 4;
+struct FiberGroup {
+  int64_t nfib;              // fibers (product of fdim)
+  int64_t wg0;               // first workgroup of the group
+  int32_t fdim[kMaxModes];   // extents of A's free modes, in A's mode order
+  int32_t r0, r1;            // FiberOut range
+};
+struct FiberOut {            // one output block of a group
+  int64_t off_c;
+  int32_t cstr[kMaxModes];   // output strides of the A-free modes (fdim order)
+  int32_t g0, ng;            // gofs_c[g0 .. g0 + ng): output offsets of the B-free elements
+  int32_t p0, p1;            // FiberPair range (plan order)
+};
+struct FiberPair {
+  int64_t off_a, off_b;
+  double alpha_re, alpha_im;
+  int32_t astr[kMaxModes];   // A strides of the A-free modes (fdim order)
+  int32_t kofs, nk;          // koffs[2*(kofs+k)], [..+1]: A / B offsets of contracted index k
+  int32_t gb0;               // gofs_b[gb0 + g]: B offset of B-free element g
+  int32_t pad;
+};
+// nf: number of A-free modes (1..kMaxModes). flags bit 0 / 1: conj(A) / conj(B).
+hipError_t launch_fiber(hipStream_t s, int cplx, int nf, const FiberGroup* groups,
+                        const FiberOut* outs, const FiberPair* pairs, const int32_t* koffs,
+                        const int32_t* gofs_c, const int32_t* gofs_b, const int32_t* wg_group,
+                        int64_t total_wgs, const void* A, const void* B, void* C, unsigned flags,
+                        double ga_re, double ga_im, double beta_re, double beta_im);
 // Workgroups a direct-contraction group needs (kDirectElemsPerWG output
 // elements each; host helper that must agree with the kernel).
 int64_t direct_group_tiles(const DirectGroup* g, int nr);
@@ -144,8 +180,8 @@ int64_t slice_block_wgs(int32_t rows, int32_t kparts);
 hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
                         const int64_t* prefix, int64_t nblk, int64_t grid, const void* src,
                         void* planes, void* exps, void* pmax);
-// Tile shape of the split GEMM: 64 x 64 (wmb = 1, tile table of gemm cfg 0);
-// other wmb values return hipErrorInvalidValue.
+// Tile shape of the split GEMM: 64 x 64 (wmb = 1, 256 threads, tile table of
+// gemm cfg 0) or 128 x 64 (wmb = 2, 512 threads, tile table of gemm cfg 2).
 // Complex: one launch per output component comp (0 = re, 1 = im); `sets`
 // packs the slice sets (0 re, 1 im, 2 -im) of A and B for the two K halves:
 // bits 0-1 A half 0, 2-3 B half 0, 4-5 A half 1, 6-7 B half 1. Both complex

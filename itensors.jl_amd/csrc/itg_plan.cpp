@@ -202,6 +202,227 @@ bool is_identity(const std::vector<int>& v) {
   return true;
 }
 
+int64_t block_size(const BlockList& T, int64_t b) {
+  int64_t n = 1;
+  for (int i = 0; i < T.N; ++i) n *= T.d(b)[i];
+  return n;
+}
+
+// Column-major strides of block b of T.
+void block_strides(const BlockList& T, int64_t b, int64_t* str) {
+  int64_t a = 1;
+  for (int i = 0; i < T.N; ++i) {
+    str[i] = a;
+    a *= T.d(b)[i];
+  }
+}
+
+int64_t uf_find(std::vector<int64_t>& par, int64_t x) {
+  while (par[x] != x) x = par[x] = par[par[x]];
+  return x;
+}
+
+// FIBER tables. Returns false (nothing built) when the shape does not suit:
+// the smaller operand is not at most a quarter of the larger, the larger has
+// no free mode, too few fibers, or too many B-free elements per output block.
+// On an upload failure returns true with *err set.
+bool build_fiber_tables(const Structure& st, const BlockList& b1, const BlockList& b2,
+                        const BlockList& bR, const std::vector<PairSpec>& pairs,
+                        const std::vector<std::pair<size_t, size_t>>& granges,
+                        const double* pair_alpha, hipStream_t s, ExecPlan* ep,
+                        std::string* err) {
+  const int NR = st.NR, nk = st.nk;
+  // operand sizes over the blocks the pairs touch
+  int64_t size1 = 0, size2 = 0;
+  {
+    std::vector<char> u1(b1.nblk(), 0), u2(b2.nblk(), 0);
+    for (const PairSpec& p : pairs) {
+      if (!u1[p.blk1]) u1[p.blk1] = 1, size1 += block_size(b1, p.blk1);
+      if (!u2[p.blk2]) u2[p.blk2] = 1, size2 += block_size(b2, p.blk2);
+    }
+  }
+  const bool a2 = size2 > size1;
+  const BlockList& TA = a2 ? b2 : b1;
+  const BlockList& TB = a2 ? b1 : b2;
+  if (4 * (a2 ? size1 : size2) > (a2 ? size2 : size1)) return false;
+  const int srcA = a2 ? 2 : 1;
+  // A-free modes in A's mode order: (A position, R position)
+  std::vector<std::pair<int, int>> fm;
+  std::vector<std::pair<int, int>> gm;  // B-free modes in R order: (B position, R position)
+  for (int r = 0; r < NR; ++r) {
+    if (st.r_src[r] == srcA)
+      fm.push_back({st.r_pos[r], r});
+    else
+      gm.push_back({st.r_pos[r], r});
+  }
+  std::sort(fm.begin(), fm.end());
+  const int nf = (int)fm.size();
+  if (nf == 0 || nf > kMaxModes) return false;
+  std::vector<int> kA(nk), kB(nk);
+  for (int i = 0; i < nk; ++i) {
+    kA[i] = a2 ? st.k2[i] : st.k1[i];
+    kB[i] = a2 ? st.k1[i] : st.k2[i];
+  }
+  auto blkA = [&](const PairSpec& p) { return a2 ? p.blk2 : p.blk1; };
+  auto blkB = [&](const PairSpec& p) { return a2 ? p.blk1 : p.blk2; };
+
+  // groups: connected components of (A block) -- (output block) over pairs
+  const int64_t NA = TA.nblk();
+  std::vector<int64_t> par((size_t)(NA + bR.nblk()));
+  std::iota(par.begin(), par.end(), 0);
+  for (const PairSpec& p : pairs) {
+    const int64_t x = uf_find(par, blkA(p)), y = uf_find(par, NA + p.blkR);
+    if (x != y) par[x] = y;
+  }
+  std::map<int64_t, int> comp_of_root;
+  std::vector<std::vector<size_t>> comp_granges;  // indices into granges, plan order
+  for (size_t gi = 0; gi < granges.size(); ++gi) {
+    const int64_t root = uf_find(par, NA + pairs[granges[gi].first].blkR);
+    auto it = comp_of_root.find(root);
+    int c;
+    if (it == comp_of_root.end()) {
+      c = (int)comp_granges.size();
+      comp_of_root[root] = c;
+      comp_granges.emplace_back();
+    } else {
+      c = it->second;
+    }
+    comp_granges[(size_t)c].push_back(gi);
+  }
+
+  std::vector<FiberGroup> fg(comp_granges.size());
+  std::vector<FiberOut> fo;
+  std::vector<FiberPair> fp(pairs.size());
+  std::vector<int32_t> koffs, gofs_c, gofs_b;
+  int64_t total_fib = 0, wgs = 0;
+  std::vector<int32_t> wgg;
+  for (size_t c = 0; c < comp_granges.size(); ++c) {
+    FiberGroup& G = fg[c];
+    std::memset(&G, 0, sizeof(G));
+    const int64_t a0 = blkA(pairs[granges[comp_granges[c][0]].first]);
+    int64_t nfib = 1;
+    for (int i = 0; i < kMaxModes; ++i) G.fdim[i] = 1;
+    for (int i = 0; i < nf; ++i) {
+      G.fdim[i] = TA.d(a0)[fm[(size_t)i].first];
+      nfib *= G.fdim[i];
+    }
+    if (nfib > INT32_MAX) return false;
+    G.nfib = nfib;
+    G.r0 = (int32_t)fo.size();
+    for (size_t gi : comp_granges[c]) {
+      const size_t q0 = granges[gi].first, q1 = granges[gi].second;
+      const int64_t br = pairs[q0].blkR;
+      int64_t sR[kMaxModes];
+      block_strides(bR, br, sR);
+      FiberOut O;
+      std::memset(&O, 0, sizeof(O));
+      O.off_c = bR.offs[br];
+      for (int i = 0; i < nf; ++i) O.cstr[i] = (int32_t)sR[fm[(size_t)i].second];
+      // B-free elements of this output block, column-major over gm (R order)
+      int64_t ng = 1;
+      for (const auto& m : gm) ng *= bR.d(br)[m.second];
+      if (ng > 1024) return false;
+      O.g0 = (int32_t)gofs_c.size();
+      O.ng = (int32_t)ng;
+      {
+        std::vector<int32_t> cc(gm.size(), 0);
+        for (int64_t g = 0; g < ng; ++g) {
+          int64_t o = 0;
+          for (size_t j = 0; j < gm.size(); ++j) o += (int64_t)cc[j] * sR[gm[j].second];
+          gofs_c.push_back((int32_t)o);
+          for (size_t j = 0; j < gm.size(); ++j) {
+            if (++cc[j] < bR.d(br)[gm[j].second]) break;
+            cc[j] = 0;
+          }
+        }
+      }
+      O.p0 = (int32_t)q0;
+      O.p1 = (int32_t)q1;
+      for (size_t q = q0; q < q1; ++q) {
+        const PairSpec& P = pairs[q];
+        const int64_t ba = blkA(P), bb = blkB(P);
+        for (int i = 0; i < nf; ++i)
+          if (TA.d(ba)[fm[(size_t)i].first] != G.fdim[i]) return false;  // not one sector
+        int64_t sA[kMaxModes], sB[kMaxModes];
+        block_strides(TA, ba, sA);
+        block_strides(TB, bb, sB);
+        FiberPair& X = fp[q];
+        std::memset(&X, 0, sizeof(X));
+        X.off_a = TA.offs[ba];
+        X.off_b = TB.offs[bb];
+        if (pair_alpha) {
+          X.alpha_re = pair_alpha[2 * q];
+          X.alpha_im = pair_alpha[2 * q + 1];
+        } else {
+          X.alpha_re = 1.0;
+        }
+        for (int i = 0; i < nf; ++i) X.astr[i] = (int32_t)sA[fm[(size_t)i].first];
+        // contracted multi-index, first contracted mode fastest
+        int64_t K = 1;
+        for (int i = 0; i < nk; ++i) K *= TA.d(ba)[kA[(size_t)i]];
+        X.kofs = (int32_t)(koffs.size() / 2);
+        X.nk = (int32_t)K;
+        {
+          std::vector<int32_t> kc((size_t)std::max(nk, 1), 0);
+          for (int64_t k = 0; k < K; ++k) {
+            int64_t oa = 0, ob = 0;
+            for (int i = 0; i < nk; ++i) {
+              oa += (int64_t)kc[(size_t)i] * sA[kA[(size_t)i]];
+              ob += (int64_t)kc[(size_t)i] * sB[kB[(size_t)i]];
+            }
+            koffs.push_back((int32_t)oa);
+            koffs.push_back((int32_t)ob);
+            for (int i = 0; i < nk; ++i) {
+              if (++kc[(size_t)i] < TA.d(ba)[kA[(size_t)i]]) break;
+              kc[(size_t)i] = 0;
+            }
+          }
+        }
+        X.gb0 = (int32_t)gofs_b.size();
+        {
+          std::vector<int32_t> cc(gm.size(), 0);
+          for (int64_t g = 0; g < ng; ++g) {
+            int64_t o = 0;
+            for (size_t j = 0; j < gm.size(); ++j) o += (int64_t)cc[j] * sB[gm[j].first];
+            gofs_b.push_back((int32_t)o);
+            for (size_t j = 0; j < gm.size(); ++j) {
+              if (++cc[j] < bR.d(br)[gm[j].second]) break;
+              cc[j] = 0;
+            }
+          }
+        }
+      }
+      fo.push_back(O);
+    }
+    G.r1 = (int32_t)fo.size();
+    G.wg0 = wgs;
+    const int64_t n = (nfib + kFibersPerWG - 1) / kFibersPerWG;
+    for (int64_t w = 0; w < n; ++w) wgg.push_back((int32_t)c);
+    wgs += n;
+    total_fib += nfib;
+  }
+  if (total_fib < 256) return false;
+  if (koffs.empty()) koffs.push_back(0), koffs.push_back(0);
+  if (gofs_c.empty()) gofs_c.push_back(0);
+  if (gofs_b.empty()) gofs_b.push_back(0);
+  if (wgg.empty()) wgg.push_back(0);
+  ep->f_a_is_2 = a2;
+  ep->f_nf = nf;
+  ep->f_wgs = wgs;
+  hipError_t e;
+  if ((e = ep->f_groups.upload(fg.data(), fg.size() * sizeof(FiberGroup), s)) != hipSuccess ||
+      (e = ep->f_outs.upload(fo.data(), fo.size() * sizeof(FiberOut), s)) != hipSuccess ||
+      (e = ep->f_pairs.upload(fp.data(), fp.size() * sizeof(FiberPair), s)) != hipSuccess ||
+      (e = ep->f_koffs.upload(koffs.data(), koffs.size() * sizeof(int32_t), s)) != hipSuccess ||
+      (e = ep->f_gofs_c.upload(gofs_c.data(), gofs_c.size() * sizeof(int32_t), s)) !=
+          hipSuccess ||
+      (e = ep->f_gofs_b.upload(gofs_b.data(), gofs_b.size() * sizeof(int32_t), s)) !=
+          hipSuccess ||
+      (e = ep->f_wgg.upload(wgg.data(), wgg.size() * sizeof(int32_t), s)) != hipSuccess)
+    *err = std::string("device table upload failed: ") + hipGetErrorString(e);
+  return true;
+}
+
 }  // namespace
 
 hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
@@ -258,6 +479,11 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
   };
 
   if (!use_gemm) {
+    std::string ferr;
+    if (build_fiber_tables(st, b1, b2, bR, pairs, granges, pair_alpha, s, ep, &ferr)) {
+      ep->strategy = ExecPlan::FIBER;
+      return ferr;
+    }
     ep->strategy = ExecPlan::DIRECT;
     std::vector<DirectPair> dp(pairs.size());
     std::vector<DirectGroup> dg(granges.size());
@@ -655,6 +881,13 @@ int default_gemm_cfg(const ExecPlan& ep, int cplx) {
   return (tot > 0 && big >= 0.5 * tot) ? 1 : 0;
 }
 
+int default_split_cfg(const ExecPlan& ep) {
+  int64_t tiles = 0;
+  for (const GemmGroup& G : ep.h_groups)
+    if (G.M > 0 && G.N > 0) tiles += (int64_t)((G.M + 127) / 128) * ((G.N + 63) / 64);
+  return tiles >= 2 * 256 ? 2 : 0;
+}
+
 int mixed_big_cfg(int cplx) { return cplx ? 2 : 1; }
 
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
@@ -841,9 +1074,9 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
     *err = std::string("slice launch: ") + hipGetErrorString(e);
     return e;
   }
-  (void)cfg;            // one split tile shape: 64 x 64
-  const int wmb = 1;
-  const int tcfg = 0;   // the FP64 tile table of the same shape
+  // cfg 2: 128 x 64 split tiles (three-stage), otherwise 64 x 64
+  const int wmb = cfg == 2 ? 2 : 1;
+  const int tcfg = wmb == 2 ? 2 : 0;  // the FP64 tile table of the same shape
   if ((e = ensure_gemm_launch(ep, tcfg, 0, s)) != hipSuccess) {
     *err = std::string("gemm table upload: ") + hipGetErrorString(e);
     return e;
@@ -899,6 +1132,21 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
                         b_re, b_im);
     }
     if (e != hipSuccess) *err = std::string("direct kernel launch: ") + hipGetErrorString(e);
+    return e;
+  }
+  if (ep.strategy == ExecPlan::FIBER) {
+    const void* A = ep.f_a_is_2 ? T2 : T1;
+    const void* B = ep.f_a_is_2 ? T1 : T2;
+    const unsigned fl = ep.f_a_is_2 ? (((flags & 1u) << 1) | ((flags & 2u) >> 1)) : flags;
+    {
+      TimeScope ts(kt, "fiber_kernel", s);
+      e = launch_fiber(s, cplx, ep.f_nf, (const FiberGroup*)ep.f_groups.p,
+                       (const FiberOut*)ep.f_outs.p, (const FiberPair*)ep.f_pairs.p,
+                       (const int32_t*)ep.f_koffs.p, (const int32_t*)ep.f_gofs_c.p,
+                       (const int32_t*)ep.f_gofs_b.p, (const int32_t*)ep.f_wgg.p, ep.f_wgs, A, B,
+                       R, fl, a_re, a_im, b_re, b_im);
+    }
+    if (e != hipSuccess) *err = std::string("fiber kernel launch: ") + hipGetErrorString(e);
     return e;
   }
   if (prec != 0)

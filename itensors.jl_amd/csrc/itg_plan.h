@@ -5,6 +5,9 @@
 // The planner picks one of two strategies for the whole list:
 //   DIRECT : small contracted extent; every output element is summed in place
 //            from strided reads (no permutations).
+//   FIBER  : DIRECT's shape class with one operand much larger than the
+//            other: a thread owns a fiber of the large operand and writes
+//            every output it feeds (the large operand crosses HBM once).
 //   GEMM   : transpose-transpose-GEMM-transpose. Operands are used in place
 //            when their modes are already grouped as (rows, contracted) or
 //            (contracted, rows) -- the GEMM kernel takes either orientation --
@@ -111,11 +114,16 @@ struct GemmLaunch {
 };
 
 struct ExecPlan {
-  enum Strategy { DIRECT, GEMM } strategy = DIRECT;
+  enum Strategy { DIRECT, FIBER, GEMM } strategy = DIRECT;
   Structure st;
   // DIRECT
   int64_t d_ngroups = 0, d_tiles = 0;
   DevBuf d_pairs, d_groups, d_koffs, d_wgg;  // d_wgg: workgroup -> group
+  // FIBER (A = the larger operand; f_a_is_2: A is T2)
+  bool f_a_is_2 = false;
+  int f_nf = 0;
+  int64_t f_wgs = 0;
+  DevBuf f_groups, f_outs, f_pairs, f_koffs, f_gofs_c, f_gofs_b, f_wgg;
   // GEMM
   bool swap = false, permA = false, permB = false, permC = false;
   int ta = 0, tb = 0;
@@ -167,6 +175,9 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
                     KernelTimer* kt = nullptr);
 // Tile cfg picked by the built-in heuristic (the autotuner may override).
 int default_gemm_cfg(const ExecPlan& ep, int cplx);
+// Split-precision tile: 2 (128 x 64, three LDS stages, one workgroup per CU)
+// when that shape still gives two workgroups per CU, else 0 (64 x 64).
+int default_split_cfg(const ExecPlan& ep);
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s);
 // XCD-balanced workgroup -> {group, tile} table for tile shape tc.
 // filter: 0 = every group, 1 = only groups with M, N >= the tile, 2 = the rest.

@@ -360,21 +360,51 @@ __device__ inline void load_lds16(const int8_t* g, v4i* lds_base) {
 // set of half 0, bits 4-5 / 6-7: half 1), accumulated into the same INT32
 // levels (both halves share the row and column scales), and written to the
 // interleaved complex output.
-template <int S, int CPLX, int WMB>
-__global__ void __launch_bounds__(kThreads, WMB == 1 ? 2 : 1)
+// s_waitcnt vmcnt(N) (expcnt and lgkmcnt left at their maxima).
+template <int N>
+__device__ inline void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | (((N >> 4) & 3) << 14));
+}
+
+// Workgroup barrier for the stage ring. __syncthreads() carries a release
+// fence, which waits for every outstanding load -- including the LDS fills
+// issued for later stages that are meant to stay in flight. Here the caller
+// has already waited (vmcnt) for the stage the next step reads; this waits
+// for the wave's own LDS reads (so no stage is refilled under a reader),
+// then synchronises. The empty asm statements keep the compiler from moving
+// memory operations across the barrier.
+__device__ inline void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xF | (7 << 4) | (0 << 8) | (3 << 14));  // lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// WM waves along M (the tile is (32*WM) x 64, 2*WM waves, one 32x32 block
+// each). WM = 2: 64 x 64 tiles, two LDS stages, two workgroups per CU.
+// WM = 4: 128 x 64 tiles, three LDS stages (126 KB) so that every stage is
+// fetched two K steps ahead; one workgroup (eight waves) per CU. Either way a
+// SIMD runs two waves.
+template <int S, int CPLX, int WM>
+__global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
     split_gemm_kernel(const SplitPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
                       const int2* __restrict__ tiles, const int8_t* __restrict__ planesA,
                       const int8_t* __restrict__ planesB, const int32_t* __restrict__ expA,
                       const int32_t* __restrict__ expB, double* __restrict__ Cg, double ga,
                       double beta, int comp, int sets, int nsetsA, int nsetsB) {
-  constexpr int BM = 64 * WMB;
-  constexpr int NL = S;               // levels 2..S+1
-  constexpr int A_ST = S * 2 * BM;    // 16-byte LDS slots of one A stage
+  constexpr int BM = 32 * WM;
+  constexpr int NW = 2 * WM;                    // waves
+  constexpr int NSTAGE = WM == 2 ? 2 : 3;
+  constexpr int RB = BM / 64;                   // 64-row blocks of the A tile
+  constexpr int NL = S;                         // levels 0..S-1
+  constexpr int A_ST = S * 2 * BM;              // 16-byte LDS slots of one A stage
   constexpr int B_ST = S * 2 * BN;
-  constexpr int NLD_A = S * 2 * WMB;  // wave loads (64 rows x 16 bytes) per A stage
+  constexpr int NLD_A = S * 2 * RB;             // wave loads (64 rows x 16 bytes) per A stage
   constexpr int NLD = NLD_A + S * 2;
-  constexpr int NLD_W = (NLD + 3) / 4;
-  __shared__ v4i lds[2 * (A_ST + B_ST)];
+  constexpr int NLD_W = (NLD + NW - 1) / NW;    // most loads a wave issues per stage
+  constexpr int kInFlight = NSTAGE == 2 ? 0 : NLD / NW;  // newest-stage loads left in flight
+  __shared__ v4i lds[NSTAGE * (A_ST + B_ST)];
 
   const int2 tt = tiles[blockIdx.x];
   if (tt.x < 0) return;
@@ -384,18 +414,16 @@ __global__ void __launch_bounds__(kThreads, WMB == 1 ? 2 : 1)
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm0 = (wave & 1) * 32 * WMB, wn0 = (wave >> 1) * 32;
+  const int wm0 = (wave % WM) * 32, wn0 = (wave / WM) * 32;
   const int l32 = lane & 31, lh = lane >> 5;
   // B row fetched by this lane; rows past the block are clamped onto its
   // last row (they only feed outputs that are never stored)
   const int brow = min(n0 + lane, grp.N - 1);
 
-  // this lane's outputs: m = m0+wm0+32b+l32 ; n = n0+wn0+(r%4)+8*(r/4)+4*lh
-  double acc[WMB][16];
+  // this lane's outputs: m = m0+wm0+l32 ; n = n0+wn0+(r%4)+8*(r/4)+4*lh
+  double acc[16];
 #pragma unroll
-  for (int b = 0; b < WMB; ++b)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[b][r] = 0.0;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
 
   for (int p = grp.p0; p < grp.p1; ++p) {
     const SplitPair pr = pairs[p];
@@ -403,27 +431,25 @@ __global__ void __launch_bounds__(kThreads, WMB == 1 ? 2 : 1)
     if (kpad <= 0) continue;
     const int8_t* pa = planesA;
     const int8_t* pb = planesB;
-    v16i lev[WMB][NL];
+    v16i lev[NL];
 #pragma unroll
-    for (int b = 0; b < WMB; ++b)
-#pragma unroll
-      for (int L = 0; L < NL; ++L) lev[b][L] = v16i{0};
+    for (int L = 0; L < NL; ++L) lev[L] = v16i{0};
 
-    // the NLD wave loads of the stage at K byte k0, into buffer buf; wave w
-    // issues loads w, w+4, ...
-    auto issue = [&](int k0, int buf) {
-      v4i* sA = lds + buf * (A_ST + B_ST);
+    // the NLD wave loads of the stage at K byte k0, into LDS stage `st`;
+    // wave w issues loads w, w+NW, ...
+    auto issue = [&](int k0, int st) {
+      v4i* sA = lds + st * (A_ST + B_ST);
       v4i* sB = sA + A_ST;
       const int c0 = k0 >> 4;
 #pragma unroll
       for (int t = 0; t < NLD_W; ++t) {
-        const int q = wave + 4 * t;
+        const int q = wave + NW * t;
         if (q >= NLD) break;
         if (q < NLD_A) {
-          const int b = q % WMB, ch = (q / WMB) & 1, pl = q / (2 * WMB);
-          const int row = min(m0 + b * 64 + lane, grp.M - 1);
+          const int rb = q % RB, ch = (q / RB) & 1, pl = q / (2 * RB);
+          const int row = min(m0 + rb * 64 + lane, grp.M - 1);
           load_lds16(pa + pl * pr.a_pstride + ((int64_t)(c0 + ch) * grp.M + row) * 16,
-                     sA + (pl * 2 + ch) * BM + b * 64);
+                     sA + (pl * 2 + ch) * BM + rb * 64);
         } else {
           const int r = q - NLD_A, ch = r & 1, pl = r >> 1;
           load_lds16(pb + pl * pr.b_pstride + ((int64_t)(c0 + ch) * grp.N + brow) * 16,
@@ -444,79 +470,84 @@ __global__ void __launch_bounds__(kThreads, WMB == 1 ? 2 : 1)
       pa = planesA + pr.a_planes;
       pb = planesB + pr.b_planes;
     }
-    // every wave is past the previous half's last barrier: buffer 0 is free
+    // Stage ring: step s computes from stage s % NSTAGE and fetches step
+    // s + NSTAGE - 1 into the stage step s - 1 used (every wave finished
+    // reading it before the barrier that closed step s - 1). Every wave is
+    // past the previous half's / pair's last barrier, so all stages are free.
+    const int nsteps = kpad / BKB;
     issue(0, 0);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    int buf = 0;
-    for (int k0 = 0; k0 < kpad; k0 += BKB) {
-      const v4i* sA = lds + buf * (A_ST + B_ST);
+    if (NSTAGE == 3 && nsteps > 1) {
+      issue(BKB, 1);
+      wait_vmcnt<kInFlight>();
+    } else {
+      wait_vmcnt<0>();
+    }
+    stage_barrier();
+    int st = 0;
+    for (int s = 0; s < nsteps; ++s) {
+      const v4i* sA = lds + st * (A_ST + B_ST);
       const v4i* sB = sA + A_ST;
+      // the fill of a later stage first, then the fragments in the order the
+      // MFMAs consume them (A slice 0, the B slices, A slices 1..): the first
+      // MFMA waits for two LDS reads, not fourteen, so a wave's reads overlap
+      // the other waves' MFMAs
+      const int nxt = s + NSTAGE - 1;
+      int nst = st + NSTAGE - 1;
+      if (nst >= NSTAGE) nst -= NSTAGE;
+      if (nxt < nsteps) issue(nxt * BKB, nst);
       v4i bv[S], av[S];
+      av[0] = sA[lh * BM + wm0 + l32];
 #pragma unroll
       for (int j = 0; j < S; ++j) bv[j] = sB[(j * 2 + lh) * BN + wn0 + l32];
 #pragma unroll
-      for (int i = 0; i < S; ++i) av[i] = sA[(i * 2 + lh) * BM + wm0 + l32];
-      // next stage into the other buffer (its last readers finished before
-      // the barrier that closed the previous step); it lands while the MFMAs
-      // below run
-      if (k0 + BKB < kpad) issue(k0 + BKB, buf ^ 1);
+      for (int i = 1; i < S; ++i) av[i] = sA[(i * 2 + lh) * BM + wm0 + l32];
 #pragma unroll
-      for (int b = 0; b < WMB; ++b) {
-        if (b > 0) {  // A fragments of the next 32-row block (one set live at a time)
+      for (int i = 0; i < S; ++i)
 #pragma unroll
-          for (int i = 0; i < S; ++i) av[i] = sA[(i * 2 + lh) * BM + wm0 + b * 32 + l32];
-        }
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-#pragma unroll
-          for (int j = 0; j < S - i; ++j)  // slice i of A with slice j of B, level i+j
-            lev[b][i + j] =
-                __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av[i], lev[b][i + j], 0, 0, 0);
-      }
+        for (int j = 0; j < S - i; ++j)  // slice i of A with slice j of B, level i+j
+          lev[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av[i], lev[i + j], 0, 0, 0);
       // The MFMAs only touch registers, so without this fence the scheduler
       // sinks them below the wait for the next stage, which then serialises
       // the fetch with the math.
       __builtin_amdgcn_sched_barrier(0);
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      __syncthreads();
-      buf ^= 1;
+      // stage s + 1 must have landed; the loads issued just now (stage
+      // s + NSTAGE - 1) may stay in flight
+      if (NSTAGE == 3 && nxt < nsteps)
+        wait_vmcnt<kInFlight>();
+      else
+        wait_vmcnt<0>();
+      stage_barrier();
+      if (++st == NSTAGE) st = 0;
     }
     }  // halves
 
     // FP64 recombination of the levels, scaled per row/column
-#pragma unroll
-    for (int b = 0; b < WMB; ++b) {
-      const int gm = m0 + wm0 + b * 32 + l32;
-      const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
-        const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
-        double h = (double)lev[b][NL - 1][r];
-#pragma unroll
-        for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[b][L][r]);
-        double v = ldexp(h, ea + eb - 12);
-        if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
-        acc[b][r] = fma(ga * pr.alpha, v, acc[b][r]);
-      }
-    }
-  }
-
-#pragma unroll
-  for (int b = 0; b < WMB; ++b) {
-    const int gm = m0 + wm0 + b * 32 + l32;
-    if (gm >= grp.M) continue;
+    const int gm = m0 + wm0 + l32;
+    const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
-      if (gn >= grp.N) continue;
-      const int64_t o0 = grp.off_c + gm + (int64_t)gn * grp.ldc;
-      const int64_t o = CPLX ? 2 * o0 + comp : o0;
-      double out = acc[b][r];
-      if (beta != 0.0) out = fma(beta, Cg[o], out);
-      Cg[o] = out;
+      const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
+      double h = (double)lev[NL - 1][r];
+#pragma unroll
+      for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[L][r]);
+      double v = ldexp(h, ea + eb - 12);
+      if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
+      acc[r] = fma(ga * pr.alpha, v, acc[r]);
     }
+  }
+
+  const int gm = m0 + wm0 + l32;
+  if (gm >= grp.M) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
+    if (gn >= grp.N) continue;
+    const int64_t o0 = grp.off_c + gm + (int64_t)gn * grp.ldc;
+    const int64_t o = CPLX ? 2 * o0 + comp : o0;
+    double out = acc[r];
+    if (beta != 0.0) out = fma(beta, Cg[o], out);
+    Cg[o] = out;
   }
 }
 
@@ -555,14 +586,14 @@ int64_t slice_block_wgs(int32_t rows, int32_t kparts) {
 }
 
 namespace {
-template <int CPLX, int WMB>
+template <int CPLX, int WM>
 hipError_t launch_sg(hipStream_t s, int comp, int sets, const SplitPair* pairs,
                      const GemmGroup* groups, const int2* tiles, int64_t grid,
                      const void* planesA, const void* planesB, const void* expA,
                      const void* expB, void* C, double ga, double beta) {
   const int ns = CPLX ? 3 : 1;
-  hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, CPLX, WMB>), dim3((unsigned)grid),
-                     dim3(kThreads), 0, s, pairs, groups, tiles, (const int8_t*)planesA,
+  hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, CPLX, WM>), dim3((unsigned)grid),
+                     dim3(128 * WM), 0, s, pairs, groups, tiles, (const int8_t*)planesA,
                      (const int8_t*)planesB, (const int32_t*)expA, (const int32_t*)expB,
                      (double*)C, ga, beta, CPLX ? comp : 0, CPLX ? sets : 0, ns, ns);
   return hipGetLastError();
@@ -576,14 +607,16 @@ hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int com
                              double beta) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
-  // 128-row tiles (WMB = 2) need more than the 512 registers of a wave with
-  // seven levels per 32x32 block; only the 64 x 64 tile is built.
-  if (wmb != 1) return hipErrorInvalidValue;
+  if (wmb != 1 && wmb != 2) return hipErrorInvalidValue;
   if (cplx)
-    return launch_sg<1, 1>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB, expA,
-                           expB, C, ga, beta);
-  return launch_sg<0, 1>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB, expA, expB,
-                         C, ga, beta);
+    return wmb == 2 ? launch_sg<1, 4>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
+                                      expA, expB, C, ga, beta)
+                    : launch_sg<1, 2>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
+                                      expA, expB, C, ga, beta);
+  return wmb == 2 ? launch_sg<0, 4>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
+                                    expA, expB, C, ga, beta)
+                  : launch_sg<0, 2>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
+                                    expA, expB, C, ga, beta);
 }
 
 }  // namespace itg

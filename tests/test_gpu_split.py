@@ -142,6 +142,27 @@ def test_split_complex_heff_network(env):
         assert err < SPLIT_RTOL, (qns, err)
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_split_large_dense_tiles(env, cplx):
+    """A 2048 x 2048 product has enough 128 x 64 tiles for the three-stage
+    128-row split kernel; K = 300 gives a partial last K step. FP64 numpy
+    products are the reference (their own error is ~1e-15)."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(16)
+    M, N, K = 2048, 2048, 300
+    A = rand(rng, (M, K), cplx)
+    B = rand(rng, (K, N), cplx)
+    got = run(env, A, (1, -1), B, (-1, 2), (1, 2))
+    err = relerr(got, A @ B)
+    assert err < SPLIT_RTOL, err
+    if cplx:
+        TA, TB = ND.DenseTensor.from_numpy(A), ND.DenseTensor.from_numpy(B)
+        T = ND.contract(TA, (1, -1), TB, (-1, 2), (1, 2), prec=PREC, conj1=True, conj2=True)
+        torch.cuda.synchronize()
+        err = relerr(T.to_numpy(), np.conj(A) @ np.conj(B))
+        assert err < SPLIT_RTOL, err
+
+
 def test_split_nonfinite_propagates(env):
     rng = np.random.default_rng(14)
     A = rand(rng, (20, 64))
