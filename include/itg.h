@@ -85,6 +85,14 @@ itg_status itg_synchronize(itg_handle* h);
 /* Library version, "major.minor.patch". */
 const char* itg_version(void);
 
+/* Tile shape of the split-precision GEMM for later contractions on this
+ * handle: 0 = per plan (the autotuned shape, else chosen by size),
+ * 1 = 64 x 64, 2 = 128 x 64, 3 = 64 x 64 with the A operand held in
+ * registers (real data; complex plans use 1). All shapes give the same
+ * result bit for bit. No counterpart in NDTensors, whose BLAS call has no
+ * tile shape; for tests and tuning. ITG_ERR_INVALID outside 0..3. */
+itg_status itg_set_split_tile(itg_handle* h, int tile);
+
 /* Per-kernel timing: the device counterpart of NDTensors' global
  * `timer::TimerOutput` (NDTensors/src/NDTensors.jl:96-99) and its
  * `@timeit_debug` sections around BLAS.gemm! and the permutations

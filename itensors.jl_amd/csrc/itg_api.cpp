@@ -33,6 +33,7 @@ struct itg_handle {
   std::unordered_map<std::string, decltype(dense_lru)::iterator> dense_map;
   // autotuned tile cfg per (bucket, cplx, prec)
   std::map<std::string, int> tuned;
+  int split_tile = 0;  // itg_set_split_tile (0 = per plan)
 };
 
 struct itg_bs_plan {
@@ -117,6 +118,7 @@ KernelTimer* active_timer(itg_handle* h) {
 }
 
 int pick_cfg(itg_handle* h, const ExecPlan& ep, int cplx, int prec) {
+  if (prec != 0 && h->split_tile > 0) return h->split_tile == 1 ? 0 : h->split_tile;
   auto it = h->tuned.find(tune_key(ep, cplx, prec));
   if (it != h->tuned.end()) return it->second;
   return prec != 0 ? default_split_cfg(ep) : default_gemm_cfg(ep, cplx);
@@ -184,6 +186,12 @@ itg_status itg_synchronize(itg_handle* h) {
   DeviceGuard g(h->device);
   hipError_t e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
+  return ITG_OK;
+}
+
+itg_status itg_set_split_tile(itg_handle* h, int tile) {
+  if (!h || tile < 0 || tile > 3) return ITG_ERR_INVALID;
+  h->split_tile = tile;
   return ITG_OK;
 }
 
@@ -696,6 +704,7 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   std::vector<int> cands;
   if (prec != ITG_PREC_FP64) {
     cands = {0, 2};  // split tiles 64 x 64 and 128 x 64
+    if (!cplx) cands.push_back(3);  // 64 x 64 with A in registers
   } else {
     for (int c = 0; c < gemm_num_cfgs(); ++c)
       if (gemm_cfg_ok(c, cplx)) cands.push_back(c);

@@ -3,7 +3,10 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
+#include <tuple>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -62,6 +65,43 @@ struct GemmGroup {
   int32_t pad;
 };
 
+// ------------------------------------------------------------ kernel launch
+// Events that the kernels launched by this thread record their own start
+// and end into (set by the per-kernel timer around one timed section, null
+// otherwise). The first launch of a section records `start` and clears it;
+// every launch records `stop`, so the section spans from the first kernel's
+// start to the last kernel's end, as the dispatches' own timestamps (no
+// gaps between separately enqueued event packets).
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool launched = false;
+};
+LaunchEvents& launch_events();
+
+template <typename Tup, size_t... I>
+inline void kernel_arg_ptrs(Tup& t, void** p, std::index_sequence<I...>) {
+  ((p[I] = (void*)&std::get<I>(t)), ...);
+}
+
+// Every kernel of the library is launched through here.
+template <typename... KArgs, typename... Args>
+inline hipError_t launch_kernel(void (*kernel)(KArgs...), dim3 grid, dim3 block, unsigned shmem,
+                                hipStream_t s, Args... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  std::tuple<KArgs...> vals(static_cast<KArgs>(args)...);
+  void* ptrs[sizeof...(KArgs)];
+  kernel_arg_ptrs(vals, ptrs, std::index_sequence_for<KArgs...>{});
+  LaunchEvents& ev = launch_events();
+  if (!ev.stop) return hipLaunchKernel((const void*)kernel, grid, block, ptrs, shmem, s);
+  hipError_t e =
+      hipExtLaunchKernel((const void*)kernel, grid, block, ptrs, shmem, s, ev.start, ev.stop, 0);
+  ev.start = nullptr;
+  ev.launched = true;
+  return e;
+}
+#define ITG_LAUNCH(kernel, grid, block, shmem, stream, ...) \
+  (void)::itg::launch_kernel(kernel, dim3(grid), dim3(block), shmem, stream, __VA_ARGS__)
+
 // --------------------------------------------------------- kernel launchers
 // All launchers enqueue on `stream` and return hipSuccess or the launch error.
 // `esize` is 1 for Float64 and 2 for ComplexF64.
@@ -115,6 +155,46 @@ struct FiberPair {
   int32_t gb0;               // gofs_b[gb0 + g]: B offset of B-free element g
   int32_t pad;
 };
+// Matrix form of a group (when it has at most kFiberSlots distinct A values
+// per fiber and kFiberOuts outputs per fiber): slot s = one (A block,
+// contracted index) of the group, output o = one (output block, B-free
+// index); out_o = sum_s a_s W[s][o] with W[s][o] = alpha_p B_p[k(s), g(o)]
+// for the pair p linking them (0 if none). Each workgroup builds its
+// group's W in LDS, loads all slots of its fibers at once (A read exactly
+// once) and writes every output.
+constexpr int kFiberSlots = 16, kFiberOuts = 32;
+constexpr int kFibersPerWGMV = 256 * 8;  // 8 rounds of one fiber per thread
+struct FiberMVGroup {
+  int64_t nfib, wg0;
+  int32_t fdim[kMaxModes];
+  int32_t nslot, nout;
+  int32_t s0, o0;            // FiberSlot / FiberOutElem ranges
+  int32_t w0, nw;            // FiberCoef range
+};
+struct FiberSlot {           // A offset = off_a + sum_i c_i * astr[i] + ka
+  int64_t off_a;
+  int32_t astr[kMaxModes];
+  int32_t ka, pad;
+};
+struct FiberOutElem {        // C offset = off_c + sum_i c_i * cstr[i] + go
+  int64_t off_c;
+  int32_t cstr[kMaxModes];
+  int32_t go;
+  uint32_t mask;             // bit s: slot s feeds this output (others are not read,
+                             // so a non-finite A value reaches only its own outputs)
+};
+struct FiberCoef {           // W[slot][out] = alpha * B[off_b]
+  int64_t off_b;
+  double alpha_re, alpha_im;
+  int32_t slot, out;
+};
+// nf: number of A-free modes (1..kMaxModes). flags bit 0 / 1: conj(A) / conj(B).
+// smax: slot bound the kernel is unrolled for (4, 8 or 16; >= every group's nslot).
+hipError_t launch_fiber_mv(hipStream_t s, int cplx, int nf, int smax, const FiberMVGroup* groups,
+                           const FiberSlot* slots, const FiberOutElem* outs,
+                           const FiberCoef* coefs, const int32_t* wg_group, int64_t total_wgs,
+                           const void* A, const void* B, void* C, unsigned flags, double ga_re,
+                           double ga_im, double beta_re, double beta_im);
 // nf: number of A-free modes (1..kMaxModes). flags bit 0 / 1: conj(A) / conj(B).
 hipError_t launch_fiber(hipStream_t s, int cplx, int nf, const FiberGroup* groups,
                         const FiberOut* outs, const FiberPair* pairs, const int32_t* koffs,

@@ -43,7 +43,7 @@ struct KernelTimer {
   std::map<std::string, std::pair<int64_t, double>> totals;  // calls, ms
   hipStream_t stream = nullptr;
   std::string open_name;
-  hipEvent_t open_a = nullptr;
+  hipEvent_t open_a = nullptr, open_b = nullptr;
 
   hipEvent_t take();
   void begin(const char* name, hipStream_t s);
@@ -121,9 +121,12 @@ struct ExecPlan {
   DevBuf d_pairs, d_groups, d_koffs, d_wgg;  // d_wgg: workgroup -> group
   // FIBER (A = the larger operand; f_a_is_2: A is T2)
   bool f_a_is_2 = false;
+  bool f_mv = false;  // matrix form (fm_* tables) instead of the loop form
   int f_nf = 0;
+  int f_smax = 0;     // matrix form: slot bound of the kernel (4, 8 or 16) >= every group's nslot
   int64_t f_wgs = 0;
   DevBuf f_groups, f_outs, f_pairs, f_koffs, f_gofs_c, f_gofs_b, f_wgg;
+  DevBuf fm_groups, fm_slots, fm_outs, fm_coefs;
   // GEMM
   bool swap = false, permA = false, permB = false, permC = false;
   int ta = 0, tb = 0;

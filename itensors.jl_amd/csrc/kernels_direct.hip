@@ -154,7 +154,7 @@ template <int CPLX, int NR>
 hipError_t launch_nr(hipStream_t s, dim3 grid, const DirectPair* pairs, const DirectGroup* groups,
                      const int32_t* koffs, const int32_t* wg_group, const void* A, const void* B,
                      void* C, unsigned flags, double ar, double ai, double br, double bi) {
-  hipLaunchKernelGGL((direct_kernel<CPLX, NR>), grid, dim3(kThreads), 0, s, pairs, groups, koffs,
+  ITG_LAUNCH((direct_kernel<CPLX, NR>), grid, dim3(kThreads), 0, s, pairs, groups, koffs,
                      wg_group, (const double*)A, (const double*)B, (double*)C, flags, ar, ai, br,
                      bi);
   return hipGetLastError();

@@ -147,6 +147,223 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Matrix form: out_o = sum_s a_s W[s][o] per fiber, W of the group in LDS.
+// A workgroup builds W once and then streams kMVRounds rounds of one fiber
+// per thread; the slots of the next round are loaded before the outputs of
+// the current one are formed, so every wave keeps its A reads in flight
+// behind its arithmetic and stores. Each A value is read once.
+constexpr int kMVRounds = kFibersPerWGMV / kThreads;
+static_assert(kMVRounds * kThreads == kFibersPerWGMV, "whole rounds per workgroup");
+
+// SM: slot bound (4, 8 or 16) the slot loops are unrolled for; every group
+// of the launch has nslot <= SM.
+template <int CPLX, int NF, int SM>
+__global__ void __launch_bounds__(kThreads)
+    fiber_mv_kernel(const FiberMVGroup* __restrict__ groups, const FiberSlot* __restrict__ slots,
+                    const FiberOutElem* __restrict__ outs, const FiberCoef* __restrict__ coefs,
+                    const int32_t* __restrict__ wg_group, const double* __restrict__ A,
+                    const double* __restrict__ B, double* __restrict__ C, unsigned flags,
+                    double ga_re, double ga_im, double beta_re, double beta_im) {
+  __shared__ double Wre[kFiberSlots * kFiberOuts];
+  __shared__ double Wim[CPLX ? kFiberSlots * kFiberOuts : 1];
+  // the group's slot and output descriptors, staged once per workgroup
+  __shared__ int64_t s_off[kFiberSlots], o_off[kFiberOuts];
+  __shared__ int32_t s_str[kFiberSlots][NF], o_str[kFiberOuts][NF];
+  __shared__ uint32_t o_mask[kFiberOuts];
+  const int64_t wg = blockIdx.x;
+  const FiberMVGroup& fg = groups[wg_group[wg]];
+  const int tid = threadIdx.x;
+  const bool conja = (flags & 1u) != 0, conjb = (flags & 2u) != 0;
+  const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
+  const int nslot = fg.nslot, nout = fg.nout;
+
+  // W = ga * alpha_p * B_p entries of the group, zero elsewhere
+  for (int i = tid; i < kFiberSlots * kFiberOuts; i += kThreads) {
+    Wre[i] = 0.0;
+    if (CPLX) Wim[i] = 0.0;
+  }
+  if (tid < kFiberSlots) {
+    const FiberSlot& sl = slots[fg.s0 + (tid < nslot ? tid : 0)];
+    s_off[tid] = sl.off_a + sl.ka;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) s_str[tid][i] = sl.astr[i];
+  } else if (tid >= 64 && tid < 64 + kFiberOuts) {
+    const int o = tid - 64;
+    const FiberOutElem& oe = outs[fg.o0 + (o < nout ? o : 0)];
+    o_off[o] = oe.off_c + oe.go;
+    o_mask[o] = o < nout ? oe.mask : 0u;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) o_str[o][i] = oe.cstr[i];
+  }
+  __syncthreads();
+  for (int i = tid; i < fg.nw; i += kThreads) {
+    const FiberCoef cf = coefs[fg.w0 + i];
+    const double al_re = ga_re * cf.alpha_re - ga_im * cf.alpha_im;
+    const double al_im = ga_re * cf.alpha_im + ga_im * cf.alpha_re;
+    const int w = cf.out * kFiberSlots + cf.slot;  // output-major: one output's weights adjacent
+    if (CPLX) {
+      const double2 b = reinterpret_cast<const double2*>(B)[cf.off_b];
+      const double bi = conjb ? -b.y : b.y;
+      Wre[w] = al_re * b.x - al_im * bi;
+      Wim[w] = al_re * bi + al_im * b.x;
+    } else {
+      Wre[w] = al_re * B[cf.off_b];
+    }
+  }
+  __syncthreads();
+
+  const int64_t fbase = (wg - fg.wg0) * kFibersPerWGMV;
+  const int64_t flast = fg.nfib - 1;
+  // rounds holding at least one fiber of the group (uniform)
+  const int64_t left = fg.nfib - fbase;
+  const int nrounds = (int)(left >= kFibersPerWGMV ? kMVRounds : (left + kThreads - 1) / kThreads);
+
+  // coordinates of fiber f (clamped to the group's last fiber: a dead lane
+  // reads valid memory and stores nothing)
+  auto coords = [&](int64_t f, uint32_t (&c)[NF]) {
+    uint32_t rem = (uint32_t)(f < flast ? f : flast);
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      if (i + 1 < NF) {
+        const uint32_t q = rem / (uint32_t)fg.fdim[i];
+        c[i] = rem - q * (uint32_t)fg.fdim[i];
+        rem = q;
+      } else {
+        c[i] = rem;
+      }
+    }
+  };
+  // every slot of one fiber up to SM (slots past nslot repeat slot 0 and
+  // are masked out below); fully unrolled so the values stay in registers and the
+  // loads issue back to back
+  auto load = [&](const uint32_t (&c)[NF], double (&xr)[SM], double (&xi)[CPLX ? SM : 1]) {
+#pragma unroll
+    for (int sl = 0; sl < SM; ++sl) {
+      int64_t ao = s_off[sl];
+#pragma unroll
+      for (int i = 0; i < NF; ++i) ao += (int64_t)c[i] * s_str[sl][i];
+      if (CPLX) {
+        const double2 x = reinterpret_cast<const double2*>(A)[ao];
+        xr[sl] = x.x;
+        xi[CPLX ? sl : 0] = conja ? -x.y : x.y;
+      } else {
+        xr[sl] = A[ao];
+      }
+    }
+  };
+
+  uint32_t c[NF];
+  double are[SM], aim[CPLX ? SM : 1];
+  coords(fbase + tid, c);
+  load(c, are, aim);
+  for (int r = 0; r < nrounds; ++r) {
+    // re-read the staged descriptors from LDS each round (broadcast reads)
+    // instead of holding all of them in registers across the rounds
+    asm volatile("" ::: "memory");
+    const int64_t f = fbase + (int64_t)r * kThreads + tid;
+    // next round's slots first (on the last round a clamped re-read of
+    // the group's last fiber, which nothing uses)
+    uint32_t cn[NF];
+    double anr[SM], ani[CPLX ? SM : 1];
+    coords(f + kThreads, cn);
+    load(cn, anr, ani);
+    const bool live = f < fg.nfib;
+    for (int o = 0; o < nout; ++o) {
+      const uint32_t m = o_mask[o];
+      double acc_re = 0.0, acc_im = 0.0;
+      // branch-free over all slots: an unlinked slot's product is formed
+      // and discarded by a select (never added), so its value cannot leak
+#pragma unroll
+      for (int sl = 0; sl < SM; ++sl) {
+        const bool lk = (m >> sl) & 1u;
+        const double wr = Wre[o * kFiberSlots + sl];
+        if (CPLX) {
+          const double wi = Wim[o * kFiberSlots + sl];
+          const double tr = fma(are[sl], wr, fma(-aim[CPLX ? sl : 0], wi, acc_re));
+          const double ti = fma(are[sl], wi, fma(aim[CPLX ? sl : 0], wr, acc_im));
+          acc_re = lk ? tr : acc_re;
+          acc_im = lk ? ti : acc_im;
+        } else {
+          const double t = fma(are[sl], wr, acc_re);
+          acc_re = lk ? t : acc_re;
+        }
+      }
+      if (live) {
+        int64_t co = o_off[o];
+#pragma unroll
+        for (int i = 0; i < NF; ++i) co += (int64_t)c[i] * o_str[o][i];
+        if (CPLX) {
+          double2 out = make_double2(acc_re, acc_im);
+          if (use_beta) {
+            const double2 cv = reinterpret_cast<double2*>(C)[co];
+            out.x += beta_re * cv.x - beta_im * cv.y;
+            out.y += beta_re * cv.y + beta_im * cv.x;
+          }
+          reinterpret_cast<double2*>(C)[co] = out;
+        } else {
+          double out = acc_re;
+          if (use_beta) out = fma(beta_re, C[co], out);
+          C[co] = out;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) c[i] = cn[i];
+#pragma unroll
+    for (int sl = 0; sl < SM; ++sl) {
+      are[sl] = anr[sl];
+      if (CPLX) aim[CPLX ? sl : 0] = ani[CPLX ? sl : 0];
+    }
+  }
+}
+
+template <int CPLX, int SM>
+hipError_t launch_mv(hipStream_t s, int nf, dim3 grid, const FiberMVGroup* groups,
+                     const FiberSlot* slots, const FiberOutElem* outs, const FiberCoef* coefs,
+                     const int32_t* wg_group, const void* A, const void* B, void* C,
+                     unsigned flags, double ar, double ai, double br, double bi) {
+#define ITG_NF_CASE(n)                                                                        \
+  case n:                                                                                     \
+    ITG_LAUNCH((fiber_mv_kernel<CPLX, n, SM>), grid, dim3(kThreads), 0, s, groups, slots,    \
+               outs, coefs, wg_group, (const double*)A, (const double*)B, (double*)C, flags, \
+               ar, ai, br, bi);                                                               \
+    return hipGetLastError();
+  switch (nf) {
+    ITG_NF_CASE(1)
+    ITG_NF_CASE(2)
+    ITG_NF_CASE(3)
+    ITG_NF_CASE(4)
+    ITG_NF_CASE(5)
+    ITG_NF_CASE(6)
+    ITG_NF_CASE(7)
+    ITG_NF_CASE(8)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef ITG_NF_CASE
+}
+
+template <int CPLX>
+hipError_t launch_mv_sm(hipStream_t s, int nf, int smax, dim3 grid, const FiberMVGroup* groups,
+                        const FiberSlot* slots, const FiberOutElem* outs,
+                        const FiberCoef* coefs, const int32_t* wg_group, const void* A,
+                        const void* B, void* C, unsigned flags, double ar, double ai, double br,
+                        double bi) {
+  switch (smax) {
+    case 4:
+      return launch_mv<CPLX, 4>(s, nf, grid, groups, slots, outs, coefs, wg_group, A, B, C,
+                                flags, ar, ai, br, bi);
+    case 8:
+      return launch_mv<CPLX, 8>(s, nf, grid, groups, slots, outs, coefs, wg_group, A, B, C,
+                                flags, ar, ai, br, bi);
+    case kFiberSlots:
+      return launch_mv<CPLX, kFiberSlots>(s, nf, grid, groups, slots, outs, coefs, wg_group, A,
+                                          B, C, flags, ar, ai, br, bi);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
 template <int CPLX>
 hipError_t launch_c(hipStream_t s, int nf, dim3 grid, const FiberGroup* groups,
                     const FiberOut* outs, const FiberPair* pairs, const int32_t* koffs,
@@ -155,7 +372,7 @@ hipError_t launch_c(hipStream_t s, int nf, dim3 grid, const FiberGroup* groups,
                     double br, double bi) {
 #define ITG_NF_CASE(n)                                                                       \
   case n:                                                                                    \
-    hipLaunchKernelGGL((fiber_kernel<CPLX, n>), grid, dim3(kThreads), 0, s, groups, outs,    \
+    ITG_LAUNCH((fiber_kernel<CPLX, n>), grid, dim3(kThreads), 0, s, groups, outs,    \
                        pairs, koffs, gofs_c, gofs_b, wg_group, (const double*)A,             \
                        (const double*)B, (double*)C, flags, ar, ai, br, bi);                 \
     return hipGetLastError();
@@ -175,6 +392,21 @@ hipError_t launch_c(hipStream_t s, int nf, dim3 grid, const FiberGroup* groups,
 }
 
 }  // namespace
+
+hipError_t launch_fiber_mv(hipStream_t s, int cplx, int nf, int smax,
+                           const FiberMVGroup* groups, const FiberSlot* slots,
+                           const FiberOutElem* outs, const FiberCoef* coefs,
+                           const int32_t* wg_group, int64_t total_wgs, const void* A,
+                           const void* B, void* C, unsigned flags, double ga_re, double ga_im,
+                           double beta_re, double beta_im) {
+  if (total_wgs == 0) return hipSuccess;
+  dim3 grid((unsigned)total_wgs);
+  if (cplx)
+    return launch_mv_sm<1>(s, nf, smax, grid, groups, slots, outs, coefs, wg_group, A, B, C,
+                           flags, ga_re, ga_im, beta_re, beta_im);
+  return launch_mv_sm<0>(s, nf, smax, grid, groups, slots, outs, coefs, wg_group, A, B, C, flags,
+                         ga_re, ga_im, beta_re, beta_im);
+}
 
 hipError_t launch_fiber(hipStream_t s, int cplx, int nf, const FiberGroup* groups,
                         const FiberOut* outs, const FiberPair* pairs, const int32_t* koffs,

@@ -252,16 +252,16 @@ hipError_t launch_tt(hipStream_t s, int ta, int tb, dim3 grid, const GemmPair* p
                      double bi) {
   dim3 block(kThreads);
   if (!ta && !tb)
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 0, 0>), grid, block, 0, s, pairs, groups,
+    ITG_LAUNCH((gemm_f64_kernel<BM, BN, CPLX, 0, 0>), grid, block, 0, s, pairs, groups,
                        tiles, A, B, C, flags, ar, ai, br, bi);
   else if (!ta && tb)
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 0, 1>), grid, block, 0, s, pairs, groups,
+    ITG_LAUNCH((gemm_f64_kernel<BM, BN, CPLX, 0, 1>), grid, block, 0, s, pairs, groups,
                        tiles, A, B, C, flags, ar, ai, br, bi);
   else if (ta && !tb)
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 1, 0>), grid, block, 0, s, pairs, groups,
+    ITG_LAUNCH((gemm_f64_kernel<BM, BN, CPLX, 1, 0>), grid, block, 0, s, pairs, groups,
                        tiles, A, B, C, flags, ar, ai, br, bi);
   else
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, CPLX, 1, 1>), grid, block, 0, s, pairs, groups,
+    ITG_LAUNCH((gemm_f64_kernel<BM, BN, CPLX, 1, 1>), grid, block, 0, s, pairs, groups,
                        tiles, A, B, C, flags, ar, ai, br, bi);
   return hipGetLastError();
 }

@@ -203,9 +203,9 @@ hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re
   if (n <= 0) return hipSuccess;
   dim3 grid((unsigned)((n + kThreads - 1) / kThreads)), block(kThreads);
   if (cplx)
-    hipLaunchKernelGGL(scale_kernel<1>, grid, block, 0, s, (double2*)x, n, b_re, b_im);
+    ITG_LAUNCH(scale_kernel<1>, grid, block, 0, s, (double2*)x, n, b_re, b_im);
   else
-    hipLaunchKernelGGL(scale_kernel<0>, grid, block, 0, s, (double*)x, n, b_re, b_im);
+    ITG_LAUNCH(scale_kernel<0>, grid, block, 0, s, (double*)x, n, b_re, b_im);
   return hipGetLastError();
 }
 
@@ -231,20 +231,20 @@ hipError_t launch_permute(hipStream_t s, int cplx, int nm, const int32_t* perm_h
   dim3 grid((unsigned)total_tiles), block(kThreads);
   if (mode_kind == 0) {
     if (cplx)
-      hipLaunchKernelGGL(permute_inner_kernel<1>, grid, block, 0, s, pa, blocks_dev,
+      ITG_LAUNCH(permute_inner_kernel<1>, grid, block, 0, s, pa, blocks_dev,
                          tile_prefix_dev, nblk, (const double2*)src, (double2*)dst, a_re, a_im,
                          b_re, b_im);
     else
-      hipLaunchKernelGGL(permute_inner_kernel<0>, grid, block, 0, s, pa, blocks_dev,
+      ITG_LAUNCH(permute_inner_kernel<0>, grid, block, 0, s, pa, blocks_dev,
                          tile_prefix_dev, nblk, (const double*)src, (double*)dst, a_re, a_im,
                          b_re, b_im);
   } else {
     if (cplx)
-      hipLaunchKernelGGL(permute_tile_kernel<1>, grid, block, 0, s, pa, blocks_dev,
+      ITG_LAUNCH(permute_tile_kernel<1>, grid, block, 0, s, pa, blocks_dev,
                          tile_prefix_dev, nblk, (const double2*)src, (double2*)dst, a_re, a_im,
                          b_re, b_im);
     else
-      hipLaunchKernelGGL(permute_tile_kernel<0>, grid, block, 0, s, pa, blocks_dev,
+      ITG_LAUNCH(permute_tile_kernel<0>, grid, block, 0, s, pa, blocks_dev,
                          tile_prefix_dev, nblk, (const double*)src, (double*)dst, a_re, a_im,
                          b_re, b_im);
   }

@@ -381,6 +381,24 @@ __device__ inline void stage_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// s_waitcnt lgkmcnt(0) alone.
+__device__ inline void wait_lgkmcnt0() {
+  __builtin_amdgcn_s_waitcnt(0xF | (7 << 4) | (0 << 8) | (3 << 14));
+}
+
+// 16-byte LDS read that the compiler does not track. The compiler's wait
+// insertion makes every LDS read it knows of wait for every outstanding
+// global->LDS fill, including fills of stages nobody reads yet, which
+// flattens a multi-stage ring. The caller waits itself: the stage was
+// completed by an explicit vmcnt wait and barrier, and the destination
+// registers are used only after an explicit wait_lgkmcnt0() that the caller
+// pins with a sched_barrier.
+__device__ inline v4i lds_read16_untracked(uint32_t addr) {
+  v4i v;
+  asm volatile("ds_read_b128 %0, %1" : "=v"(v) : "v"(addr) : "memory");
+  return v;
+}
+
 // WM waves along M (the tile is (32*WM) x 64, 2*WM waves, one 32x32 block
 // each). WM = 2: 64 x 64 tiles, two LDS stages, two workgroups per CU.
 // WM = 4: 128 x 64 tiles, three LDS stages (126 KB) so that every stage is
@@ -551,6 +569,175 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
   }
 }
 
+// Real split GEMM with the A operand in registers: 64 x 64 tiles, two waves
+// per workgroup, each wave 32 rows x 64 columns (both 32-column B blocks
+// share its A fragments). A fragments come straight from the chunk-major
+// planes into registers (a wave owns its rows, nothing to share), prefetched
+// one K step ahead; B goes through LDS (three stages, filled two steps
+// ahead). Per MFMA this moves half the LDS bytes of the 32x32-per-wave
+// kernels, which the LDS-activity counters show are LDS-bound. 7 levels x 2
+// blocks of INT32 accumulators need one wave per SIMD (two workgroups per
+// CU).
+template <int S>
+__global__ void __launch_bounds__(128, 1)
+    split_gemm_areg_kernel(const SplitPair* __restrict__ pairs,
+                           const GemmGroup* __restrict__ groups, const int2* __restrict__ tiles,
+                           const int8_t* planesA, const int8_t* __restrict__ planesB,
+                           const int32_t* __restrict__ expA, const int32_t* __restrict__ expB,
+                           double* __restrict__ Cg, double ga, double beta) {
+  constexpr int BM = 64;
+  constexpr int NL = S;
+  constexpr int B_ST = S * 2 * BN;  // 16-byte LDS slots of one B stage
+  constexpr int NLD = S * 2;        // wave loads (64 rows x 16 bytes) per B stage
+  constexpr int NLD_W = NLD / 2;    // per wave (two waves)
+  static_assert(NLD % 2 == 0, "B stage loads split evenly over the two waves");
+  constexpr int NSTAGE = 3;
+  __shared__ v4i lds[NSTAGE * B_ST];
+
+  const int2 tt = tiles[blockIdx.x];
+  if (tt.x < 0) return;
+  const GemmGroup grp = groups[tt.x];
+  const int tiles_m = (grp.M + BM - 1) / BM;
+  const int tm = tt.y % tiles_m, tn = tt.y / tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int l32 = lane & 31, lh = lane >> 5;
+  const int arow = min(m0 + wave * 32 + l32, grp.M - 1);  // A row of this lane's fragment
+  const int brow = min(n0 + lane, grp.N - 1);             // B row this lane fetches
+  const uint32_t lds_base = (uint32_t)(uintptr_t)(lds_ptr_t)lds;
+
+  // outputs: m = m0 + 32*wave + l32 ; n = n0 + 32*jb + (r%4) + 8*(r/4) + 4*lh
+  double acc[2][16];
+#pragma unroll
+  for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[jb][r] = 0.0;
+
+  for (int p = grp.p0; p < grp.p1; ++p) {
+    const SplitPair pr = pairs[p];
+    const int kpad = pr.kpad;
+    if (kpad <= 0) continue;
+    const int8_t* pa = planesA + pr.a_planes;
+    const int8_t* pb = planesB + pr.b_planes;
+    v16i lev[2][NL];
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+      for (int L = 0; L < NL; ++L) lev[jb][L] = v16i{0};
+
+    auto fill_b = [&](int k0, int st) {
+      v4i* sB = lds + st * B_ST;
+      const int c0 = k0 >> 4;
+#pragma unroll
+      for (int t = 0; t < NLD_W; ++t) {
+        const int q = wave + 2 * t, ch = q & 1, pl = q >> 1;
+        load_lds16(pb + pl * pr.b_pstride + ((int64_t)(c0 + ch) * grp.N + brow) * 16,
+                   sB + (pl * 2 + ch) * BN);
+      }
+    };
+    v4i av[S], an[S];
+    auto load_a = [&](int k0) {  // A fragments of the step at K byte k0 -> an
+      const int c = (k0 >> 4) + lh;
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+        an[i] = *reinterpret_cast<const v4i*>(pa + i * pr.a_pstride +
+                                              ((int64_t)c * grp.M + arow) * 16);
+    };
+
+    // prologue: B stage 0, A step 0, B stage 1 (in that order, so that
+    // waiting for all but the last NLD_W loads covers the first two). Past
+    // the last step the prefetches re-fetch the last step (clamped) into a
+    // stage / registers nobody reads, so every step issues the same loads
+    // and waits for the same count.
+    const int nsteps = kpad / BKB;
+    const int klast = (nsteps - 1) * BKB;
+    fill_b(0, 0);
+    load_a(0);
+    fill_b(min(BKB, klast), 1);
+    wait_vmcnt<NLD_W>();
+    stage_barrier();
+#pragma unroll
+    for (int i = 0; i < S; ++i) av[i] = an[i];
+    int st = 0;
+    for (int s = 0; s < nsteps; ++s) {
+      // A of the next step first, then the B fill two steps ahead: the wait
+      // at the end of the step then leaves only that fill in flight
+      load_a(min((s + 1) * BKB, klast));
+      int nst = st + 2;
+      if (nst >= NSTAGE) nst -= NSTAGE;
+      fill_b(min((s + 2) * BKB, klast), nst);
+      // B fragments of this stage; the second block's reads complete under
+      // the first block's MFMAs. Both prefetches stay ahead of the math.
+      const uint32_t sb = lds_base + (uint32_t)(st * B_ST + lh * BN + l32) * 16u;
+      v4i bv[2][S];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < S; ++j) bv[0][j] = lds_read16_untracked(sb + (j * 2 * BN) * 16);
+      wait_lgkmcnt0();
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < S; ++j) bv[1][j] = lds_read16_untracked(sb + (j * 2 * BN + 32) * 16);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int jb = 0; jb < 2; ++jb) {
+        if (jb == 1) {
+          __builtin_amdgcn_sched_barrier(0);
+          wait_lgkmcnt0();
+          __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+#pragma unroll
+          for (int j = 0; j < S - i; ++j)
+            lev[jb][i + j] =
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[jb][j], av[i], lev[jb][i + j], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      wait_vmcnt<NLD_W>();
+      stage_barrier();
+#pragma unroll
+      for (int i = 0; i < S; ++i) av[i] = an[i];
+      if (++st == NSTAGE) st = 0;
+    }
+    // the last (unused) fill must land before the next pair's prologue
+    // writes the stages again
+    wait_vmcnt<0>();
+    stage_barrier();
+
+    // FP64 recombination of the levels, scaled per row/column
+    const int gm = m0 + wave * 32 + l32;
+    const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gn = n0 + jb * 32 + (r & 3) + 8 * (r >> 2) + 4 * lh;
+        const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
+        double h = (double)lev[jb][NL - 1][r];
+#pragma unroll
+        for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[jb][L][r]);
+        double v = ldexp(h, ea + eb - 12);
+        if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
+        acc[jb][r] = fma(ga * pr.alpha, v, acc[jb][r]);
+      }
+  }
+
+  const int gm = m0 + wave * 32 + l32;
+  if (gm >= grp.M) return;
+#pragma unroll
+  for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int gn = n0 + jb * 32 + (r & 3) + 8 * (r >> 2) + 4 * lh;
+      if (gn >= grp.N) continue;
+      const int64_t o = grp.off_c + gm + (int64_t)gn * grp.ldc;
+      double out = acc[jb][r];
+      if (beta != 0.0) out = fma(beta, Cg[o], out);
+      Cg[o] = out;
+    }
+}
+
 }  // namespace
 
 int split_slices(int prec) { return prec == 2 ? kSliceCount : 0; }
@@ -562,19 +749,19 @@ hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const Slic
   if (prec != 2) return hipErrorNotSupported;
   const dim3 g((unsigned)grid), blk(kThreads);
   if (cplx) {
-    hipLaunchKernelGGL((slice_max_kernel<1>), g, blk, 0, s, blocks, prefix, nblk,
+    ITG_LAUNCH((slice_max_kernel<1>), g, blk, 0, s, blocks, prefix, nblk,
                        (const double*)src, (int32_t*)pmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((slice_kernel<kSliceCount, 1>), g, blk, 0, s, blocks, prefix, nblk,
+    ITG_LAUNCH((slice_kernel<kSliceCount, 1>), g, blk, 0, s, blocks, prefix, nblk,
                        (const double*)src, (int8_t*)planes, (int32_t*)exps,
                        (const int32_t*)pmax, nsets);
   } else {
-    hipLaunchKernelGGL((slice_max_kernel<0>), g, blk, 0, s, blocks, prefix, nblk,
+    ITG_LAUNCH((slice_max_kernel<0>), g, blk, 0, s, blocks, prefix, nblk,
                        (const double*)src, (int32_t*)pmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((slice_kernel<kSliceCount, 0>), g, blk, 0, s, blocks, prefix, nblk,
+    ITG_LAUNCH((slice_kernel<kSliceCount, 0>), g, blk, 0, s, blocks, prefix, nblk,
                        (const double*)src, (int8_t*)planes, (int32_t*)exps,
                        (const int32_t*)pmax, 1);
   }
@@ -592,7 +779,7 @@ hipError_t launch_sg(hipStream_t s, int comp, int sets, const SplitPair* pairs,
                      const void* planesA, const void* planesB, const void* expA,
                      const void* expB, void* C, double ga, double beta) {
   const int ns = CPLX ? 3 : 1;
-  hipLaunchKernelGGL((split_gemm_kernel<kSliceCount, CPLX, WM>), dim3((unsigned)grid),
+  ITG_LAUNCH((split_gemm_kernel<kSliceCount, CPLX, WM>), dim3((unsigned)grid),
                      dim3(128 * WM), 0, s, pairs, groups, tiles, (const int8_t*)planesA,
                      (const int8_t*)planesB, (const int32_t*)expA, (const int32_t*)expB,
                      (double*)C, ga, beta, CPLX ? comp : 0, CPLX ? sets : 0, ns, ns);
@@ -607,6 +794,13 @@ hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int com
                              double beta) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
+  if (wmb == 3) {  // A in registers (real data; 64 x 64 tile table)
+    if (cplx) return hipErrorInvalidValue;
+    ITG_LAUNCH((split_gemm_areg_kernel<kSliceCount>), dim3((unsigned)grid), dim3(128), 0,
+                       s, pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
+                       (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta);
+    return hipGetLastError();
+  }
   if (wmb != 1 && wmb != 2) return hipErrorInvalidValue;
   if (cplx)
     return wmb == 2 ? launch_sg<1, 4>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,

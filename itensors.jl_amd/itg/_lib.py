@@ -38,7 +38,7 @@ EXPORTS = (
     "itg_bs_execute", "itg_bs_plan_destroy", "itg_bs_permute", "itg_bs_autotune",
     "itg_batch_plan_create", "itg_batch_execute", "itg_batch_plan_destroy",
     "itg_malloc", "itg_free", "itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod",
-    "itg_timer_enable", "itg_timer_count", "itg_timer_read",
+    "itg_timer_enable", "itg_timer_count", "itg_timer_read", "itg_set_split_tile",
 )
 
 
@@ -108,6 +108,7 @@ def lib():
         for n in ("itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod"):
             getattr(L, n).argtypes = [P, P, P, SZ]
         L.itg_timer_enable.argtypes = [P, I]
+        L.itg_set_split_tile.argtypes = [P, I]
         L.itg_timer_count.argtypes = [P]
         L.itg_timer_read.argtypes = [P, I64, ctypes.POINTER(ctypes.c_char_p), I64P, DP]
         for name in EXPORTS:

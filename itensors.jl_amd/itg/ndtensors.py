@@ -74,6 +74,17 @@ def _handle(dev):
     return h
 
 
+def set_split_tile(tile: int, device=None) -> None:
+    """Tile shape of the split-precision GEMM on `device`: 0 = per plan
+    (autotuned, else by size), 1 = 64 x 64, 2 = 128 x 64, 3 = 64 x 64 with
+    the A operand in registers (real data; complex uses 1). The shapes agree
+    bit for bit; this exists for tests and tuning."""
+    dev = torch.device(device) if device is not None else torch.device(
+        "cuda", torch.cuda.current_device())
+    h = _handle(dev)
+    h.check(_lib.lib().itg_set_split_tile(h.h, int(tile)))
+
+
 class timer:
     """Per-kernel device timing, the counterpart of NDTensors' global
     `timer::TimerOutput` (NDTensors/src/NDTensors.jl:96-99): inside the block

@@ -76,6 +76,7 @@ def test_null_handle_is_rejected():
     assert L.itg_bs_plan_destroy(None) == _lib.ITG_OK
     assert L.itg_timer_enable(None, 1) == _lib.ITG_ERR_INVALID
     assert L.itg_timer_count(None) == 0
+    assert L.itg_set_split_tile(None, 1) == _lib.ITG_ERR_INVALID
 
 
 # ---------------------------------------------------------------- host logic

@@ -61,6 +61,26 @@ def test_dense_mpo_step_both_orders(env, cplx):
         assert err < FP64_RTOL, (first_big, "alpha/beta", err)
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_dense_beyond_matrix_form(env, cplx):
+    """More distinct A values (K = 20 > 16 slots) or more outputs per fiber
+    (40 > 32) than the matrix form holds: the loop form of the kernel."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(22)
+    for X, lX, W, lW in [
+        (rand(rng, (37, 20, 41), cplx), (1, -1, 2), rand(rng, (20, 3), cplx), (-1, 3)),
+        (rand(rng, (37, 4, 41), cplx), (1, -1, 2), rand(rng, (4, 40), cplx), (-1, 3)),
+    ]:
+        for T1, l1, T2, l2 in ((X, lX, W, lW), (W, lW, X, lX)):
+            lR = ND.contract_labels(l1, l2)
+            T = ND.contract(ND.DenseTensor.from_numpy(T1), l1, ND.DenseTensor.from_numpy(T2), l2,
+                            lR, conj1=cplx)
+            torch.cuda.synchronize()
+            ref = O.dense_contract(None, lR, np.conj(T1) if cplx else T1, l1, T2, l2, exact=True)
+            err = relerr(T.to_numpy(), ref)
+            assert err < FP64_RTOL, (T1.shape, T2.shape, err)
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
 def test_blocksparse_mpo_steps_both_orders(env, dtype):
     torch, itg, ND, O = env

@@ -163,6 +163,51 @@ def test_split_large_dense_tiles(env, cplx):
         assert err < SPLIT_RTOL, err
 
 
+@pytest.mark.parametrize("shape", [(200, 150, 300), (70, 64, 20), (64, 130, 2000)])
+def test_split_tile_shapes_agree(env, shape):
+    """The split tile shapes (64 x 64, 128 x 64, 64 x 64 with A in registers)
+    form the same exact INT32 level sums and recombine them in the same
+    order, so their results agree bit for bit. K = 20 is a single K step,
+    K = 300 ends in a partial one, K = 2000 runs the stage ring through all
+    its phases; M, N leave partial tiles."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(21)
+    M, N, K = shape
+    A = rand(rng, (M, K))
+    B = rand(rng, (K, N))
+    outs = []
+    try:
+        for tile in (1, 2, 3):
+            ND.set_split_tile(tile)
+            outs.append(run(env, A, (1, -1), B, (-1, 2), (1, 2)))
+    finally:
+        ND.set_split_tile(0)
+    assert relerr(outs[0], A @ B) < SPLIT_RTOL
+    for tile, o in zip((2, 3), outs[1:]):
+        assert np.array_equal(o, outs[0]), tile
+
+
+def test_split_tile_shapes_agree_blocksparse(env):
+    """Grouped block-sparse products (several block pairs per output block,
+    ragged block sizes): every tile shape gives the same bits."""
+    torch, itg, ND, O = env
+    from itg.configs import heff_network
+    L, psi, W1, W2, R, ix = heff_network(128, qns=True, seed=3)
+    outs = []
+    try:
+        for tile in (1, 2, 3):
+            ND.set_split_tile(tile)
+            cur = L
+            for T in (psi, W1, W2, R):
+                cur = itg.contract(cur, T, prec=PREC)
+            torch.cuda.synchronize()
+            outs.append(cur.tensor.data.cpu().numpy())
+    finally:
+        ND.set_split_tile(0)
+    for tile, o in zip((2, 3), outs[1:]):
+        assert np.array_equal(o, outs[0]), tile
+
+
 def test_split_nonfinite_propagates(env):
     rng = np.random.default_rng(14)
     A = rand(rng, (20, 64))
