@@ -208,6 +208,31 @@ def test_split_tile_shapes_agree_blocksparse(env):
         assert np.array_equal(o, outs[0]), tile
 
 
+def test_split_repeated_calls_one_plan(env):
+    """The same (cached) plan run again on the same data gives the same
+    bits, and on new data the new product: nothing of the previous call's
+    partial row maxima, exponents or slices leaks into the next. K = 1000
+    gives 8 K parts per row tile; a zero row and a non-finite row are
+    included."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(22)
+    M, N, K = 300, 200, 1000
+    A = rand(rng, (M, K))
+    B = rand(rng, (K, N))
+    A[7, :] = 0.0
+    first = run(env, A, (1, -1), B, (-1, 2), (1, 2))
+    again = run(env, A, (1, -1), B, (-1, 2), (1, 2))
+    assert np.array_equal(first, again)
+    assert relerr(first, A @ B) < SPLIT_RTOL
+    assert np.all(first[7, :] == 0.0)
+    A2 = rand(rng, (M, K))
+    A2[11, 500] = np.inf
+    third = run(env, A2, (1, -1), B, (-1, 2), (1, 2))
+    assert np.all(np.isnan(third[11, :]))
+    keep = np.delete(np.arange(M), 11)
+    assert relerr(third[keep], (A2 @ B)[keep]) < SPLIT_RTOL
+
+
 def test_split_nonfinite_propagates(env):
     rng = np.random.default_rng(14)
     A = rand(rng, (20, 64))
