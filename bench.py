@@ -52,6 +52,9 @@ def parse():
 
 CALIB_MIB = 256
 KERNELS = {"fp64": "gemm_f64_kernel", "split_int8": "split_gemm_kernel"}
+# dispatch-name prefixes of every tile variant of each mode's GEMM (the
+# library's timer reports them all under the KERNELS name)
+KERNEL_PREFIX = {"fp64": "gemm_f64_", "split_int8": "split_gemm_"}
 
 
 def traffic_probe(args):
@@ -131,7 +134,8 @@ def measure_traffic(args):
         scale = CALIB_MIB * 1024.0 / max(copies)  # counters read KiB
         for m, kname in KERNELS.items():
             n = int(counts.get(m, 0))
-            v = [float(r["Counter_Value"]) for r in rows if kname in r["Kernel_Name"]]
+            v = [float(r["Counter_Value"]) for r in rows
+                 if f"::{KERNEL_PREFIX[m]}" in r["Kernel_Name"]]
             if n > 0 and len(v) >= 3 * n:
                 # the last 3 steps, after tuning: bytes per launch
                 out[m][j] = sum(v[-3 * n:]) / (3 * n) * scale * 1024.0
