@@ -222,6 +222,26 @@ itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
                           const int64_t* offs_dst, const int32_t* perm,
                           const void* src, void* dst);
 
+/* Grouped strided block copy: for each block b (0 <= b < nblk) and every
+ * multi-index x over dims[b*nmodes .. b*nmodes+nmodes),
+ *   dst[dst_off[b] + sum_m x_m*dst_str[b*nmodes+m]]
+ *       = src[src_off[b] + sum_m x_m*src_str[b*nmodes+m]]
+ * with element offsets and strides (a ComplexF64 counts as one element).
+ * src_len / dst_len are the lengths of the two device arrays in elements;
+ * every access is checked against them on the host before anything runs
+ * (ITG_ERR_INVALID otherwise). Destinations of different elements must not
+ * coincide. This is the data movement of the block-sparse combiner
+ * contraction: the per-block permutedims! of a block into its range of a
+ * merged output block when combining (permutedims_combine,
+ * NDTensors/src/blocksparse/blocksparsetensor.jl:571-630) and the copyto!
+ * of a range of a merged block into an output block when uncombining
+ * (uncombine, blocksparsetensor.jl:726-795). Synchronous on the handle's
+ * stream; not for use under stream capture (it uploads its tables). */
+itg_status itg_bs_copy(itg_handle* h, itg_dtype dtype, int64_t nblk, int nmodes,
+                       const int64_t* dims, const int64_t* src_off, const int64_t* src_str,
+                       const int64_t* dst_off, const int64_t* dst_str, const void* src,
+                       int64_t src_len, void* dst, int64_t dst_len);
+
 /* ---------------------------------------------------- batched contractions */
 
 typedef struct itg_batch_plan itg_batch_plan;

@@ -7,6 +7,7 @@
 // itg_bs_plan_create and executed any number of times.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <map>
@@ -678,6 +679,71 @@ itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
   if (e != hipSuccess) return hip_fail(h, e, "permute launch");
   e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "permute");
+  return ITG_OK;
+}
+
+itg_status itg_bs_copy(itg_handle* h, itg_dtype dtype, int64_t nblk, int nmodes,
+                       const int64_t* dims, const int64_t* src_off, const int64_t* src_str,
+                       const int64_t* dst_off, const int64_t* dst_str, const void* src,
+                       int64_t src_len, void* dst, int64_t dst_len) {
+  if (!h) return ITG_ERR_INVALID;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (nmodes < 0 || nmodes > kMaxModes) return fail(h, ITG_ERR_UNSUPPORTED, "too many modes");
+  if (nblk < 0) return fail(h, ITG_ERR_INVALID, "negative block count");
+  if (nblk == 0) return ITG_OK;
+  if (!src_off || !dst_off || !src || !dst || (nmodes > 0 && (!dims || !src_str || !dst_str)))
+    return fail(h, ITG_ERR_INVALID, "null argument");
+  std::vector<CopyBlock> cb((size_t)nblk);
+  std::vector<int64_t> prefix(1, 0);
+  for (int64_t b = 0; b < nblk; ++b) {
+    CopyBlock& d = cb[(size_t)b];
+    std::memset(&d, 0, sizeof(d));
+    const int64_t* db = dims + b * nmodes;
+    const int64_t* sb = src_str + b * nmodes;
+    const int64_t* tb = dst_str + b * nmodes;
+    // traversal order: the destination's smallest stride first (coalesced stores)
+    int ord[kMaxModes];
+    for (int i = 0; i < nmodes; ++i) ord[i] = i;
+    std::stable_sort(ord, ord + nmodes, [&](int x, int y) {
+      return std::llabs(tb[x]) < std::llabs(tb[y]);
+    });
+    int64_t n = 1, slo = src_off[b], shi = src_off[b], dlo = dst_off[b], dhi = dst_off[b];
+    for (int i = 0; i < nmodes; ++i) {
+      const int m = ord[i];
+      if (db[m] < 0) return fail(h, ITG_ERR_INVALID, "negative dimension");
+      d.dims[i] = (int32_t)std::min<int64_t>(db[m], INT32_MAX);
+      d.sstr[i] = sb[m];
+      d.dstr[i] = tb[m];
+      n *= db[m];
+      if (n > INT32_MAX) return fail(h, ITG_ERR_UNSUPPORTED, "block too large");
+      if (db[m] > 0) {
+        const int64_t es = (db[m] - 1) * sb[m], ed = (db[m] - 1) * tb[m];
+        (es < 0 ? slo : shi) += es;
+        (ed < 0 ? dlo : dhi) += ed;
+      }
+    }
+    for (int i = nmodes; i < kMaxModes; ++i) d.dims[i] = 1;
+    d.off_src = src_off[b];
+    d.off_dst = dst_off[b];
+    d.nelem = n;
+    if (n > 0 && (slo < 0 || shi >= src_len || dlo < 0 || dhi >= dst_len))
+      return fail(h, ITG_ERR_INVALID, "block copy reaches outside the source or destination");
+    prefix.push_back(prefix.back() + copy_block_wgs(n));
+  }
+  DeviceGuard g(h->device);
+  DevBuf tb, tp;
+  hipError_t e;
+  if ((e = tb.upload(cb.data(), cb.size() * sizeof(CopyBlock), h->stream)) != hipSuccess ||
+      (e = tp.upload(prefix.data(), prefix.size() * sizeof(int64_t), h->stream)) != hipSuccess)
+    return hip_fail(h, e, "copy tables");
+  KernelTimer* kt = active_timer(h);
+  if (kt) kt->begin("copy_blocks_kernel", h->stream);
+  e = launch_copy_blocks(h->stream, dtype == ITG_C128, std::max(nmodes, 1), (const CopyBlock*)tb.p,
+                         (const int64_t*)tp.p, nblk, prefix.back(), src, dst);
+  if (kt) kt->end(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "copy launch");
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "copy");
   return ITG_OK;
 }
 

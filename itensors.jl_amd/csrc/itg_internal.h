@@ -118,6 +118,19 @@ hipError_t launch_permute(hipStream_t s, int cplx, int nm, const int32_t* perm_h
 // 1 = 2-D tile transpose of src mode 0 and src mode perm[0].
 int64_t permute_block_tiles(int nm, const int32_t* perm, const int32_t* dims, int mode_kind);
 
+// One block of a grouped strided copy (element offsets and strides; modes
+// in traversal order, mode 0 fastest; nelem = product of dims < 2^31).
+struct CopyBlock {
+  int64_t off_src, off_dst;
+  int64_t nelem;
+  int32_t dims[kMaxModes];
+  int64_t sstr[kMaxModes], dstr[kMaxModes];
+};
+hipError_t launch_copy_blocks(hipStream_t s, int cplx, int nm, const CopyBlock* blocks_dev,
+                              const int64_t* prefix_dev, int64_t nblk, int64_t total_wgs,
+                              const void* src, void* dst);
+int64_t copy_block_wgs(int64_t nelem);  // workgroups of one block (host helper)
+
 // wg_group: the group of every workgroup (total_tiles entries).
 hipError_t launch_direct(hipStream_t s, int cplx, int nr, const DirectPair* pairs,
                          const DirectGroup* groups, const int32_t* koffs,

@@ -197,6 +197,40 @@ __global__ void __launch_bounds__(kThreads)
   x[i] = axpby(b_re, b_im, x[i], 0.0, 0.0, y, false);
 }
 
+// Grouped strided copy: dst[off_dst + sum_m x_m dstr[m]] = src[off_src +
+// sum_m x_m sstr[m]] over every multi-index x of each block. A workgroup
+// takes kElemsPerWG consecutive elements of one block in the block's mode
+// order (mode 0 fastest; the host puts the destination's unit-stride mode
+// first so that stores coalesce).
+template <int CPLX>
+__global__ void __launch_bounds__(kThreads)
+    copy_blocks_kernel(const CopyBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
+                       int64_t nblk, int nm, const typename Elem<CPLX>::T* __restrict__ src,
+                       typename Elem<CPLX>::T* __restrict__ dst) {
+  const int64_t wg = blockIdx.x;
+  const int64_t b = find_block(prefix, nblk, wg);
+  const CopyBlock& d = blocks[b];
+  const int64_t e0 = (wg - prefix[b]) * kElemsPerWG;
+#pragma unroll
+  for (int u = 0; u < kElemsPerThread; ++u) {
+    const int64_t e = e0 + u * kThreads + threadIdx.x;
+    if (e >= d.nelem) break;
+    uint32_t rem = (uint32_t)e;
+    int64_t so = d.off_src, dof = d.off_dst;
+    for (int m = 0; m < nm; ++m) {
+      uint32_t x = rem;
+      if (m + 1 < nm) {
+        const uint32_t q = rem / (uint32_t)d.dims[m];
+        x = rem - q * (uint32_t)d.dims[m];
+        rem = q;
+      }
+      so += (int64_t)x * d.sstr[m];
+      dof += (int64_t)x * d.dstr[m];
+    }
+    dst[dof] = src[so];
+  }
+}
+
 }  // namespace
 
 hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re, double b_im) {
@@ -248,6 +282,22 @@ hipError_t launch_permute(hipStream_t s, int cplx, int nm, const int32_t* perm_h
                          tile_prefix_dev, nblk, (const double*)src, (double*)dst, a_re, a_im,
                          b_re, b_im);
   }
+  return hipGetLastError();
+}
+
+int64_t copy_block_wgs(int64_t nelem) { return (nelem + kElemsPerWG - 1) / kElemsPerWG; }
+
+hipError_t launch_copy_blocks(hipStream_t s, int cplx, int nm, const CopyBlock* blocks_dev,
+                              const int64_t* prefix_dev, int64_t nblk, int64_t total_wgs,
+                              const void* src, void* dst) {
+  if (total_wgs == 0) return hipSuccess;
+  dim3 grid((unsigned)total_wgs), block(kThreads);
+  if (cplx)
+    ITG_LAUNCH(copy_blocks_kernel<1>, grid, block, 0, s, blocks_dev, prefix_dev, nblk, nm,
+               (const double2*)src, (double2*)dst);
+  else
+    ITG_LAUNCH(copy_blocks_kernel<0>, grid, block, 0, s, blocks_dev, prefix_dev, nblk, nm,
+               (const double*)src, (double*)dst);
   return hipGetLastError();
 }
 

@@ -39,6 +39,7 @@ EXPORTS = (
     "itg_batch_plan_create", "itg_batch_execute", "itg_batch_plan_destroy",
     "itg_malloc", "itg_free", "itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod",
     "itg_timer_enable", "itg_timer_count", "itg_timer_read", "itg_set_split_tile",
+    "itg_bs_copy",
 )
 
 
@@ -97,6 +98,7 @@ def lib():
         L.itg_bs_plan_destroy.argtypes = [P]
         L.itg_bs_permute.argtypes = [P, I, I, I64, I64P, I64P, I64P, I32P, P, P]
         L.itg_bs_autotune.argtypes = [P, P, I, P, P, P, I]
+        L.itg_bs_copy.argtypes = [P, I, I64, I, I64P, I64P, I64P, I64P, I64P, P, I64, P, I64]
         PP = ctypes.POINTER(P)
         L.itg_batch_plan_create.argtypes = [P, I, I64, I, I64P, I64P, PP, I, I64P, I64P, PP,
                                             I, I64P, I64P, PP, ctypes.POINTER(P)]

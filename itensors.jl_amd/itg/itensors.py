@@ -10,6 +10,10 @@
   A * B, contract(A, B)       src/tensor_operations/tensor_algebra.jl:1-77
   contract(As; sequence)      src/tensor_operations/tensor_algebra.jl:106-159
   dag(T)                      src/itensor.jl:1806-1846
+  combiner, combinedind       src/qn/qnitensor.jl:462-467, src/tensor_operations/
+                              itensor_combiner.jl:1-34; index fusion src/qn/qnindex.jl:
+                              359-388 with NDTensors/src/blocksparse/blockdims.jl:165-174;
+                              combineblocks src/qn/qnindex.jl:407-424
 
 Storage is an `ndtensors.DenseTensor` (no QNs) or `ndtensors.BlockSparseTensor`
 (QN indices) on the HIP device; every contraction runs in libitg.so.
@@ -23,6 +27,7 @@ from collections import OrderedDict
 
 import numpy as np
 
+from . import combiners as CB
 from . import ndtensors as ND
 
 Out, In, Neither = 1, -1, 0
@@ -232,6 +237,8 @@ class ITensor:
         return self.tensor.eltype
 
     def hasqns(self):
+        if isinstance(self.tensor, CB.CombinerTensor):
+            return bool(self.inds) and all(i.hasqns for i in self.inds)
         return isinstance(self.tensor, ND.BlockSparseTensor)
 
     def nnzblocks(self):
@@ -288,7 +295,11 @@ class ITensor:
 
 def dag(T):
     """dag(T): conjugate the data and flip every arrow (src/itensor.jl:1806-1846).
-    Real data is aliased, as the reference does when aliasing is allowed."""
+    Real data is aliased, as the reference does when aliasing is allowed. A
+    combiner flips its arrows and its storage's conjugation flag
+    (NDTensors/src/combiner/combiner.jl:39-40)."""
+    if iscombiner(T):
+        return ITensor(T.tensor.conj(), tuple(i.dag() for i in T.inds))
     data = T.tensor.data
     if data.is_complex():
         data = data.conj().resolve_conj()
@@ -367,6 +378,89 @@ def blocksparse_itensor(data, blocks, inds, device="cuda"):
     return ITensor(ND.BlockSparseTensor(t, offs, bdims), inds)
 
 
+# --------------------------------------------------------------- combiner
+def _qn_order_key(names):
+    """QN ordering (src/lib/QuantumNumbers/src/qn.jl:273-291): values compared
+    in name order, a missing name counting as zero."""
+    return lambda q: tuple(q.vals.get(n, (0, 1))[0] for n in names)
+
+
+def combineblocks(space):
+    """src/qn/qnindex.jl:407-424: stable sort of the sectors by QN, then
+    merge neighbours with equal QN. Returns (merged space, perm, comb),
+    perm and comb 1-based (comb[i] = merged sector of sorted sector i)."""
+    names = sorted({n for q, _ in space for n in q.vals})
+    key = _qn_order_key(names)
+    perm = sorted(range(len(space)), key=lambda i: key(space[i][0]))
+    merged, comb = [], []
+    for i in perm:
+        q, d = space[i]
+        if merged and merged[-1][0] == q:
+            merged[-1] = (q, merged[-1][1] + d)
+        else:
+            merged.append((q, d))
+        comb.append(len(merged))
+    return merged, [i + 1 for i in perm], comb
+
+
+def fused_space(inds, dir):
+    """Sectors of ⊗(inds...; dir): dir * sum_k dir(i_k) * qn_k with the first
+    index's sectors fastest, sizes multiplied (src/qn/qnindex.jl:359-388;
+    the left fold of NDTensors/src/blocksparse/blockdims.jl:165-167 leaves
+    the intermediate arrows cancelled)."""
+    spaces = [[(i.dir * q, d) for q, d in i.space] for i in inds]
+    out = []
+    for combo in itertools.product(*reversed(spaces)):
+        q, d = QN(), 1
+        for qk, dk in reversed(combo):
+            q = q + qk
+            d *= dk
+        out.append((dir * q, d))
+    return out
+
+
+def combiner(*inds, dir=None, tags="CMB,Link"):
+    """combiner(is...; dir, tags): an ITensor with Combiner storage whose
+    indices are (combined index, dag.(is)...) (src/qn/qnitensor.jl:462-467,
+    src/tensor_operations/itensor_combiner.jl:1-12). For QN indices the
+    combined index holds the product sectors sorted by QN and merged; its
+    arrow defaults to In when every index is In, else Out."""
+    if len(inds) == 1 and isinstance(inds[0], (tuple, list)):
+        inds = tuple(inds[0])
+    if not inds:
+        return ITensor(CB.CombinerTensor((), (), ()), ())
+    if all(i.hasqns for i in inds):
+        if dir is None:
+            dir = In if all(i.dir == In for i in inds) else Out
+        merged, perm, comb = combineblocks(fused_space(inds, dir))
+        c = Index(merged, tags, dir=dir)
+    else:
+        c = Index(math.prod(i.dim for i in inds), tags)
+        perm, comb = (), ()
+    C = CB.CombinerTensor(perm, comb, (c.blockdims(),) + tuple(i.blockdims() for i in inds))
+    return ITensor(C, (c,) + tuple(i.dag() for i in inds))
+
+
+def iscombiner(T):
+    return isinstance(T.tensor, CB.CombinerTensor)
+
+
+def combinedind(C):
+    """The combined index of a combiner, its first index (itensor_combiner.jl:25-30)."""
+    return C.inds[0] if iscombiner(C) and C.inds else None
+
+
+def _contract_combiner(A, B):
+    lA, lB = compute_contraction_labels(A.inds, B.inds)
+    first = iscombiner(A)
+    C, T, lC, lT = (A, B, lA, lB) if first else (B, A, lB, lA)
+    if iscombiner(T):
+        raise ValueError("contracting two combiners")
+    R, lR = CB.contract(T.tensor, lT, C.tensor, lC, combiner_first=first)
+    indsR = tuple(C.inds[lC.index(x)] if x in lC else T.inds[lT.index(x)] for x in lR)
+    return ITensor(R, indsR)
+
+
 # ------------------------------------------------------------- contraction
 _warn_order = [14]
 
@@ -375,6 +469,8 @@ def contract(A, B, *, prec=None, conjA=False, conjB=False):
     """A * B (src/tensor_operations/tensor_algebra.jl:1-77): contract every
     index A and B share; the result holds A's free indices (in A's order)
     then B's. QN indices must meet with opposite arrows."""
+    if iscombiner(A) or iscombiner(B):
+        return _contract_combiner(A, B)
     if len(A.inds) + len(B.inds) >= _warn_order[0] * 2:
         warnings.warn("contraction of high-order tensors")
     lA, lB = compute_contraction_labels(A.inds, B.inds)

@@ -427,3 +427,166 @@ def heff_apply_dense(L, psi, W1, W2, R, exact=False):
     Z, lZ = contract_dense(Y, (1, -1, 3, 4, -2), W2, (-2, -1, 6, 7), exact=exact)  # Z[a',b,s1',s2',w'']
     O, lO = contract_dense(Z, (1, -1, 3, 4, -2), R, (5, -2, -1), exact=exact)     # O[a',s1',s2',b']
     return O
+
+
+# ------------------------------------------------------------------ combiner
+# A combiner fuses indices i_1..i_n (first fastest) into one index whose
+# sectors are the product sectors sorted by QN and merged when equal
+# (src/qn/qnitensor.jl:462-467, src/qn/qnindex.jl:359-424). Restated here
+# element by element: every multi-index (x_1..x_n) of the fused indices has
+# one position in the combined index, and the combined tensor is the dense
+# tensor re-indexed through that map, cut into the reference's blocks.
+
+
+def qn_order_key(names, q):
+    """QN ordering (src/lib/QuantumNumbers/src/qn.jl:273-291): values in
+    name order, missing names counting as zero."""
+    return tuple(q.get(n, (0, 1))[0] for n in names)
+
+
+def combined_index(inds, dir=None):
+    """(sectors of the combined index, sorted product sector order, merge map).
+
+    dir defaults to In (-1) when every index is In, else Out (+1): the left
+    fold of outer (NDTensors/src/blocksparse/blockdims.jl:165-167) with
+    src/qn/qnindex.jl:369-379 at each step. Product sectors: first index
+    fastest, QN = dir * sum_k dir_k * q_k, size = product. Sorting is stable
+    (Julia's sortperm); merged neighbours have equal QN (qnindex.jl:407-424).
+    Returned order / merge are 0-based lists over sorted positions."""
+    if dir is None:
+        dir = -1 if all(i.dir == -1 for i in inds) else 1
+    grids = [range(i.nblocks()) for i in inds]
+    prod = []
+    for rev in itertools.product(*reversed(grids)):
+        bs = tuple(reversed(rev))
+        q = {}
+        d = 1
+        for i, b in zip(inds, bs):
+            q = qn_add(q, qn_scale(i.dir, i.qn(b)))
+            d *= i.blockdim(b)
+        prod.append((bs, qn_scale(dir, q), d))
+    names = sorted({n for _b, q, _d in prod for n in q})
+    order = sorted(range(len(prod)), key=lambda k: qn_order_key(names, prod[k][1]))
+    sectors, merge = [], []
+    for k in order:
+        _bs, q, d = prod[k]
+        if sectors and qn_eq(sectors[-1][0], q):
+            sectors[-1] = (sectors[-1][0], sectors[-1][1] + d)
+        else:
+            sectors.append((q, d))
+        merge.append(len(sectors) - 1)
+    return sectors, order, merge, prod, dir
+
+
+def combined_position_map(inds, order):
+    """For every multi-index x of the fused indices (dense, 0-based), its
+    position in the combined index: sorted product sectors laid end to end,
+    each sector's elements column-major over its block-local indices."""
+    grids = [range(i.nblocks()) for i in inds]
+    prods = []
+    for rev in itertools.product(*reversed(grids)):
+        prods.append(tuple(reversed(rev)))
+    starts = [np.concatenate([[0], np.cumsum([i.blockdim(b) for b in range(i.nblocks())])]) for i in inds]
+    pos = {}
+    at = 0
+    for k in order:
+        bs = prods[k]
+        dims = [inds[m].blockdim(b) for m, b in enumerate(bs)]
+        for rev in itertools.product(*[range(d) for d in reversed(dims)]):
+            loc = tuple(reversed(rev))
+            x = tuple(int(starts[m][b]) + l for m, (b, l) in enumerate(zip(bs, loc)))
+            pos[x] = at
+            at += 1
+    return pos
+
+
+def _sector_of(starts, x):
+    return int(np.searchsorted(starts, x, side="right") - 1)
+
+
+def combine_blocksparse(T, lT, cinds, lC, dir=None):
+    """A * combiner(cinds...) for a block-sparse A (NDTensors/src/blocksparse/
+    combiner.jl:25-81, blocksparsetensor.jl:537-630). lC = (combined label,
+    labels of cinds in T). Returns (BlockSparse R, labels (combined, rest of
+    T in order), combined Index).
+
+    Output blocks: each T block's (sorted position of its product sector,
+    rest) in column-major order, first coordinate then replaced by its
+    merged sector, first occurrences kept (blocksparsetensor.jl:560-567,
+    blockoffsets.jl:35-37)."""
+    clabel, luc = lC[0], tuple(lC[1:])
+    ucpos = [lT.index(x) for x in luc]
+    rest = [p for p in range(len(lT)) if lT[p] not in luc]
+    lR = (clabel,) + tuple(lT[p] for p in rest)
+    sectors, order, merge, prod, cdir = combined_index(cinds, dir)
+    cind = Index(sectors, "CMB,Link", dir=cdir)
+    where = {prod[k][0]: s for s, k in enumerate(order)}  # product sector -> sorted position
+    keys = []
+    for b in T.offsets:
+        s = where[tuple(b[p] for p in ucpos)]
+        keys.append((s,) + tuple(b[p] for p in rest))
+    keys.sort(key=lambda t: t[::-1])
+    blocks = []
+    for k in keys:
+        nb = (merge[k[0]],) + k[1:]
+        if nb not in blocks:
+            blocks.append(nb)
+    indsR = (cind,) + tuple(T.inds[p] for p in rest)
+    offs, nnz = blockoffsets(blocks, indsR)
+    # dense re-indexing through the position map, then cut into blocks
+    pos = combined_position_map([T.inds[p] for p in ucpos], order)
+    Td = T.dense()
+    Rd = np.zeros(tuple(i.dim for i in indsR), dtype=Td.dtype)
+    it = np.nditer(Td, flags=["multi_index"])
+    for v in it:
+        x = it.multi_index
+        Rd[(pos[tuple(x[p] for p in ucpos)],) + tuple(x[p] for p in rest)] = v
+    data = np.zeros(nnz, dtype=Td.dtype)
+    R = BlockSparse(data, offs, indsR)
+    starts = [np.concatenate([[0], np.cumsum([i.blockdim(b) for b in range(i.nblocks())])]) for i in indsR]
+    for b, o in offs.items():
+        sl = tuple(slice(starts[m][c], starts[m][c + 1]) for m, c in enumerate(b))
+        blk = Rd[sl]
+        data[o:o + blk.size] = blk.reshape(-1, order="F")
+    return R, lR, cind
+
+
+def uncombine_blocksparse(T, lT, cind, cinds, lC, order, merge):
+    """A * dag(combiner) for a block-sparse A holding the combined index
+    (NDTensors/src/blocksparse/combiner.jl:82-140, blocksparsetensor.jl:
+    664-795). lC = (combined label, labels the uncombined indices get).
+    Returns (BlockSparse R, labels (uncombined, rest of T in order)).
+    Output blocks: for each T block in order, the sorted sectors merged into
+    its combined sector in sorted order, as product sectors."""
+    clabel, luc = lC[0], tuple(lC[1:])
+    cpos = lT.index(clabel)
+    rest = [p for p in range(len(lT)) if p != cpos]
+    lR = luc + tuple(lT[p] for p in rest)
+    grids = [range(i.nblocks()) for i in cinds]
+    prods = [tuple(reversed(rev)) for rev in itertools.product(*reversed(grids))]
+    blocks = []
+    for b in T.offsets:
+        for s in range(len(order)):
+            if merge[s] == b[cpos]:
+                blocks.append(prods[order[s]] + tuple(b[p] for p in rest))
+    indsR = tuple(cinds) + tuple(T.inds[p] for p in rest)
+    offs, nnz = blockoffsets(blocks, indsR)
+    pos = combined_position_map(list(cinds), order)
+    Td = T.dense()
+    Rd = np.zeros(tuple(i.dim for i in indsR), dtype=Td.dtype)
+    n = len(cinds)
+    it = np.nditer(Rd, flags=["multi_index"])
+    for _v in it:
+        x = it.multi_index
+        src = [0] * len(lT)
+        src[cpos] = pos[x[:n]]
+        for r, p in enumerate(rest):
+            src[p] = x[n + r]
+        Rd[x] = Td[tuple(src)]
+    data = np.zeros(nnz, dtype=Td.dtype)
+    starts = [np.concatenate([[0], np.cumsum([i.blockdim(b) for b in range(i.nblocks())])]) for i in indsR]
+    for b, o in offs.items():
+        sl = tuple(slice(starts[m][c], starts[m][c + 1]) for m, c in enumerate(b))
+        blk = Rd[sl]
+        data[o:o + blk.size] = blk.reshape(-1, order="F")
+    return BlockSparse(data, offs, indsR), lR

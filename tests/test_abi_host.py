@@ -77,6 +77,8 @@ def test_null_handle_is_rejected():
     assert L.itg_timer_enable(None, 1) == _lib.ITG_ERR_INVALID
     assert L.itg_timer_count(None) == 0
     assert L.itg_set_split_tile(None, 1) == _lib.ITG_ERR_INVALID
+    assert L.itg_bs_copy(None, _lib.ITG_F64, 1, 1, None, None, None, None, None, None, 0,
+                         None, 0) == _lib.ITG_ERR_INVALID
 
 
 # ---------------------------------------------------------------- host logic
