@@ -202,4 +202,77 @@ function NDTensors.contract(
     end
 end
 
+# --------------------------------------------------------------- combiner
+# Block-sparse combiner contraction (NDTensors/src/blocksparse/combiner.jl):
+# NDTensors' own bookkeeping chooses the output blocks; each block's
+# permutedims! into (or copyto! out of) a range of a merged block becomes one
+# descriptor of a single grouped strided copy on the device (itg_bs_copy,
+# bounds-checked on the host by the library).
+function itg_copy!(R::ItgVector{T}, S::ItgVector{T}, descs) where {T}
+    isempty(descs) && return R
+    nm = maximum(d -> length(d[1]), descs)
+    pad(v, x) = vcat(collect(Int64, v), fill(Int64(x), nm - length(v)))
+    dims = reduce(vcat, [pad(d[1], 1) for d in descs])
+    ss = reduce(vcat, [pad(d[3], 0) for d in descs])
+    ds = reduce(vcat, [pad(d[5], 0) for d in descs])
+    so = Int64[d[2] for d in descs]
+    dof = Int64[d[4] for d in descs]
+    check(ccall((:itg_bs_copy, libitg), Cint,
+                (Ptr{Cvoid}, Cint, Int64, Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Int64}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64),
+                handle(), eltcode(T), length(descs), nm, dims, so, ss, dof, ds,
+                S.ptr, length(S), R.ptr, length(R)))
+    return R
+end
+
+colmajor_strides(d) = Int64[prod(d[1:(k - 1)]; init = 1) for k in 1:length(d)]
+
+# permutedims_combine (NDTensors/src/blocksparse/blocksparsetensor.jl:571-630)
+# for device storage: the same per-block offsets, one copy launch.
+function NDTensors.permutedims_combine(
+        T::BlockSparseTensor{ElT, N, <:Any, <:BlockSparse{<:Any, <:ItgVector}}, is,
+        perm::NTuple{N, Int}, combdims::NTuple{NC, Int}, blockperm::Vector{Int},
+        blockcomb::Vector{Int}
+    ) where {ElT, N, NC}
+    Rh = NDTensors.permutedims_combine_output(cpu(T), is, perm, combdims, blockperm, blockcomb)
+    R = tensor(BlockSparse(ItgVector(zeros(ElT, length(data(Rh)))), blockoffsets(Rh)), inds(Rh))
+    inds_perm = NDTensors.permute(inds(T), perm)
+    combdims_perm = NDTensors.TupleTools.sort(NDTensors._permute_combdims(combdims, perm))
+    cloc = minimum(combdims_perm)
+    ind_comb = NDTensors.permuteblocks(NDTensors.outer(NDTensors.getindices(inds_perm, combdims_perm)...), blockperm)
+    iperm = invperm(perm)
+    descs = Any[]
+    for (b, off) in pairs(blockoffsets(T))
+        bc = NDTensors.perm_block(NDTensors.combine_dims(NDTensors.permute(b, perm), inds_perm, combdims_perm), cloc, blockperm)
+        bin = bc[cloc]
+        nb = blockcomb[bin]
+        o, k = 0, 1
+        while bin - k > 0 && blockcomb[bin - k] == nb
+            o += blockdim(ind_comb, bin - k)
+            k += 1
+        end
+        bnew = NDTensors.setindex(bc, nb, cloc)
+        dT = NDTensors.blockdims(T, b)
+        dR = NDTensors.blockdims(R, bnew)
+        sR = colmajor_strides(dR)
+        # permuted T mode j sits at R mode j (before cloc), inside the
+        # combined range (combdims_perm), or at R mode j - NC + 1 (after)
+        dperm = NDTensors.permute(dT, perm)
+        dstr = zeros(Int64, N)
+        acc = 1
+        for j in 1:N
+            if j in combdims_perm
+                dstr[perm[j]] = acc
+                acc *= dperm[j]
+            else
+                dstr[perm[j]] = sR[j < cloc ? j : j - NC + 1]
+            end
+        end
+        push!(descs, (collect(Int64, dT), Int64(off), colmajor_strides(dT),
+                      Int64(NDTensors.offset(R, bnew) + o), dstr))
+    end
+    itg_copy!(data(R), data(T), descs)
+    return R
+end
+
 end # module

@@ -28,6 +28,18 @@ Pinning status (what checks each function; see tests/test_oracle.py):
                                         densify(A), densify(B)) on random
                                         structures
   rrule_contract                     -- central finite differences
+  combined_index /
+  combined_position_map              -- hand-derived known answer (mixed arrows,
+                                        uneven sectors: order, merge, sizes,
+                                        element positions) and the reference's
+                                        one-index merge (test_qncombiner.jl:4-13),
+                                        arrow keyword (test_qnitensor.jl:603-625)
+  combine_blocksparse /
+  uncombine_blocksparse              -- combine-then-uncombine identity over every
+                                        subset and order of order-3/4 tensors
+                                        (test_qnitensor.jl:666-784), nnz kept,
+                                        per-block flux kept, the two-QN-name
+                                        example (test_qnitensor.jl:786-828)
 """
 from __future__ import annotations
 
