@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the dominant "
                          "kernel's HBM traffic")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the other configs of the metric (dense Float64 chi=1024, "
+                         "block-sparse ComplexF64 chi=4096, dense ComplexF64 chi=4096)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="(internal) run step 1 in every mode a few times plus a 256 MiB copy "
                          "and exit; the workload of the counter passes")
@@ -237,6 +240,98 @@ def cpu_baseline(args, net_tensors, gpu_outputs):
     return dt, threads, errs, flops
 
 
+def _time_network(torch, seq, mode, steps, warmup):
+    seq.autotune(mode)
+    g = seq.capture(mode)
+    return time_steps(torch, g.replay, steps, warmup)
+
+
+def _dense_oracle_rows(net, rows, O, itgmod):
+    """The network's output restricted to rows `rows` of L's first index
+    (free through every step, and the output's first index), by the oracle:
+    every later step only restricts the same index."""
+    arrays = [t.array() for t in net]
+    cur, cur_inds = arrays[0][rows], net[0].inds
+    for B, A in zip(net[1:], arrays[1:]):
+        lA, lB = itgmod.compute_contraction_labels(cur_inds, B.inds)
+        lR = O.contract_labels(lA, lB)
+        cur = O.dense_contract(None, lR, cur, lA, A, lB)
+        cur_inds = tuple(cur_inds[lA.index(x)] if x in lA else B.inds[lB.index(x)] for x in lR)
+    return cur
+
+
+def secondary_configs(args, torch, dev):
+    """The metric's other configurations, each in FP64 MFMA and split
+    precision (graph replay): dense Float64 two-site network at chi = 1024,
+    the block-sparse U(1) network at chi = 4096 with ComplexF64 entries, and
+    the dense network at chi = 4096 with ComplexF64 entries. Errors are
+    relative Frobenius against the oracle: all of the block-sparse output
+    (per block and overall), a sample of 8 rows of L's free index for the
+    dense ones (exact for those rows)."""
+    import numpy as np
+    import itg
+    from itg.configs import heff_network, heff_flops
+    from itg.network import ContractionSequence
+    from oracle import ndtensors_oracle as O
+
+    out = {}
+    for name, chi, qns, dt, steps in (("heff_dense_chi1024_f64", 1024, False, np.float64, 10),
+                                      ("heff_u1_blocksparse_chi4096_c128", 4096, True, np.complex128, 10),
+                                      ("heff_dense_chi4096_c128", 4096, False, np.complex128, 2)):
+        log(args, f"[bench] secondary config {name}")
+        net = heff_network(chi, qns=qns, dtype=dt, seed=0, device=dev)[:5]
+        cplx = np.dtype(dt).kind == "c"
+        flops = float(sum(heff_flops(*net))) * (4 if cplx else 1)
+        seq = ContractionSequence(net)
+        r = {"gflop_per_apply": flops / 1e9, "eltype": "ComplexF64" if cplx else "Float64"}
+        outs = {}
+        for mode in ("fp64", "split_int8"):
+            ms = _time_network(torch, seq, mode, steps, 1)
+            outs[mode] = seq.result.tensor.data.detach().cpu().numpy().copy()
+            r[mode] = {"ms": ms, "gflops": flops / (ms * 1e-3) / 1e9}
+        r["speedup_split_vs_fp64"] = r["fp64"]["ms"] / r["split_int8"]["ms"]
+        if not args.no_cpu:
+            if qns:
+                ts = []
+                for T in net:
+                    host = T.tensor.data.detach().cpu().numpy()
+                    from collections import OrderedDict
+                    offs = OrderedDict((tuple(c - 1 for c in b), o) for b, o in T.tensor.blockoffsets.items())
+                    inds = tuple(O.Index([(O.QN(*[(n, v, m) for n, (v, m) in q.vals.items()]), d)
+                                          for q, d in i.space], tags=i.tags, plev=i.plev,
+                                         dir=i.dir, id=i.id) for i in T.inds)
+                    ts.append(O.BlockSparse(host, offs, inds))
+                cur = ts[0]
+                for B in ts[1:]:
+                    lA, lB = O.compute_contraction_labels(cur.inds, B.inds)
+                    cur, _p = O.blocksparse_contract(cur, lA, B, lB, O.contract_labels(lA, lB))
+                ref = cur.data
+                for mode, v in outs.items():
+                    worst = 0.0
+                    for b, o in cur.offsets.items():
+                        n = cur.block(b).size
+                        nb = np.linalg.norm(ref[o:o + n])
+                        if nb > 0:
+                            worst = max(worst, float(np.linalg.norm(v[o:o + n] - ref[o:o + n]) / nb))
+                    r[mode]["rel_err_vs_oracle"] = float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
+                    r[mode]["max_block_rel_err_vs_oracle"] = worst
+            else:
+                rows = np.random.default_rng(1).choice(chi, 8, replace=False)
+                ref = _dense_oracle_rows(net, rows, O, itg)
+                shape = tuple(i.dim for i in seq.result.inds)
+                for mode, v in outs.items():
+                    got = v.reshape(shape, order="F")[rows]
+                    r[mode]["rel_err_vs_oracle_rows"] = float(np.linalg.norm(got - ref) /
+                                                              np.linalg.norm(ref))
+                r["oracle_rows"] = [int(x) for x in rows]
+        out[name] = r
+        log(args, f"[bench]   {json.dumps(r)}")
+        del seq, net, outs
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -360,6 +455,14 @@ def main():
     }
     if head != "fp64":
         out["speedup_vs_fp64"] = results["fp64"]["ms"] / r["ms"]
+    if not args.no_secondary:
+        # release the headline network (tensors, plans, graphs) first
+        seq = net = run = g = L = psi = W1 = W2 = R = outputs = None
+        import gc
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["secondary_configs"] = secondary_configs(args, torch, dev)
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib * (1 << 20) // 8
         src = torch.randn(n, dtype=torch.float64, device=dev)

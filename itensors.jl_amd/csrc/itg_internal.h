@@ -83,11 +83,16 @@ inline void kernel_arg_ptrs(Tup& t, void** p, std::index_sequence<I...>) {
   ((p[I] = (void*)&std::get<I>(t)), ...);
 }
 
-// Every kernel of the library is launched through here.
+// Every kernel of the library is launched through here. The thread's
+// last-error state is cleared first: launchers report hipGetLastError()
+// after the launch, and an error left behind by an unrelated earlier HIP
+// call in the process (the host application's own) must not be reported as
+// this launch's.
 template <typename... KArgs, typename... Args>
 inline hipError_t launch_kernel(void (*kernel)(KArgs...), dim3 grid, dim3 block, unsigned shmem,
                                 hipStream_t s, Args... args) {
   static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  (void)hipGetLastError();
   std::tuple<KArgs...> vals(static_cast<KArgs>(args)...);
   void* ptrs[sizeof...(KArgs)];
   kernel_arg_ptrs(vals, ptrs, std::index_sequence_for<KArgs...>{});

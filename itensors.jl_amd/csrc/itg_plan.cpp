@@ -2,6 +2,7 @@
 #include "itg_plan.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 
@@ -1086,9 +1087,18 @@ int default_split_cfg(const ExecPlan& ep) {
 
 int mixed_big_cfg(int cplx) { return cplx ? 2 : 1; }
 
+// Key of a cached launch: the mixed choice's tile shapes depend on the
+// element type (a plan is shared by real and complex tensors of one block
+// structure), every other tile table does not.
+int gemm_launch_key(int cfg, int cplx) {
+  if (cfg == kCfgMixed) return kCfgMixed + (cplx ? 1 : 0);
+  return gemm_cfg_ok(cfg, cplx) ? cfg : 0;
+}
+
 hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
   if (cfg != kCfgMixed && !gemm_cfg_ok(cfg, cplx)) cfg = 0;
-  auto it = ep.launches.find(cfg);
+  const int key = gemm_launch_key(cfg, cplx);
+  auto it = ep.launches.find(key);
   if (it != ep.launches.end()) return hipSuccess;
   std::unique_ptr<GemmLaunch> gl(new GemmLaunch);
   gl->cfg = cfg;
@@ -1109,7 +1119,7 @@ hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
     if ((e = gl->tiles.upload(table.data(), table.size() * sizeof(int2), s)) != hipSuccess)
       return e;
   }
-  ep.launches[cfg] = std::move(gl);
+  ep.launches[key] = std::move(gl);
   return hipSuccess;
 }
 
@@ -1399,8 +1409,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     *err = std::string("gemm table upload: ") + hipGetErrorString(e);
     return e;
   }
-  int ecfg = (cfg == kCfgMixed || gemm_cfg_ok(cfg, cplx)) ? cfg : 0;
-  GemmLaunch& gl = *ep.launches[ecfg];
+  GemmLaunch& gl = *ep.launches[gemm_launch_key(cfg, cplx)];
   {
     TimeScope ts(kt, "gemm_f64_kernel", s);
     e = launch_gemm_f64(s, gl.cfg, cplx, ep.ta, ep.tb, (const GemmPair*)ep.g_pairs.p,
@@ -1414,7 +1423,12 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
                         C, gflags, a_re, a_im, gbr, gbi);
   }
   if (e != hipSuccess) {
-    *err = std::string("gemm launch: ") + hipGetErrorString(e);
+    char what[160];
+    std::snprintf(what, sizeof what,
+                  "gemm launch (tile cfg %d / %d, workgroups %lld / %lld, %s, ta %d, tb %d): ",
+                  gl.cfg, gl.cfg2, (long long)gl.grid, (long long)gl.grid2,
+                  cplx ? "complex" : "real", ep.ta, ep.tb);
+    *err = std::string(what) + hipGetErrorString(e);
     return e;
   }
   if (ep.permC) {

@@ -157,3 +157,26 @@ def test_heff_chi2048_structure_and_values(env):
         npairs.append(len(plan))
         cur = nxt
     assert npairs == [372, 940, 952, 372]
+
+
+def test_plan_shared_by_real_and_complex_after_autotune(env):
+    """Plans are cached by block structure, so a real and a complex tensor of
+    one structure share a plan. Autotuning the real contraction caches every
+    tile choice, including the mixed one (large tiles for the big output
+    blocks, 64 x 64 for the rest), whose shapes differ for complex data; the
+    complex autotune and run that follow must use their own."""
+    torch, itg, ND, O = env
+    QN, Index = itg.QN, itg.Index
+    i = Index([(QN(0), 160), (QN(1), 40)], "i")
+    j = Index([(QN(0), 150), (QN(1), 30)], "j")
+    k = Index([(QN(0), 140), (QN(1), 50)], "k")
+    for dtype in (float, complex):
+        A = itg.random_itensor(QN(0), i, j.dag(), dtype=dtype, rng=np.random.default_rng(40))
+        B = itg.random_itensor(QN(0), j, k.dag(), dtype=dtype, rng=np.random.default_rng(41))
+        lA, lB = itg.compute_contraction_labels(A.inds, B.inds)
+        lR = ND.contract_labels(lA, lB)
+        p = ND.bs_plan(A.tensor, lA, B.tensor, lB, lR)
+        R = torch.empty(p.nnzR, dtype=A.eltype, device="cuda")
+        p.autotune(R, A.tensor.data, B.tensor.data, prec="fp64")
+        C = A * B
+        check_against_oracle(env, A, B, C)

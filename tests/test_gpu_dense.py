@@ -204,3 +204,22 @@ def test_permutedims(env):
             R = ND.permutedims(T, [p + 1 for p in perm])
             torch.cuda.synchronize()
             assert np.array_equal(R.to_numpy(), np.transpose(x, perm)), (shape, perm)
+
+
+def test_stale_hip_error_from_the_host_is_not_reported(env):
+    """An error left in HIP's per-thread last-error state by an unrelated
+    host call (here hipSetDevice with an invalid ordinal, which touches no
+    device) must not make the next contraction report a launch failure."""
+    import ctypes
+    torch, itg, ND, O = env
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipSetDevice.restype = ctypes.c_int
+    hip.hipPeekAtLastError.restype = ctypes.c_int
+    assert hip.hipSetDevice(9999) != 0
+    assert hip.hipPeekAtLastError() != 0
+    rng = np.random.default_rng(30)
+    A, B = rng.standard_normal((64, 48)), rng.standard_normal((48, 80))
+    T = ND.contract(ND.DenseTensor.from_numpy(A), (1, -1), ND.DenseTensor.from_numpy(B), (-1, 2),
+                    (1, 2))
+    torch.cuda.synchronize()
+    assert relerr(T.to_numpy(), A @ B) < FP64_RTOL
