@@ -257,6 +257,8 @@ struct SliceBlock {
   int32_t R, K, Kpad;
   int32_t kparts;     // ceil(Kpad / kSliceKPart): workgroups along K
   int32_t nrm, nkm;   // number of row / K modes
+  int32_t side;       // operand: 0 = A, 1 = B (one launch slices both)
+  int32_t pad;
   int32_t rdim[kSliceModes], rstr[kSliceModes];
   int32_t kdim[kSliceModes], kstr[kSliceModes];
 };
@@ -271,13 +273,20 @@ struct SplitPair {
 int split_slices(int prec);  // slices per value for a precision mode (0: none)
 // Workgroups of one block: ceil(rows / kSliceRows) x kparts.
 int64_t slice_block_wgs(int32_t rows, int32_t kparts);
-// Two launches over the same grid: partial row maxima (as exponents, into
-// pmax), then the slices (planes) and the row exponents (exps).
-// Complex operands: nsets = 3 slice sets (re, im, -im); plane buffers hold
-// nsets x the per-set bytes.
+// The buffers of the two operands of a split GEMM; SliceBlock::side picks.
+struct SliceOperands {
+  const double* src[2];  // FP64 operand (complex: interleaved)
+  int8_t* planes[2];     // slice planes
+  int32_t* exps[2];      // row exponents
+  int32_t* pmax[2];      // partial row exponents
+};
+// Two launches over the same grid (the blocks of both operands, A's first):
+// partial row maxima (as exponents, into pmax), then the slices (planes) and
+// the row exponents (exps). Complex operands: nsets = 3 slice sets (re, im,
+// -im); plane buffers hold nsets x the per-set bytes.
 hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
-                        const int64_t* prefix, int64_t nblk, int64_t grid, const void* src,
-                        void* planes, void* exps, void* pmax);
+                        const int64_t* prefix, int64_t nblk, int64_t grid,
+                        const SliceOperands& ops);
 // Tile shape of the split GEMM: 64 x 64 (wmb = 1, 256 threads, tile table of
 // gemm cfg 0) or 128 x 64 (wmb = 2, 512 threads, tile table of gemm cfg 2).
 // Complex: one launch per output component comp (0 = re, 1 = im); `sets`

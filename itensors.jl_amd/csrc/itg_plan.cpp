@@ -1051,17 +1051,22 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
     if (ep->h_pairs[p].alpha_im != 0.0) ep->cplx_pair_alpha = true;
     ep->split_kmax = std::max<int64_t>(ep->split_kmax, dA[bp].K);
   }
+  // one slicing grid over both operands: A's blocks, then B's (side 1)
+  std::vector<SliceBlock> dAB(dA);
+  std::vector<int64_t> pAB(pA);
+  for (size_t b = 0; b < dB.size(); ++b) {
+    SliceBlock d = dB[b];
+    d.side = 1;
+    dAB.push_back(d);
+    pAB.push_back(pA.back() + pB[b + 1]);
+  }
   hipError_t e;
-  if ((e = ep->sA_desc.upload(dA.data(), dA.size() * sizeof(SliceBlock), s)) != hipSuccess ||
-      (e = ep->sA_prefix.upload(pA.data(), pA.size() * sizeof(int64_t), s)) != hipSuccess ||
-      (e = ep->sB_desc.upload(dB.data(), dB.size() * sizeof(SliceBlock), s)) != hipSuccess ||
-      (e = ep->sB_prefix.upload(pB.data(), pB.size() * sizeof(int64_t), s)) != hipSuccess ||
+  if ((e = ep->s_desc.upload(dAB.data(), dAB.size() * sizeof(SliceBlock), s)) != hipSuccess ||
+      (e = ep->s_prefix.upload(pAB.data(), pAB.size() * sizeof(int64_t), s)) != hipSuccess ||
       (e = ep->s_pairs.upload(sp.data(), sp.size() * sizeof(SplitPair), s)) != hipSuccess)
     return std::string("device table upload failed: ") + hipGetErrorString(e);
-  ep->sA_blocks = P.nblk();
-  ep->sB_blocks = Q.nblk();
-  ep->sA_grid = pA.back();
-  ep->sB_grid = pB.back();
+  ep->s_blocks = (int64_t)dAB.size();
+  ep->s_grid = pAB.back();
   ep->split_ok = true;
   return "";
 }
@@ -1265,16 +1270,18 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
   w += align256((size_t)ep.sB_pmax * 4);
   hipError_t e;
   {
+    SliceOperands ops;
+    ops.src[0] = (const double*)A;
+    ops.src[1] = (const double*)B;
+    ops.planes[0] = (int8_t*)planesA;
+    ops.planes[1] = (int8_t*)planesB;
+    ops.exps[0] = (int32_t*)expA;
+    ops.exps[1] = (int32_t*)expB;
+    ops.pmax[0] = (int32_t*)pmaxA;
+    ops.pmax[1] = (int32_t*)pmaxB;
     TimeScope ts(kt, "slice_max_kernel+slice_kernel", s);
-    e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.sA_desc.p,
-                     (const int64_t*)ep.sA_prefix.p, ep.sA_blocks, ep.sA_grid, A, planesA, expA,
-                     pmaxA);
-  }
-  if (e == hipSuccess) {
-    TimeScope ts(kt, "slice_max_kernel+slice_kernel", s);
-    e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.sB_desc.p,
-                     (const int64_t*)ep.sB_prefix.p, ep.sB_blocks, ep.sB_grid, B, planesB, expB,
-                     pmaxB);
+    e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.s_desc.p,
+                     (const int64_t*)ep.s_prefix.p, ep.s_blocks, ep.s_grid, ops);
   }
   if (e != hipSuccess) {
     *err = std::string("slice launch: ") + hipGetErrorString(e);

@@ -145,10 +145,10 @@ struct ExecPlan {
   bool cplx_pair_alpha = false;  // some per-pair alpha has an imaginary part
   int64_t split_kmax = 0;        // largest contracted extent of a pair
   std::string split_why;         // reason when !split_ok
-  int64_t sA_blocks = 0, sA_grid = 0, sB_blocks = 0, sB_grid = 0;
+  int64_t s_blocks = 0, s_grid = 0;  // slicing grid over both operands
   int64_t sA_plane_bytes = 0, sB_plane_bytes = 0, sA_rows = 0, sB_rows = 0;
   int64_t sA_pmax = 0, sB_pmax = 0;  // partial row exponents (ints)
-  DevBuf sA_desc, sA_prefix, sB_desc, sB_prefix, s_pairs;
+  DevBuf s_desc, s_prefix, s_pairs;
 };
 
 // Build the execution plan of `pairs` (grouped: all pairs of one output block

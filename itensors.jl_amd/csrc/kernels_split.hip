@@ -156,13 +156,14 @@ __device__ inline void load_row_offsets(const SliceBlock& d, int r0, int tid, in
 template <int CPLX>
 __global__ void __launch_bounds__(kThreads)
     slice_max_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
-                     int64_t nblk, const double* __restrict__ src, int32_t* __restrict__ pmax) {
+                     int64_t nblk, SliceOperands ops) {
   __shared__ double red[4][kSliceRows];
   __shared__ int64_t srow[kSliceRows];
   const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
   const SliceBlock& d = blocks[t.b];
   const int tid = threadIdx.x;
-  const double* x = src + (CPLX ? 2 : 1) * d.off;
+  const double* x = ops.src[d.side] + (CPLX ? 2 : 1) * d.off;
+  int32_t* pmax = ops.pmax[d.side];
   const int k_lo = t.kp * kSliceKPart, k_hi = min(d.K, k_lo + kSliceKPart);
   const bool kfast = d.nkm == 1 && d.kstr[0] == 1;
   // Every load below is unconditional, from an offset clamped into the block
@@ -246,15 +247,17 @@ __global__ void __launch_bounds__(kThreads)
 template <int S, int CPLX>
 __global__ void __launch_bounds__(kThreads)
     slice_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
-                 int64_t nblk, const double* __restrict__ src, int8_t* __restrict__ planes,
-                 int32_t* __restrict__ exps, const int32_t* __restrict__ pmax, int nsets) {
+                 int64_t nblk, SliceOperands ops, int nsets) {
   __shared__ int32_t sexp[kSliceRows];
   __shared__ int64_t srow[kSliceRows];
   const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
   const SliceBlock& d = blocks[t.b];
   const int r0 = t.r0;
   const int tid = threadIdx.x;
-  const double* x = src + (CPLX ? 2 : 1) * d.off;
+  const double* x = ops.src[d.side] + (CPLX ? 2 : 1) * d.off;
+  int8_t* planes = ops.planes[d.side];
+  int32_t* exps = ops.exps[d.side];
+  const int32_t* pmax = ops.pmax[d.side];
   load_row_offsets(d, r0, tid, srow);
   if (tid < kSliceRows) {
     int e = kZeroRow;
@@ -743,27 +746,21 @@ __global__ void __launch_bounds__(128, 1)
 int split_slices(int prec) { return prec == 2 ? kSliceCount : 0; }
 
 hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
-                        const int64_t* prefix, int64_t nblk, int64_t grid, const void* src,
-                        void* planes, void* exps, void* pmax) {
+                        const int64_t* prefix, int64_t nblk, int64_t grid,
+                        const SliceOperands& ops) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
   const dim3 g((unsigned)grid), blk(kThreads);
   if (cplx) {
-    ITG_LAUNCH((slice_max_kernel<1>), g, blk, 0, s, blocks, prefix, nblk,
-                       (const double*)src, (int32_t*)pmax);
+    ITG_LAUNCH((slice_max_kernel<1>), g, blk, 0, s, blocks, prefix, nblk, ops);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    ITG_LAUNCH((slice_kernel<kSliceCount, 1>), g, blk, 0, s, blocks, prefix, nblk,
-                       (const double*)src, (int8_t*)planes, (int32_t*)exps,
-                       (const int32_t*)pmax, nsets);
+    ITG_LAUNCH((slice_kernel<kSliceCount, 1>), g, blk, 0, s, blocks, prefix, nblk, ops, nsets);
   } else {
-    ITG_LAUNCH((slice_max_kernel<0>), g, blk, 0, s, blocks, prefix, nblk,
-                       (const double*)src, (int32_t*)pmax);
+    ITG_LAUNCH((slice_max_kernel<0>), g, blk, 0, s, blocks, prefix, nblk, ops);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    ITG_LAUNCH((slice_kernel<kSliceCount, 0>), g, blk, 0, s, blocks, prefix, nblk,
-                       (const double*)src, (int8_t*)planes, (int32_t*)exps,
-                       (const int32_t*)pmax, 1);
+    ITG_LAUNCH((slice_kernel<kSliceCount, 0>), g, blk, 0, s, blocks, prefix, nblk, ops, 1);
   }
   return hipGetLastError();
 }
