@@ -165,19 +165,24 @@ def time_steps(torch, fn, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def time_one_step_kernel(torch, seq, i, prec, reps):
-    """Average duration of contraction step i alone (one grouped launch for the
-    GEMM steps without permutes), HIP events on the launching stream."""
+def time_calls(torch, fn, reps):
+    """Average duration of fn's launches, HIP events on the launching stream."""
     s = torch.cuda.current_stream()
-    seq.run_step(i, prec)
+    fn()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     for a, b in ev:
         a.record(s)
-        seq.run_step(i, prec)
+        fn()
         b.record(s)
     torch.cuda.synchronize()
     return sum(a.elapsed_time(b) for a, b in ev) / reps
+
+
+def time_one_step_kernel(torch, seq, i, prec, reps):
+    """Average duration of contraction step i alone (unfused; one grouped
+    launch for the GEMM steps without permutes)."""
+    return time_calls(torch, lambda: seq.run_step(i, prec), reps)
 
 
 def kernel_times(torch, fn, reps):
@@ -387,7 +392,13 @@ def main():
         results[m] = {"ms": ms, "gflops": flops / (ms * 1e-3) / 1e9, "step_ms": per_step,
                       "kernels_per_apply": {k: {"launches": n, "ms_per_launch": t}
                                             for k, (n, t) in sorted(kt.items())}}
-        log(args, f"[bench] {m}: {ms:.3f} ms/step, {results[m]['gflops']:.1f} GFLOP/s, steps {per_step}")
+        for i0 in sorted(seq.chains):
+            # steps i0+1 and i0+2 (1-based) run as one fused launch in run()
+            results[m].setdefault("fused_steps", []).append(
+                {"steps": [i0 + 1, i0 + 2],
+                 "ms": time_calls(torch, lambda i0=i0: seq.run_chain(i0), max(5, args.steps // 2))})
+        log(args, f"[bench] {m}: {ms:.3f} ms/step, {results[m]['gflops']:.1f} GFLOP/s, steps {per_step}, "
+                  f"fused {results[m].get('fused_steps')}")
 
     head = modes[-1] if args.prec == "auto" else (args.prec if args.prec in results else "fp64")
     # the headline mode must meet the accuracy bar; otherwise fall back to FP64

@@ -212,6 +212,30 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype,
 
 itg_status itg_bs_plan_destroy(itg_bs_plan* p);
 
+/* Two consecutive block-sparse contractions fused into one pass over HBM:
+ * Z = (X * B1) * B2 with p1 the plan of X * B1 (output Y) and p2 the plan
+ * of Y * B2 (Y its first operand), without storing Y. Available when both
+ * steps take the fiber path with few distinct operand values per fiber and
+ * the open modes of Y that only one step iterates over have unit sector
+ * sizes (the two MPO tensors of a two-site effective Hamiltonian applied
+ * one after the other); otherwise create returns ITG_ERR_UNSUPPORTED and
+ * the caller runs the two plans. Z holds p2's output structure. Results
+ * are bit-identical to itg_bs_execute(p1) then itg_bs_execute(p2) with
+ * alpha = 1, no per-pair factors and no conjugation, in every precision
+ * mode (both steps compute in FP64). The chain holds its own tables; it
+ * stays valid after p1 and p2 are destroyed.
+ * Replaces two successive contract! calls on block-sparse tensors
+ * (NDTensors/src/blocksparse/contract.jl:57-76) made by the left-to-right
+ * fold of contract(::Vector{ITensor}) (src/tensor_operations/
+ * tensor_algebra.jl:121-130, each step the A * B of tensor_algebra.jl:
+ * 62-77), for the consecutive pairs that admit it. */
+typedef struct itg_bs_chain itg_bs_chain;
+itg_status itg_bs_chain_create(itg_handle* h, const itg_bs_plan* p1, const itg_bs_plan* p2,
+                               itg_bs_chain** out);
+itg_status itg_bs_chain_execute(itg_handle* h, itg_bs_chain* c, itg_dtype dtype, void* Z,
+                                const void* X, const void* B1, const void* B2);
+itg_status itg_bs_chain_destroy(itg_bs_chain* c);
+
 /* Block-sparse permutedims: dst block b (same order and offsets as the caller
  * passes in offsR) = permutedims(src block b, perm). dims of each block are
  * N x nblk (block b at dims[b*N..]). Replaces permutedims!(R::BlockSparseTensor,

@@ -49,6 +49,11 @@ struct itg_bs_plan {
   std::vector<double> ep_alpha_key;
 };
 
+struct itg_bs_chain {
+  int device = 0;
+  ChainPlan cp;
+};
+
 struct itg_batch_plan {
   int device = 0;
   itg_dtype dtype = ITG_F64;
@@ -633,6 +638,53 @@ itg_status itg_bs_plan_destroy(itg_bs_plan* p) {
   DeviceGuard g(p->device);
   (void)hipDeviceSynchronize();
   delete p;
+  return ITG_OK;
+}
+
+itg_status itg_bs_chain_create(itg_handle* h, const itg_bs_plan* p1, const itg_bs_plan* p2,
+                               itg_bs_chain** out) {
+  if (!h || !p1 || !p2 || !out) return ITG_ERR_INVALID;
+  *out = nullptr;
+  if (p1->device != h->device || p2->device != h->device)
+    return fail(h, ITG_ERR_INVALID, "plan built on another device");
+  const BlockList &y1 = p1->bR, &y2 = p2->b1;
+  if (y1.N != y2.N || y1.offs != y2.offs || y1.dims != y2.dims)
+    return fail(h, ITG_ERR_INVALID,
+                "the first plan's output structure is not the second plan's first operand");
+  if (p1->pairs.empty() || p2->pairs.empty())
+    return fail(h, ITG_ERR_UNSUPPORTED, "empty plan: no fused form");
+  DeviceGuard g(h->device);
+  std::unique_ptr<itg_bs_chain> c(new itg_bs_chain);
+  c->device = h->device;
+  const std::string err = build_fiber_chain(*p1->ep, *p2->ep, h->stream, &c->cp);
+  if (err == "-")
+    return fail(h, ITG_ERR_UNSUPPORTED, "the two contractions have no fused form");
+  if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "chain table upload");
+  *out = c.release();
+  return ITG_OK;
+}
+
+itg_status itg_bs_chain_execute(itg_handle* h, itg_bs_chain* c, itg_dtype dtype, void* Z,
+                                const void* X, const void* B1, const void* B2) {
+  if (!h || !c) return ITG_ERR_INVALID;
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (!Z || !X || !B1 || !B2) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  if (c->device != h->device) return fail(h, ITG_ERR_INVALID, "chain built on another device");
+  DeviceGuard g(h->device);
+  std::string err;
+  hipError_t e = exec_chain(c->cp, dtype == ITG_C128, h->stream, X, B1, B2, Z, &err,
+                            active_timer(h));
+  if (e != hipSuccess) return fail(h, ITG_ERR_HIP, err);
+  return ITG_OK;
+}
+
+itg_status itg_bs_chain_destroy(itg_bs_chain* c) {
+  if (!c) return ITG_OK;
+  DeviceGuard g(c->device);
+  (void)hipDeviceSynchronize();
+  delete c;
   return ITG_OK;
 }
 

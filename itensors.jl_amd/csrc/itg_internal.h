@@ -213,6 +213,34 @@ hipError_t launch_fiber_mv(hipStream_t s, int cplx, int nf, int smax, const Fibe
                            const FiberCoef* coefs, const int32_t* wg_group, int64_t total_wgs,
                            const void* A, const void* B, void* C, unsigned flags, double ga_re,
                            double ga_im, double beta_re, double beta_im);
+// Two consecutive matrix-form fiber contractions fused, Y = X*B1 then
+// Z = Y*B2, without storing Y. A super-fiber is one value of the modes that
+// are fiber modes of both steps (the other fiber modes of either step have
+// unit sector sizes, so a super-fiber is one fiber of each step). Per
+// super-group: x = the X value of every step-1 slot (step-1 groups in
+// order), y = every Y value a step-2 slot reads (in step-2 slot order), z =
+// every step-2 output; y_v = sum_u x_u W1[v][u] over linked u in order,
+// z_q = sum_v y_v W2[q][v] over linked v in order: the same operations in
+// the same order as the two separate passes, so Z is bit-identical. Slot
+// and output strides are given over the shared modes (fdim order).
+constexpr int kChainX = 16, kChainY = 16, kChainZ = 32;
+constexpr int kFibersPerWGChain = 256 * 8;
+struct ChainGroup {
+  int64_t nfib, wg0;
+  int32_t fdim[kMaxModes];     // extents of the shared modes (1 past nf)
+  int32_t nx, ny, nz;
+  int32_t x0, y0, z0;          // FiberSlot (x) / ymask (y) / FiberOutElem (z) ranges
+  int32_t w10, nw1, w20, nw2;  // coefs1 (slot = x, out = y) / coefs2 (slot = y, out = z)
+  int32_t pad;
+};
+// xm, ym: unroll bounds (8 or 16). B1 / B2: the small operands of the steps.
+hipError_t launch_fiber_chain(hipStream_t s, int cplx, int nf, int xm, int ym,
+                              const ChainGroup* groups, const FiberSlot* slots,
+                              const FiberOutElem* zouts, const uint32_t* ymask,
+                              const FiberCoef* coefs1, const FiberCoef* coefs2,
+                              const int32_t* wg_group, int64_t total_wgs, const void* X,
+                              const void* B1, const void* B2, void* Z);
+
 // nf: number of A-free modes (1..kMaxModes). flags bit 0 / 1: conj(A) / conj(B).
 hipError_t launch_fiber(hipStream_t s, int cplx, int nf, const FiberGroup* groups,
                         const FiberOut* outs, const FiberPair* pairs, const int32_t* koffs,

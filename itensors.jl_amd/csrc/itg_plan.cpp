@@ -2,6 +2,8 @@
 #include "itg_plan.h"
 
 #include <algorithm>
+#include <map>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -374,6 +376,11 @@ std::string build_fiber_mv(const Structure& st, const BlockList& TA, const Block
   ep->f_wgs = wgs;
   ep->f_mv = true;
   ep->f_smax = smax <= 4 ? 4 : (smax <= 8 ? 8 : kFiberSlots);
+  ep->hfm_groups = groups;
+  ep->hfm_slots = slots;
+  ep->hfm_outs = outs;
+  ep->hfm_coefs = coefs;
+  ep->f_fm = fm;
   hipError_t e;
   if ((e = ep->fm_groups.upload(groups.data(), groups.size() * sizeof(FiberMVGroup), s)) !=
           hipSuccess ||
@@ -1525,6 +1532,217 @@ KernelTimer::~KernelTimer() {
     pool.push_back(p.b);
   }
   for (hipEvent_t ev : pool) (void)hipEventDestroy(ev);
+}
+
+// Fused pair of matrix-form fiber steps (ChainGroup). Returns "" when
+// built, "-" when the pair has no fused form, or an error message.
+std::string build_fiber_chain(const ExecPlan& e1, const ExecPlan& e2, hipStream_t s,
+                              ChainPlan* cp) {
+  if (e1.strategy != ExecPlan::FIBER || e2.strategy != ExecPlan::FIBER || !e1.f_mv ||
+      !e2.f_mv || e1.f_a_is_2 || e2.f_a_is_2)
+    return "-";
+  const int nf1 = (int)e1.f_fm.size(), nf2 = (int)e2.f_fm.size();
+  // Y modes step 1 iterates over as fibers (its output modes) and those
+  // step 2 iterates over (its A modes); Y is step 1's output and step 2's A
+  std::vector<int> p1(kMaxModes * 2, -1), p2(kMaxModes * 2, -1);
+  for (int i = 0; i < nf1; ++i) p1[(size_t)e1.f_fm[(size_t)i].second] = i;
+  for (int i = 0; i < nf2; ++i) p2[(size_t)e2.f_fm[(size_t)i].first] = i;
+  std::vector<int> i1, i2;  // fiber positions of the shared modes in each step
+  for (int i = 0; i < nf1; ++i) {
+    const int y = e1.f_fm[(size_t)i].second;
+    if (p2[(size_t)y] >= 0) {
+      i1.push_back(i);
+      i2.push_back(p2[(size_t)y]);
+    }
+  }
+  const int ns = (int)i1.size();
+  if (ns > 4) return "-";
+  // the modes only one step iterates over must have unit extents
+  for (const FiberMVGroup& g : e1.hfm_groups)
+    for (int i = 0; i < nf1; ++i)
+      if (p2[(size_t)e1.f_fm[(size_t)i].second] < 0 && g.fdim[i] != 1) return "-";
+  for (const FiberMVGroup& g : e2.hfm_groups)
+    for (int i = 0; i < nf2; ++i)
+      if (p1[(size_t)e2.f_fm[(size_t)i].first] < 0 && g.fdim[i] != 1) return "-";
+
+  // Y element (offset at fiber 0) -> (step-1 group, its output)
+  const size_t n1 = e1.hfm_groups.size(), n2 = e2.hfm_groups.size();
+  std::unordered_map<int64_t, std::pair<int32_t, int32_t>> prod;
+  for (size_t g = 0; g < n1; ++g) {
+    const FiberMVGroup& G = e1.hfm_groups[g];
+    for (int o = 0; o < G.nout; ++o) {
+      const FiberOutElem& oe = e1.hfm_outs[(size_t)(G.o0 + o)];
+      if (!prod.emplace(oe.off_c + oe.go, std::make_pair((int32_t)g, (int32_t)o)).second)
+        return "-";  // written twice
+    }
+  }
+  std::vector<size_t> par(n1 + n2);
+  std::iota(par.begin(), par.end(), 0);
+  auto find = [&](size_t x) {
+    while (par[x] != x) x = par[x] = par[par[x]];
+    return x;
+  };
+  for (size_t h = 0; h < n2; ++h) {
+    const FiberMVGroup& H = e2.hfm_groups[h];
+    for (int t = 0; t < H.nslot; ++t) {
+      const FiberSlot& sl = e2.hfm_slots[(size_t)(H.s0 + t)];
+      auto it = prod.find(sl.off_a + sl.ka);
+      if (it == prod.end()) return "-";  // a Y value step 1 does not write
+      const FiberMVGroup& G = e1.hfm_groups[(size_t)it->second.first];
+      const FiberOutElem& oe = e1.hfm_outs[(size_t)(G.o0 + it->second.second)];
+      for (int j = 0; j < ns; ++j)  // the same Y element at every super-fiber
+        if (oe.cstr[i1[(size_t)j]] != sl.astr[i2[(size_t)j]] ||
+            G.fdim[i1[(size_t)j]] != H.fdim[i2[(size_t)j]])
+          return "-";
+      par[find((size_t)it->second.first)] = find(n1 + h);
+    }
+  }
+  std::map<size_t, std::pair<std::vector<size_t>, std::vector<size_t>>> comps;
+  for (size_t g = 0; g < n1; ++g) comps[find(g)].first.push_back(g);
+  for (size_t h = 0; h < n2; ++h) comps[find(n1 + h)].second.push_back(h);
+
+  std::vector<ChainGroup> groups;
+  std::vector<FiberSlot> slots;
+  std::vector<FiberOutElem> zouts;
+  std::vector<uint32_t> ymask;
+  std::vector<FiberCoef> coefs1, coefs2;
+  std::vector<int32_t> wgg;
+  int64_t wgs = 0;
+  int xmax = 1, ymax = 1;
+  for (auto& kv : comps) {
+    const std::vector<size_t>& G1 = kv.second.first;
+    const std::vector<size_t>& G2 = kv.second.second;
+    if (G2.empty()) continue;  // Y values nothing downstream reads
+    if (G1.empty()) return "-";
+    ChainGroup C;
+    std::memset(&C, 0, sizeof(C));
+    const FiberMVGroup& g0 = e1.hfm_groups[G1[0]];
+    C.nfib = g0.nfib;
+    for (int i = 0; i < kMaxModes; ++i) C.fdim[i] = 1;
+    for (int j = 0; j < ns; ++j) C.fdim[j] = g0.fdim[i1[(size_t)j]];
+    for (size_t g : G1)
+      for (int j = 0; j < ns; ++j)
+        if (e1.hfm_groups[g].fdim[i1[(size_t)j]] != C.fdim[j]) return "-";
+    // x: every step-1 slot, groups in order, strides over the shared modes
+    C.x0 = (int32_t)slots.size();
+    std::map<size_t, int32_t> xbase;
+    for (size_t g : G1) {
+      const FiberMVGroup& G = e1.hfm_groups[g];
+      xbase[g] = (int32_t)(slots.size() - (size_t)C.x0);
+      for (int u = 0; u < G.nslot; ++u) {
+        const FiberSlot& s0 = e1.hfm_slots[(size_t)(G.s0 + u)];
+        FiberSlot sl;
+        std::memset(&sl, 0, sizeof(sl));
+        sl.off_a = s0.off_a;
+        sl.ka = s0.ka;
+        for (int j = 0; j < ns; ++j) sl.astr[j] = s0.astr[i1[(size_t)j]];
+        slots.push_back(sl);
+      }
+    }
+    C.nx = (int32_t)(slots.size() - (size_t)C.x0);
+    if (C.nx == 0) return "-";
+    // y: every step-2 slot, groups in order, and the step-1 output writing it
+    C.y0 = (int32_t)ymask.size();
+    std::map<size_t, int32_t> ybase;
+    std::map<std::pair<int32_t, int32_t>, int32_t> yof;
+    for (size_t h : G2) {
+      const FiberMVGroup& H = e2.hfm_groups[h];
+      ybase[h] = (int32_t)(ymask.size() - (size_t)C.y0);
+      for (int t = 0; t < H.nslot; ++t) {
+        const FiberSlot& sl = e2.hfm_slots[(size_t)(H.s0 + t)];
+        if (!yof.emplace(prod.at(sl.off_a + sl.ka), (int32_t)(ymask.size() - (size_t)C.y0))
+                 .second)
+          return "-";  // one Y value in two step-2 slots
+        ymask.push_back(0u);
+      }
+    }
+    C.ny = (int32_t)(ymask.size() - (size_t)C.y0);
+    if (C.nx > kChainX || C.ny > kChainY) return "-";
+    // W1: the step-1 coefficients of the Y values read downstream
+    C.w10 = (int32_t)coefs1.size();
+    for (size_t g : G1) {
+      const FiberMVGroup& G = e1.hfm_groups[g];
+      for (int i = 0; i < G.nw; ++i) {
+        FiberCoef cf = e1.hfm_coefs[(size_t)(G.w0 + i)];
+        auto it = yof.find({(int32_t)g, cf.out});
+        if (it == yof.end()) continue;
+        cf.slot = xbase[g] + cf.slot;
+        cf.out = it->second;
+        ymask[(size_t)(C.y0 + cf.out)] |= 1u << cf.slot;
+        coefs1.push_back(cf);
+      }
+    }
+    C.nw1 = (int32_t)(coefs1.size() - (size_t)C.w10);
+    // z: every step-2 output (masks over y, strides over the shared modes); W2
+    C.z0 = (int32_t)zouts.size();
+    C.w20 = (int32_t)coefs2.size();
+    for (size_t h : G2) {
+      const FiberMVGroup& H = e2.hfm_groups[h];
+      const int32_t zb = (int32_t)(zouts.size() - (size_t)C.z0);
+      for (int o = 0; o < H.nout; ++o) {
+        const FiberOutElem& o0 = e2.hfm_outs[(size_t)(H.o0 + o)];
+        FiberOutElem oe;
+        std::memset(&oe, 0, sizeof(oe));
+        oe.off_c = o0.off_c;
+        oe.go = o0.go;
+        for (int j = 0; j < ns; ++j) oe.cstr[j] = o0.cstr[i2[(size_t)j]];
+        for (int t = 0; t < H.nslot; ++t)
+          if ((o0.mask >> t) & 1u) oe.mask |= 1u << (ybase[h] + t);
+        zouts.push_back(oe);
+      }
+      for (int i = 0; i < H.nw; ++i) {
+        FiberCoef cf = e2.hfm_coefs[(size_t)(H.w0 + i)];
+        cf.slot = ybase[h] + cf.slot;
+        cf.out = zb + cf.out;
+        coefs2.push_back(cf);
+      }
+    }
+    C.nz = (int32_t)(zouts.size() - (size_t)C.z0);
+    C.nw2 = (int32_t)(coefs2.size() - (size_t)C.w20);
+    if (C.nz == 0 || C.nz > kChainZ || C.ny == 0) return "-";
+    xmax = std::max(xmax, (int)C.nx);
+    ymax = std::max(ymax, (int)C.ny);
+    C.wg0 = wgs;
+    const int64_t n = (C.nfib + kFibersPerWGChain - 1) / kFibersPerWGChain;
+    for (int64_t w = 0; w < n; ++w) wgg.push_back((int32_t)groups.size());
+    wgs += n;
+    groups.push_back(C);
+  }
+  if (groups.empty()) return "-";
+  if (coefs1.empty()) coefs1.push_back(FiberCoef{});
+  if (coefs2.empty()) coefs2.push_back(FiberCoef{});
+  cp->nf = std::max(ns, 1);
+  cp->xm = xmax <= 8 ? 8 : kChainX;
+  cp->ym = ymax <= 8 ? 8 : kChainY;
+  cp->wgs = wgs;
+  hipError_t e;
+  if ((e = cp->groups.upload(groups.data(), groups.size() * sizeof(ChainGroup), s)) != hipSuccess ||
+      (e = cp->slots.upload(slots.data(), slots.size() * sizeof(FiberSlot), s)) != hipSuccess ||
+      (e = cp->zouts.upload(zouts.data(), zouts.size() * sizeof(FiberOutElem), s)) !=
+          hipSuccess ||
+      (e = cp->ymask.upload(ymask.data(), ymask.size() * sizeof(uint32_t), s)) != hipSuccess ||
+      (e = cp->coefs1.upload(coefs1.data(), coefs1.size() * sizeof(FiberCoef), s)) !=
+          hipSuccess ||
+      (e = cp->coefs2.upload(coefs2.data(), coefs2.size() * sizeof(FiberCoef), s)) !=
+          hipSuccess ||
+      (e = cp->wgg.upload(wgg.data(), wgg.size() * sizeof(int32_t), s)) != hipSuccess)
+    return std::string("device table upload failed: ") + hipGetErrorString(e);
+  return "";
+}
+
+hipError_t exec_chain(const ChainPlan& cp, int cplx, hipStream_t s, const void* X, const void* B1,
+                      const void* B2, void* Z, std::string* err, KernelTimer* kt) {
+  hipError_t e;
+  {
+    TimeScope ts(kt, "fiber_chain_kernel", s);
+    e = launch_fiber_chain(s, cplx, cp.nf, cp.xm, cp.ym, (const ChainGroup*)cp.groups.p,
+                           (const FiberSlot*)cp.slots.p, (const FiberOutElem*)cp.zouts.p,
+                           (const uint32_t*)cp.ymask.p, (const FiberCoef*)cp.coefs1.p,
+                           (const FiberCoef*)cp.coefs2.p, (const int32_t*)cp.wgg.p, cp.wgs, X, B1,
+                           B2, Z);
+  }
+  if (e != hipSuccess) *err = std::string("fiber chain kernel launch: ") + hipGetErrorString(e);
+  return e;
 }
 
 }  // namespace itg

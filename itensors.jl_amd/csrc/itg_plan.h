@@ -127,6 +127,13 @@ struct ExecPlan {
   int64_t f_wgs = 0;
   DevBuf f_groups, f_outs, f_pairs, f_koffs, f_gofs_c, f_gofs_b, f_wgg;
   DevBuf fm_groups, fm_slots, fm_outs, fm_coefs;
+  // host copies of the matrix-form tables and the fiber modes (A mode, R
+  // mode), for fusing two consecutive steps (build_fiber_chain)
+  std::vector<FiberMVGroup> hfm_groups;
+  std::vector<FiberSlot> hfm_slots;
+  std::vector<FiberOutElem> hfm_outs;
+  std::vector<FiberCoef> hfm_coefs;
+  std::vector<std::pair<int, int>> f_fm;
   // GEMM
   bool swap = false, permA = false, permB = false, permC = false;
   int ta = 0, tb = 0;
@@ -185,5 +192,21 @@ hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s);
 // XCD-balanced workgroup -> {group, tile} table for tile shape tc.
 // filter: 0 = every group, 1 = only groups with M, N >= the tile, 2 = the rest.
 std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc, int filter = 0);
+
+// Two consecutive matrix-form fiber contractions run as one pass (see
+// ChainGroup). Built from the two steps' plans; the device tables live here.
+struct ChainPlan {
+  int nf = 0;            // modes of the fiber index (max of both steps)
+  int xm = 0, ym = 0;    // unroll bounds (8 or 16) >= every group's nx / ny
+  int64_t wgs = 0;
+  DevBuf groups, slots, zouts, ymask, coefs1, coefs2, wgg;
+};
+// "" on success, "-" when the two steps cannot be fused (the caller runs
+// them one after the other), otherwise an error message.
+std::string build_fiber_chain(const ExecPlan& e1, const ExecPlan& e2, hipStream_t s,
+                              ChainPlan* cp);
+// Z = (X * B1) * B2 through a ChainPlan (unit scale, beta 0, no conjugation).
+hipError_t exec_chain(const ChainPlan& cp, int cplx, hipStream_t s, const void* X, const void* B1,
+                      const void* B2, void* Z, std::string* err, KernelTimer* kt = nullptr);
 
 }  // namespace itg

@@ -364,6 +364,235 @@ hipError_t launch_mv_sm(hipStream_t s, int nf, int smax, dim3 grid, const FiberM
   }
 }
 
+// Fused pair of matrix-form steps (ChainGroup): per super-fiber, the x
+// values are loaded once, y = W1 x is formed in registers and z = W2 y is
+// stored; Y never reaches memory. W1 and W2 of the super-group sit in LDS.
+// The arithmetic per y (per z) value is fiber_mv_kernel's, operation for
+// operation, with unit scale, zero beta and no conjugation.
+template <int CPLX, int NF, int XM, int YM>
+__global__ void __launch_bounds__(kThreads)
+    fiber_chain_kernel(const ChainGroup* __restrict__ groups, const FiberSlot* __restrict__ slots,
+                       const FiberOutElem* __restrict__ zouts,
+                       const uint32_t* __restrict__ ymask, const FiberCoef* __restrict__ coefs1,
+                       const FiberCoef* __restrict__ coefs2,
+                       const int32_t* __restrict__ wg_group, const double* __restrict__ X,
+                       const double* __restrict__ B1, const double* __restrict__ B2,
+                       double* __restrict__ Z) {
+  __shared__ double W1re[kChainY * kChainX];
+  __shared__ double W1im[CPLX ? kChainY * kChainX : 1];
+  __shared__ double W2re[kChainZ * kChainY];
+  __shared__ double W2im[CPLX ? kChainZ * kChainY : 1];
+  __shared__ int64_t x_off[kChainX], z_off[kChainZ];
+  __shared__ int32_t x_str[kChainX][NF], z_str[kChainZ][NF];
+  __shared__ uint32_t y_mask[kChainY], z_mask[kChainZ];
+  const int64_t wg = blockIdx.x;
+  const ChainGroup& cg = groups[wg_group[wg]];
+  const int tid = threadIdx.x;
+  const int nx = cg.nx, ny = cg.ny, nz = cg.nz;
+
+  for (int i = tid; i < kChainY * kChainX; i += kThreads) {
+    W1re[i] = 0.0;
+    if (CPLX) W1im[i] = 0.0;
+  }
+  for (int i = tid; i < kChainZ * kChainY; i += kThreads) {
+    W2re[i] = 0.0;
+    if (CPLX) W2im[i] = 0.0;
+  }
+  if (tid < kChainX) {
+    const FiberSlot& sl = slots[cg.x0 + (tid < nx ? tid : 0)];
+    x_off[tid] = sl.off_a + sl.ka;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) x_str[tid][i] = sl.astr[i];
+  } else if (tid >= 64 && tid < 64 + kChainZ) {
+    const int o = tid - 64;
+    const FiberOutElem& oe = zouts[cg.z0 + (o < nz ? o : 0)];
+    z_off[o] = oe.off_c + oe.go;
+    z_mask[o] = o < nz ? oe.mask : 0u;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) z_str[o][i] = oe.cstr[i];
+  } else if (tid >= 128 && tid < 128 + kChainY) {
+    const int v = tid - 128;
+    y_mask[v] = v < ny ? ymask[cg.y0 + v] : 0u;
+  }
+  __syncthreads();
+  auto put = [&](const FiberCoef& cf, const double* Bp, double* Wr, double* Wi, int w) {
+    // the scale factor is one: the products below are fiber_mv_kernel's
+    const double ga_re = 1.0, ga_im = 0.0;
+    const double al_re = ga_re * cf.alpha_re - ga_im * cf.alpha_im;
+    const double al_im = ga_re * cf.alpha_im + ga_im * cf.alpha_re;
+    if (CPLX) {
+      const double2 b = reinterpret_cast<const double2*>(Bp)[cf.off_b];
+      Wr[w] = al_re * b.x - al_im * b.y;
+      Wi[w] = al_re * b.y + al_im * b.x;
+    } else {
+      Wr[w] = al_re * Bp[cf.off_b];
+    }
+  };
+  for (int i = tid; i < cg.nw1; i += kThreads) {
+    const FiberCoef cf = coefs1[cg.w10 + i];
+    put(cf, B1, W1re, W1im, cf.out * kChainX + cf.slot);
+  }
+  for (int i = tid; i < cg.nw2; i += kThreads) {
+    const FiberCoef cf = coefs2[cg.w20 + i];
+    put(cf, B2, W2re, W2im, cf.out * kChainY + cf.slot);
+  }
+  __syncthreads();
+
+  const int64_t fbase = (wg - cg.wg0) * kFibersPerWGChain;
+  const int64_t flast = cg.nfib - 1;
+  const int64_t left = cg.nfib - fbase;
+  constexpr int kRounds = kFibersPerWGChain / kThreads;
+  const int nrounds = (int)(left >= kFibersPerWGChain ? kRounds : (left + kThreads - 1) / kThreads);
+
+  auto coords = [&](int64_t f, uint32_t (&c)[NF]) {
+    uint32_t rem = (uint32_t)(f < flast ? f : flast);
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      if (i + 1 < NF) {
+        const uint32_t q = rem / (uint32_t)cg.fdim[i];
+        c[i] = rem - q * (uint32_t)cg.fdim[i];
+        rem = q;
+      } else {
+        c[i] = rem;
+      }
+    }
+  };
+  auto load = [&](const uint32_t (&c)[NF], double (&xr)[XM], double (&xi)[CPLX ? XM : 1]) {
+#pragma unroll
+    for (int u = 0; u < XM; ++u) {
+      int64_t ao = x_off[u];
+#pragma unroll
+      for (int i = 0; i < NF; ++i) ao += (int64_t)c[i] * x_str[u][i];
+      if (CPLX) {
+        const double2 x = reinterpret_cast<const double2*>(X)[ao];
+        xr[u] = x.x;
+        xi[CPLX ? u : 0] = x.y;
+      } else {
+        xr[u] = X[ao];
+      }
+    }
+  };
+
+  uint32_t c[NF];
+  double xre[XM], xim[CPLX ? XM : 1];
+  coords(fbase + tid, c);
+  load(c, xre, xim);
+  for (int r = 0; r < nrounds; ++r) {
+    asm volatile("" ::: "memory");
+    const int64_t f = fbase + (int64_t)r * kThreads + tid;
+    uint32_t cn[NF];
+    double xnr[XM], xni[CPLX ? XM : 1];
+    coords(f + kThreads, cn);
+    load(cn, xnr, xni);
+    // y = W1 x (masked, in slot order)
+    double yre[YM], yim[CPLX ? YM : 1];
+#pragma unroll
+    for (int v = 0; v < YM; ++v) {
+      const uint32_t m = y_mask[v];
+      double acc_re = 0.0, acc_im = 0.0;
+#pragma unroll
+      for (int u = 0; u < XM; ++u) {
+        const bool lk = (m >> u) & 1u;
+        const double wr = W1re[v * kChainX + u];
+        if (CPLX) {
+          const double wi = W1im[v * kChainX + u];
+          const double tr = fma(xre[u], wr, fma(-xim[CPLX ? u : 0], wi, acc_re));
+          const double ti = fma(xre[u], wi, fma(xim[CPLX ? u : 0], wr, acc_im));
+          acc_re = lk ? tr : acc_re;
+          acc_im = lk ? ti : acc_im;
+        } else {
+          const double t = fma(xre[u], wr, acc_re);
+          acc_re = lk ? t : acc_re;
+        }
+      }
+      yre[v] = acc_re;
+      if (CPLX) yim[CPLX ? v : 0] = acc_im;
+    }
+    // z = W2 y (masked, in slot order)
+    const bool live = f < cg.nfib;
+    for (int q = 0; q < nz; ++q) {
+      const uint32_t m = z_mask[q];
+      double acc_re = 0.0, acc_im = 0.0;
+#pragma unroll
+      for (int v = 0; v < YM; ++v) {
+        const bool lk = (m >> v) & 1u;
+        const double wr = W2re[q * kChainY + v];
+        if (CPLX) {
+          const double wi = W2im[q * kChainY + v];
+          const double tr = fma(yre[v], wr, fma(-yim[CPLX ? v : 0], wi, acc_re));
+          const double ti = fma(yre[v], wi, fma(yim[CPLX ? v : 0], wr, acc_im));
+          acc_re = lk ? tr : acc_re;
+          acc_im = lk ? ti : acc_im;
+        } else {
+          const double t = fma(yre[v], wr, acc_re);
+          acc_re = lk ? t : acc_re;
+        }
+      }
+      if (live) {
+        int64_t co = z_off[q];
+#pragma unroll
+        for (int i = 0; i < NF; ++i) co += (int64_t)c[i] * z_str[q][i];
+        if (CPLX)
+          reinterpret_cast<double2*>(Z)[co] = make_double2(acc_re, acc_im);
+        else
+          Z[co] = acc_re;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) c[i] = cn[i];
+#pragma unroll
+    for (int u = 0; u < XM; ++u) {
+      xre[u] = xnr[u];
+      if (CPLX) xim[CPLX ? u : 0] = xni[CPLX ? u : 0];
+    }
+  }
+}
+
+template <int CPLX, int XM, int YM>
+hipError_t launch_chain(hipStream_t s, int nf, dim3 grid, const ChainGroup* groups,
+                        const FiberSlot* slots, const FiberOutElem* zouts,
+                        const uint32_t* ymask, const FiberCoef* coefs1, const FiberCoef* coefs2,
+                        const int32_t* wg_group, const void* X, const void* B1, const void* B2,
+                        void* Z) {
+#define ITG_NF_CASE(n)                                                                        \
+  case n:                                                                                     \
+    ITG_LAUNCH((fiber_chain_kernel<CPLX, n, XM, YM>), grid, dim3(kThreads), 0, s, groups,    \
+               slots, zouts, ymask, coefs1, coefs2, wg_group, (const double*)X,               \
+               (const double*)B1, (const double*)B2, (double*)Z);                             \
+    return hipGetLastError();
+  switch (nf) {
+    ITG_NF_CASE(1)
+    ITG_NF_CASE(2)
+    ITG_NF_CASE(3)
+    ITG_NF_CASE(4)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef ITG_NF_CASE
+}
+
+template <int CPLX>
+hipError_t launch_chain_xy(hipStream_t s, int nf, int xm, int ym, dim3 grid,
+                           const ChainGroup* groups, const FiberSlot* slots,
+                           const FiberOutElem* zouts, const uint32_t* ymask,
+                           const FiberCoef* coefs1, const FiberCoef* coefs2,
+                           const int32_t* wg_group, const void* X, const void* B1,
+                           const void* B2, void* Z) {
+  if (xm == 8 && ym == 8)
+    return launch_chain<CPLX, 8, 8>(s, nf, grid, groups, slots, zouts, ymask, coefs1, coefs2,
+                                    wg_group, X, B1, B2, Z);
+  if (xm == kChainX && ym == kChainY)
+    return launch_chain<CPLX, kChainX, kChainY>(s, nf, grid, groups, slots, zouts, ymask, coefs1,
+                                                coefs2, wg_group, X, B1, B2, Z);
+  if (xm == 8 && ym == kChainY)
+    return launch_chain<CPLX, 8, kChainY>(s, nf, grid, groups, slots, zouts, ymask, coefs1,
+                                          coefs2, wg_group, X, B1, B2, Z);
+  if (xm == kChainX && ym == 8)
+    return launch_chain<CPLX, kChainX, 8>(s, nf, grid, groups, slots, zouts, ymask, coefs1,
+                                          coefs2, wg_group, X, B1, B2, Z);
+  return hipErrorInvalidValue;
+}
+
 template <int CPLX>
 hipError_t launch_c(hipStream_t s, int nf, dim3 grid, const FiberGroup* groups,
                     const FiberOut* outs, const FiberPair* pairs, const int32_t* koffs,
@@ -406,6 +635,21 @@ hipError_t launch_fiber_mv(hipStream_t s, int cplx, int nf, int smax,
                            flags, ga_re, ga_im, beta_re, beta_im);
   return launch_mv_sm<0>(s, nf, smax, grid, groups, slots, outs, coefs, wg_group, A, B, C, flags,
                          ga_re, ga_im, beta_re, beta_im);
+}
+
+hipError_t launch_fiber_chain(hipStream_t s, int cplx, int nf, int xm, int ym,
+                              const ChainGroup* groups, const FiberSlot* slots,
+                              const FiberOutElem* zouts, const uint32_t* ymask,
+                              const FiberCoef* coefs1, const FiberCoef* coefs2,
+                              const int32_t* wg_group, int64_t total_wgs, const void* X,
+                              const void* B1, const void* B2, void* Z) {
+  if (total_wgs == 0) return hipSuccess;
+  dim3 grid((unsigned)total_wgs);
+  if (cplx)
+    return launch_chain_xy<1>(s, nf, xm, ym, grid, groups, slots, zouts, ymask, coefs1, coefs2,
+                              wg_group, X, B1, B2, Z);
+  return launch_chain_xy<0>(s, nf, xm, ym, grid, groups, slots, zouts, ymask, coefs1, coefs2,
+                            wg_group, X, B1, B2, Z);
 }
 
 hipError_t launch_fiber(hipStream_t s, int cplx, int nf, const FiberGroup* groups,

@@ -39,7 +39,7 @@ EXPORTS = (
     "itg_batch_plan_create", "itg_batch_execute", "itg_batch_plan_destroy",
     "itg_malloc", "itg_free", "itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod",
     "itg_timer_enable", "itg_timer_count", "itg_timer_read", "itg_set_split_tile",
-    "itg_bs_copy",
+    "itg_bs_copy", "itg_bs_chain_create", "itg_bs_chain_execute", "itg_bs_chain_destroy",
 )
 
 
@@ -99,6 +99,9 @@ def lib():
         L.itg_bs_permute.argtypes = [P, I, I, I64, I64P, I64P, I64P, I32P, P, P]
         L.itg_bs_autotune.argtypes = [P, P, I, P, P, P, I]
         L.itg_bs_copy.argtypes = [P, I, I64, I, I64P, I64P, I64P, I64P, I64P, P, I64, P, I64]
+        L.itg_bs_chain_create.argtypes = [P, P, P, ctypes.POINTER(P)]
+        L.itg_bs_chain_execute.argtypes = [P, P, I, P, P, P, P]
+        L.itg_bs_chain_destroy.argtypes = [P]
         PP = ctypes.POINTER(P)
         L.itg_batch_plan_create.argtypes = [P, I, I64, I, I64P, I64P, PP, I, I64P, I64P, PP,
                                             I, I64P, I64P, PP, ctypes.POINTER(P)]

@@ -353,6 +353,34 @@ class BSPlan:
             pass
 
 
+class BSChain:
+    """Two consecutive block-sparse contractions in one pass: Z = (X*B1)*B2
+    from the plans p1 (X*B1 -> Y) and p2 (Y*B2 -> Z), Y never stored.
+    Raises ItgUnsupported when the pair has no fused form. Z is
+    bit-identical to p1.execute then p2.execute with alpha = 1 (FP64 in
+    every precision mode)."""
+
+    def __init__(self, p1, p2):
+        self.h = p1.h
+        chain = ctypes.c_void_p()
+        self.h.check(_lib.lib().itg_bs_chain_create(self.h.h, p1.plan, p2.plan,
+                                                    ctypes.byref(chain)))
+        self.chain = chain
+
+    def execute(self, Z_data, X_data, B1_data, B2_data):
+        h = _handle(Z_data.device)
+        h.check(_lib.lib().itg_bs_chain_execute(h.h, self.chain, _dtype_code(Z_data.dtype),
+                                                _ptr(Z_data), _ptr(X_data), _ptr(B1_data),
+                                                _ptr(B2_data)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "chain", None) and _lib._lib is not None:
+                _lib._lib.itg_bs_chain_destroy(self.chain)
+        except Exception:
+            pass
+
+
 _plan_cache: "OrderedDict[tuple, BSPlan]" = OrderedDict()
 _PLAN_CACHE_MAX = 64
 

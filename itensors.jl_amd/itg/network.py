@@ -14,16 +14,23 @@ from __future__ import annotations
 import math
 
 from . import ndtensors as ND
+from ._lib import ItgUnsupported
 from .itensors import ITensor, compute_contraction_labels
 
 
 class ContractionSequence:
-    def __init__(self, tensors, *, prec=None):
+    """fuse: run consecutive block-sparse steps that admit it (ND.BSChain:
+    two fiber-path contractions whose intermediate is only read by the
+    next) as one kernel that never stores the intermediate. The result is
+    bit-identical either way; the skipped intermediates are left unwritten."""
+
+    def __init__(self, tensors, *, prec=None, fuse=True):
         import torch
 
         self.tensors = list(tensors)
         self.prec = prec
         self.steps = []
+        self.chains = {}  # first step index -> BSChain over it and the next
         cur = self.tensors[0]
         for B in self.tensors[1:]:
             lA, lB = compute_contraction_labels(cur.inds, B.inds)
@@ -44,6 +51,23 @@ class ContractionSequence:
                 self.steps.append(("dense", None, cur.tensor, B.tensor, R, (lA, lB, lR)))
             cur = ITensor(R, indsR)
         self.result = cur
+        i = 0
+        while fuse and i + 1 < len(self.steps):
+            s1, s2 = self.steps[i], self.steps[i + 1]
+            if s1[0] == "bs" and s2[0] == "bs" and s1[1].npairs and s2[1].npairs:
+                try:
+                    self.chains[i] = ND.BSChain(s1[1], s2[1])
+                    i += 2
+                    continue
+                except ItgUnsupported:
+                    pass
+            i += 1
+
+    def run_chain(self, i):
+        """Steps i and i + 1 as one fused launch."""
+        _k, _p, X, B1, _Y, _l = self.steps[i]
+        _k, _p, _Y2, B2, Z, _l = self.steps[i + 1]
+        self.chains[i].execute(Z.data, X.data, B1.data, B2.data)
 
     def run_step(self, i, prec=None):
         kind, p, A, B, R, (lA, lB, lR) = self.steps[i]
@@ -55,8 +79,14 @@ class ContractionSequence:
             ND.contract_(R, lR, A, lA, B, lB, 1.0, 0.0, prec=prec)
 
     def run(self, prec=None):
-        for i in range(len(self.steps)):
-            self.run_step(i, prec)
+        i = 0
+        while i < len(self.steps):
+            if i in self.chains:
+                self.run_chain(i)
+                i += 2
+            else:
+                self.run_step(i, prec)
+                i += 1
         return self.result
 
     def autotune(self, prec=None):
