@@ -6,6 +6,8 @@ Layers (host side mirrors the reference's interface, device side is libitg.so):
   itg.ndtensors   NDTensors.contract / contract! / permutedims on device storage
   itg.chainrules  rrule(contract) with a device adjoint
   itg.configs     effective-Hamiltonian workloads (dense and U(1) block-sparse)
+  itg.network     fixed contraction sequences (fused steps, graph capture)
+  itg.hdf5        HDF5 read / write in the ITensors.jl layout (host, libhdf5)
   itg._lib        the ctypes image of the C ABI (include/itg.h)
 """
 from ._lib import LIB_PATH, ItgError, ItgUnsupported, lib  # noqa: F401
@@ -16,5 +18,7 @@ from .itensors import (  # noqa: F401
     random_itensor, blocksparse_itensor, combiner, combinedind, combineblocks, iscombiner,
 )
 from .ndtensors import precision, set_precision  # noqa: F401
+from .itensors import tagstring  # noqa: F401
+from . import hdf5  # noqa: F401
 
 __version__ = "0.1.0"

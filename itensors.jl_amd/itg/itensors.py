@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import itertools
 import math
+import struct
 import warnings
 from collections import OrderedDict
 
@@ -103,6 +104,28 @@ class QN:
         return "QN(" + ", ".join(f"({n!r}, {v})" for n, (v, m) in self.vals.items()) + ")"
 
 
+# ------------------------------------------------------------------ TagSet
+def _tag_key(t):
+    """Order of two tags: each stored as 16 UInt16 characters and compared as
+    the big-endian integer of their little-endian bytes
+    (src/lib/SmallStrings/src/smallstring.jl:37-45,81-85,103), i.e. bytewise
+    over (low byte, high byte) of each character, shorter tags first."""
+    return b"".join(struct.pack("<H", ord(c)) for c in t).ljust(32, b"\0")
+
+
+def tagstring(tags):
+    """Canonical string of a tag set: whitespace dropped, tags cut to 16
+    characters, empty and repeated tags dropped, at most four kept (in the
+    order given), sorted (src/lib/TagSets/src/TagSets.jl:81-95,131-180,
+    317-327)."""
+    out = []
+    for t in str(tags).replace(" ", "").split(","):
+        t = t[:16]
+        if t and t not in out and len(out) < 4:
+            out.append(t)
+    return ",".join(sorted(out, key=_tag_key))
+
+
 # ------------------------------------------------------------------- Index
 _next_id = itertools.count(1)
 
@@ -119,7 +142,7 @@ class Index:
             self.space = int(space)
         else:
             self.space = tuple((q if isinstance(q, QN) else QN(q), int(d)) for q, d in space)
-        self.tags = tags
+        self.tags = tagstring(tags)
         self.plev = int(plev)
         self.id = next(_next_id) if id is None else id
         if dir is None:
