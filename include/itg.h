@@ -212,6 +212,36 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype,
 
 itg_status itg_bs_plan_destroy(itg_bs_plan* p);
 
+/* Block-sparse tensor T1 times a block-diagonal tensor D (DiagBlockSparse:
+ * every stored block has equal block coordinates (k, ..., k) and holds only
+ * its diagonal, min(block dims) elements at offsD[b]). Same arguments as
+ * itg_bs_plan_create with D in the place of T2; the output structure
+ * (itg_bs_plan_sizes / _blocks / _pairs) is the one the reference's
+ * contraction_output gives. ITG_ERR_INVALID if a block of D is off the
+ * block diagonal (the reference raises an error there too).
+ * Replaces contraction_output + contract! for (BlockSparse, DiagBlockSparse)
+ * in either order (NDTensors/src/blocksparse/diagblocksparse.jl:596-675),
+ * reached from ITensor * ITensor with a delta / diag_itensor operand
+ * (src/qn/qnitensor.jl:484-543). */
+itg_status itg_bs_diag_plan_create(itg_handle* h,
+                                   int N1, int64_t nblk1, const int64_t* blocks1,
+                                   const int64_t* offs1, const int64_t* nsec1,
+                                   const int64_t* secdims1, const int64_t* labels1,
+                                   int ND, int64_t nblkD, const int64_t* blocksD,
+                                   const int64_t* offsD, const int64_t* nsecD,
+                                   const int64_t* secdimsD, const int64_t* labelsD,
+                                   int NR, const int64_t* labelsR, itg_bs_plan** out);
+/* R = alpha * (T1 * D) on every output block of a diagonal plan (blocks
+ * overwritten; pairs of one output block summed in plan order). uniform:
+ * NULL, or {re, im} of the one value on every diagonal element (delta,
+ * UniformDiagBlockSparse; D is then unused). Per pair this is the dense
+ * contract! of the T1 block with the diagonal block
+ * (NDTensors/src/diag/tensoralgebra/contract.jl:103-160), without forming
+ * the dense block. */
+itg_status itg_bs_diag_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* R,
+                               const void* T1, const void* D, const double* uniform,
+                               const double* alpha);
+
 /* Two consecutive block-sparse contractions fused into one pass over HBM:
  * Z = (X * B1) * B2 with p1 the plan of X * B1 (output Y) and p2 the plan
  * of Y * B2 (Y its first operand), without storing Y. Available when both

@@ -47,6 +47,12 @@ struct itg_bs_plan {
   std::unique_ptr<ExecPlan> ep;      // pair_alpha = NULL
   std::unique_ptr<ExecPlan> ep_alpha;  // last per-pair alpha variant
   std::vector<double> ep_alpha_key;
+  // T2 block diagonal (itg_bs_diag_plan_create): device tables of
+  // diag_contract_kernel instead of ep
+  bool diag = false;
+  int d_nr = 1;
+  int64_t d_wgs = 0;
+  DevBuf d_outs, d_pairs, d_wgg;
 };
 
 struct itg_bs_chain {
@@ -433,15 +439,21 @@ itg_status itg_contract_dense(itg_handle* h, itg_dtype dtype, int NR, const int6
 }
 
 // ----------------------------------------------------------- block sparse
-itg_status itg_bs_plan_create(itg_handle* h, int N1, int64_t nblk1, const int64_t* blocks1,
+}  // extern "C"
+
+namespace {
+// Block lists, output structure and the sequential pair list of T1 x T2
+// (contract_blockoffsets, NDTensors/src/blocksparse/contract_sequential.jl:1-41);
+// shared by the plain and the diagonal plans. `diag2`: T2 is block diagonal
+// (every block (k, ..., k)) and each block stores only its diagonal, min of
+// its dims elements.
+itg_status build_bs_structure(itg_handle* h, int N1, int64_t nblk1, const int64_t* blocks1,
                               const int64_t* offs1, const int64_t* nsec1,
                               const int64_t* secdims1, const int64_t* labels1, int N2,
                               int64_t nblk2, const int64_t* blocks2, const int64_t* offs2,
                               const int64_t* nsec2, const int64_t* secdims2,
                               const int64_t* labels2, int NR, const int64_t* labelsR,
-                              itg_bs_plan** out) {
-  if (!h || !out) return ITG_ERR_INVALID;
-  *out = nullptr;
+                              bool diag2, std::unique_ptr<itg_bs_plan>& p) {
   if (nblk1 < 0 || nblk2 < 0) return fail(h, ITG_ERR_INVALID, "negative block count");
   if ((nblk1 > 0 && (!blocks1 || !offs1)) || (nblk2 > 0 && (!blocks2 || !offs2)))
     return fail(h, ITG_ERR_INVALID, "null block arrays");
@@ -471,32 +483,35 @@ itg_status itg_bs_plan_create(itg_handle* h, int N1, int64_t nblk1, const int64_
   for (int i = 0; i < st.nk; ++i)
     if (sec1[st.k1[i]] != sec2[st.k2[i]])
       return fail(h, ITG_ERR_INVALID, "contracted indices have different block structure");
-  std::unique_ptr<itg_bs_plan> p(new itg_bs_plan);
+  p.reset(new itg_bs_plan);
   p->device = h->device;
   p->NR = NR;
   p->st = st;
   auto fill = [&](BlockList& bl, int N, int64_t nblk, const int64_t* blocks, const int64_t* offs,
-                  const std::vector<std::vector<int64_t>>& sec) -> std::string {
+                  const std::vector<std::vector<int64_t>>& sec, bool diag) -> std::string {
     bl.N = N;
     bl.dims.resize((size_t)std::max<int64_t>(nblk, 0) * std::max(N, 1), 1);
     bl.offs.assign(offs, offs + nblk);
     int64_t end = 0;
     for (int64_t b = 0; b < nblk; ++b) {
-      int64_t sz = 1;
+      int64_t sz = 1, dl = INT64_MAX;
       for (int i = 0; i < N; ++i) {
         const int64_t c = blocks[b * N + i];
         if (c < 1 || c > (int64_t)sec[i].size()) return "block coordinate out of range";
         bl.dims[b * std::max(N, 1) + i] = (int32_t)sec[i][c - 1];
         sz *= sec[i][c - 1];
+        dl = std::min(dl, sec[i][c - 1]);
+        if (diag && c != blocks[b * N]) return "the diagonal tensor must be block diagonal";
       }
       if (sz > INT32_MAX) return "block with more than 2^31-1 elements";
-      end = std::max(end, offs[b] + sz);
+      if (offs[b] < 0) return "negative block offset";
+      end = std::max(end, offs[b] + (diag ? (N > 0 ? dl : 1) : sz));
     }
     bl.nnz = end;
     return "";
   };
-  std::string err = fill(p->b1, N1, nblk1, blocks1, offs1, sec1);
-  if (err.empty()) err = fill(p->b2, N2, nblk2, blocks2, offs2, sec2);
+  std::string err = fill(p->b1, N1, nblk1, blocks1, offs1, sec1, false);
+  if (err.empty()) err = fill(p->b2, N2, nblk2, blocks2, offs2, sec2, diag2);
   if (!err.empty()) return fail(h, ITG_ERR_INVALID, err);
 
   // sequential pair enumeration (contract_sequential.jl:26-39), with T2
@@ -549,13 +564,145 @@ itg_status itg_bs_plan_create(itg_handle* h, int N1, int64_t nblk1, const int64_
                    [](const PairSpec& a, const PairSpec& b) { return a.blkR < b.blkR; });
   if (N1 == 0) p->b1.dims.assign((size_t)nblk1, 1);
   if (N2 == 0) p->b2.dims.assign((size_t)nblk2, 1);
+  return ITG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+itg_status itg_bs_plan_create(itg_handle* h, int N1, int64_t nblk1, const int64_t* blocks1,
+                              const int64_t* offs1, const int64_t* nsec1,
+                              const int64_t* secdims1, const int64_t* labels1, int N2,
+                              int64_t nblk2, const int64_t* blocks2, const int64_t* offs2,
+                              const int64_t* nsec2, const int64_t* secdims2,
+                              const int64_t* labels2, int NR, const int64_t* labelsR,
+                              itg_bs_plan** out) {
+  if (!h || !out) return ITG_ERR_INVALID;
+  *out = nullptr;
+  std::unique_ptr<itg_bs_plan> p;
+  itg_status st = build_bs_structure(h, N1, nblk1, blocks1, offs1, nsec1, secdims1, labels1, N2,
+                                     nblk2, blocks2, offs2, nsec2, secdims2, labels2, NR,
+                                     labelsR, false, p);
+  if (st != ITG_OK) return st;
   if (!p->pairs.empty()) {
     DeviceGuard g(h->device);
     p->ep.reset(new ExecPlan);
-    err = build_exec_plan(st, p->b1, p->b2, p->bR, p->pairs, nullptr, h->stream, p->ep.get());
+    std::string err =
+        build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, nullptr, h->stream, p->ep.get());
     if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
   }
   *out = p.release();
+  return ITG_OK;
+}
+
+itg_status itg_bs_diag_plan_create(itg_handle* h, int N1, int64_t nblk1,
+                                   const int64_t* blocks1, const int64_t* offs1,
+                                   const int64_t* nsec1, const int64_t* secdims1,
+                                   const int64_t* labels1, int ND, int64_t nblkD,
+                                   const int64_t* blocksD, const int64_t* offsD,
+                                   const int64_t* nsecD, const int64_t* secdimsD,
+                                   const int64_t* labelsD, int NR, const int64_t* labelsR,
+                                   itg_bs_plan** out) {
+  if (!h || !out) return ITG_ERR_INVALID;
+  *out = nullptr;
+  std::unique_ptr<itg_bs_plan> p;
+  itg_status st = build_bs_structure(h, N1, nblk1, blocks1, offs1, nsec1, secdims1, labels1, ND,
+                                     nblkD, blocksD, offsD, nsecD, secdimsD, labelsD, NR,
+                                     labelsR, true, p);
+  if (st != ITG_OK) return st;
+  p->diag = true;
+  p->d_nr = std::max(NR, 1);
+  std::vector<DiagOut> outs;
+  std::vector<DiagPair> dps;
+  std::vector<int32_t> wgg;
+  int64_t wgs = 0;
+  const Structure& S = p->st;
+  for (size_t q = 0; q < p->pairs.size(); ++q) {
+    const PairSpec& ps = p->pairs[q];
+    if (outs.empty() || q == 0 || p->pairs[q - 1].blkR != ps.blkR) {
+      DiagOut o;
+      std::memset(&o, 0, sizeof(o));
+      o.offR = p->bR.offs[(size_t)ps.blkR];
+      o.nelem = 1;
+      for (int i = 0; i < kMaxModes; ++i) o.dims[i] = 1;
+      for (int r = 0; r < NR; ++r) {
+        o.dims[r] = p->bR.d(ps.blkR)[r];
+        o.nelem *= o.dims[r];
+      }
+      o.p0 = o.p1 = (int32_t)q;
+      o.wg0 = wgs;
+      const int64_t n = (o.nelem + kDiagChunk - 1) / kDiagChunk;
+      for (int64_t w = 0; w < n; ++w) wgg.push_back((int32_t)outs.size());
+      wgs += n;
+      outs.push_back(o);
+    }
+    outs.back().p1 = (int32_t)q + 1;
+    int64_t s1[kMaxModes];
+    {
+      int64_t a = 1;  // column-major strides of the T1 block
+      for (int i = 0; i < p->b1.N; ++i) {
+        s1[i] = a;
+        a *= p->b1.d(ps.blk1)[i];
+      }
+    }
+    DiagPair d;
+    std::memset(&d, 0, sizeof(d));
+    d.off1 = p->b1.offs[(size_t)ps.blk1];
+    d.offD = p->b2.offs[(size_t)ps.blk2];
+    d.dlen = ND > 0 ? INT32_MAX : 1;
+    for (int i = 0; i < ND; ++i) d.dlen = std::min(d.dlen, p->b2.d(ps.blk2)[i]);
+    for (int i = 0; i < S.nk; ++i) d.kstr += s1[S.k1[i]];
+    d.u0 = -1;
+    for (int r = 0; r < NR; ++r) {
+      if (S.r_src[r] == 1) {
+        d.str1[r] = (int32_t)s1[S.r_pos[r]];
+      } else if (d.u0 < 0) {
+        d.u0 = r;
+      } else {
+        d.umask |= 1u << r;
+      }
+    }
+    dps.push_back(d);
+  }
+  if (!outs.empty()) {
+    DeviceGuard g(h->device);
+    hipError_t e;
+    if ((e = p->d_outs.upload(outs.data(), outs.size() * sizeof(DiagOut), h->stream)) !=
+            hipSuccess ||
+        (e = p->d_pairs.upload(dps.data(), dps.size() * sizeof(DiagPair), h->stream)) !=
+            hipSuccess ||
+        (e = p->d_wgg.upload(wgg.data(), wgg.size() * sizeof(int32_t), h->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+      return hip_fail(h, e, "diagonal plan table upload");
+  }
+  p->d_wgs = wgs;
+  *out = p.release();
+  return ITG_OK;
+}
+
+itg_status itg_bs_diag_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* R,
+                               const void* T1, const void* D, const double* uniform,
+                               const double* alpha) {
+  if (!h || !p) return ITG_ERR_INVALID;
+  if (!p->diag) return fail(h, ITG_ERR_INVALID, "not a diagonal plan (use itg_bs_execute)");
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (p->pairs.empty()) return ITG_OK;
+  if (!R || !T1 || (!D && !uniform)) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
+  const int cplx = dtype == ITG_C128;
+  DeviceGuard g(h->device);
+  const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
+  const double u_re = uniform ? uniform[0] : 0.0, u_im = (uniform && cplx) ? uniform[1] : 0.0;
+  KernelTimer* kt = active_timer(h);
+  if (kt) kt->begin("diag_contract_kernel", h->stream);
+  hipError_t e = launch_diag(h->stream, cplx, p->d_nr, (const DiagOut*)p->d_outs.p,
+                             (const DiagPair*)p->d_pairs.p, (const int32_t*)p->d_wgg.p, p->d_wgs,
+                             T1, uniform ? T1 : D, uniform ? 1 : 0, u_re, u_im, a_re, a_im, R);
+  if (kt) kt->end(h->stream);
+  if (e != hipSuccess)
+    return fail(h, ITG_ERR_HIP, std::string("diagonal kernel launch: ") + hipGetErrorString(e));
   return ITG_OK;
 }
 
@@ -599,6 +746,7 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* 
   if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
     return fail(h, ITG_ERR_INVALID, "unknown precision mode");
+  if (p->diag) return fail(h, ITG_ERR_INVALID, "diagonal plan (use itg_bs_diag_execute)");
   if (p->pairs.empty()) return ITG_OK;
   if (!R || !T1 || !T2) return fail(h, ITG_ERR_INVALID, "null data pointer");
   if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
@@ -651,8 +799,8 @@ itg_status itg_bs_chain_create(itg_handle* h, const itg_bs_plan* p1, const itg_b
   if (y1.N != y2.N || y1.offs != y2.offs || y1.dims != y2.dims)
     return fail(h, ITG_ERR_INVALID,
                 "the first plan's output structure is not the second plan's first operand");
-  if (p1->pairs.empty() || p2->pairs.empty())
-    return fail(h, ITG_ERR_UNSUPPORTED, "empty plan: no fused form");
+  if (p1->pairs.empty() || p2->pairs.empty() || !p1->ep || !p2->ep)
+    return fail(h, ITG_ERR_UNSUPPORTED, "empty or diagonal plan: no fused form");
   DeviceGuard g(h->device);
   std::unique_ptr<itg_bs_chain> c(new itg_bs_chain);
   c->device = h->device;

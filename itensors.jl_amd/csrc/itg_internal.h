@@ -241,6 +241,35 @@ hipError_t launch_fiber_chain(hipStream_t s, int cplx, int nf, int xm, int ym,
                               const int32_t* wg_group, int64_t total_wgs, const void* X,
                               const void* B1, const void* B2, void* Z);
 
+// Block-sparse tensor times a block-diagonal one (DiagBlockSparse: blocks
+// (k, ..., k) storing only their diagonals), grouped over the output
+// blocks: a workgroup writes kDiagChunk consecutive elements of one output
+// block. For output element r and pair p (T1 block, D block):
+//   u0 >= 0 (D has open modes): j = r's coordinate on R mode u0; the term
+//     is d_j * T1[off1 + sum_i c_i str1[i] + j * kstr] when every R mode in
+//     umask also has coordinate j and j < dlen, else 0;
+//   u0 < 0 (D fully contracted): the sum over j < dlen of the same term.
+// Pairs of one output block are summed in plan order; R = alpha * sum.
+constexpr int kDiagChunk = 1024;
+struct DiagOut {
+  int64_t offR, nelem, wg0;
+  int32_t dims[kMaxModes];
+  int32_t p0, p1;            // DiagPair range
+};
+struct DiagPair {
+  int64_t off1, offD, kstr;
+  int32_t str1[kMaxModes];   // T1 stride of each R mode taken from T1 (0 for D's)
+  int32_t u0, dlen;
+  uint32_t umask;
+  int32_t pad;
+};
+// nr: R modes (1..kMaxModes; an order-0 output is one mode of size 1).
+// uniform: D is one value (u_re, u_im) on every diagonal element; D unused.
+hipError_t launch_diag(hipStream_t s, int cplx, int nr, const DiagOut* outs,
+                       const DiagPair* pairs, const int32_t* wg_out, int64_t total_wgs,
+                       const void* T1, const void* D, int uniform, double u_re, double u_im,
+                       double a_re, double a_im, void* R);
+
 // nf: number of A-free modes (1..kMaxModes). flags bit 0 / 1: conj(A) / conj(B).
 hipError_t launch_fiber(hipStream_t s, int cplx, int nf, const FiberGroup* groups,
                         const FiberOut* outs, const FiberPair* pairs, const int32_t* koffs,

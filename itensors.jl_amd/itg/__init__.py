@@ -16,6 +16,7 @@ from .itensors import (  # noqa: F401
     QN, Index, ITensor, In, Neither, Out, compute_contraction_labels, contract,
     contract_network, dag, eachblock, flux_of_block, itensor_from_array, nzblocks, prime,
     random_itensor, blocksparse_itensor, combiner, combinedind, combineblocks, iscombiner,
+    delta, diag_itensor, nzdiagblocks,
 )
 from .ndtensors import precision, set_precision  # noqa: F401
 from .itensors import tagstring  # noqa: F401

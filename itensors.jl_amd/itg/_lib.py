@@ -40,6 +40,7 @@ EXPORTS = (
     "itg_malloc", "itg_free", "itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod",
     "itg_timer_enable", "itg_timer_count", "itg_timer_read", "itg_set_split_tile",
     "itg_bs_copy", "itg_bs_chain_create", "itg_bs_chain_execute", "itg_bs_chain_destroy",
+    "itg_bs_diag_plan_create", "itg_bs_diag_execute",
 )
 
 
@@ -100,6 +101,8 @@ def lib():
         L.itg_bs_autotune.argtypes = [P, P, I, P, P, P, I]
         L.itg_bs_copy.argtypes = [P, I, I64, I, I64P, I64P, I64P, I64P, I64P, P, I64, P, I64]
         L.itg_bs_chain_create.argtypes = [P, P, P, ctypes.POINTER(P)]
+        L.itg_bs_diag_plan_create.argtypes = L.itg_bs_plan_create.argtypes
+        L.itg_bs_diag_execute.argtypes = [P, P, I, P, P, P, DP, DP]
         L.itg_bs_chain_execute.argtypes = [P, P, I, P, P, P, P]
         L.itg_bs_chain_destroy.argtypes = [P]
         PP = ctypes.POINTER(P)

@@ -260,9 +260,13 @@ class ITensor:
         return self.tensor.eltype
 
     def hasqns(self):
-        if isinstance(self.tensor, CB.CombinerTensor):
+        if isinstance(self.tensor, (CB.CombinerTensor, ND.DiagBlockSparseTensor)):
             return bool(self.inds) and all(i.hasqns for i in self.inds)
         return isinstance(self.tensor, ND.BlockSparseTensor)
+
+    def isdiag(self):
+        """src/itensor.jl:893."""
+        return isinstance(self.tensor, ND.DiagBlockSparseTensor)
 
     def nnzblocks(self):
         return self.tensor.nnzblocks() if self.hasqns() else 1
@@ -289,7 +293,8 @@ class ITensor:
 
     def array(self, *order):
         """Dense host array (numpy), optionally in a given index order."""
-        a = self.tensor.to_numpy_dense() if self.hasqns() else self.tensor.to_numpy()
+        a = (self.tensor.to_numpy_dense() if self.hasqns() or self.isdiag()
+             else self.tensor.to_numpy())
         if order:
             perm = [self.inds.index(i) for i in order]
             a = np.transpose(a, perm)
@@ -488,6 +493,73 @@ def _contract_combiner(A, B):
 _warn_order = [14]
 
 
+# ------------------------------------------------------ diagonal ITensors
+def nzdiagblocks(flux, inds):
+    """Diagonal blocks (k, ..., k) whose flux is `flux`
+    (src/qn/qnindexset.jl:20-29 over eachdiagblock)."""
+    if not inds:
+        return []
+    n = min(i.nblocks() for i in inds)
+    return [b for b in ((k,) * len(inds) for k in range(1, n + 1)) if flux_of_block(inds, b) == flux]
+
+
+def _diag_storage(inds, flux, values=None, uniform=None, dtype=np.float64, device="cuda"):
+    import torch
+    tdt = torch.complex128 if np.dtype(dtype) == np.complex128 else torch.float64
+    if all(i.hasqns for i in inds) and inds:
+        blocks = nzdiagblocks(QN() if flux is None else flux, inds)
+        bdims = tuple(i.blockdims() for i in inds)
+    elif not any(i.hasqns for i in inds):
+        blocks = [(1,) * len(inds)]
+        bdims = tuple((i.dim,) for i in inds)
+    else:
+        raise ValueError("diagonal ITensor over a mix of QN and dense indices")
+    offs, n = OrderedDict(), 0
+    for b in blocks:
+        offs[b] = n
+        n += min(bdims[i][c - 1] for i, c in enumerate(b)) if inds else 1
+    if uniform is not None:
+        return ND.DiagBlockSparseTensor(None, offs, bdims, uniform=uniform, dtype=tdt)
+    if values is None:
+        data = torch.zeros(n, dtype=tdt, device=device)
+    elif np.isscalar(values):
+        data = torch.full((n,), values, dtype=tdt, device=device)
+    else:
+        v = np.asarray(values)
+        if v.size != n:
+            raise ValueError(f"{v.size} diagonal values for {n} diagonal elements")
+        data = torch.from_numpy(np.ascontiguousarray(v, dtype=np.dtype(dtype))).to(device)
+    return ND.DiagBlockSparseTensor(data, offs, bdims)
+
+
+def delta(*inds, flux=None, dtype=np.float64):
+    """delta([flux,] is...): ones on the diagonal of the blocks allowed by
+    `flux` (UniformDiagBlockSparse; src/qn/qnitensor.jl:520-543, Diag for
+    indices without QNs, src/itensor.jl:607-613)."""
+    if inds and isinstance(inds[0], QN):
+        flux, inds = inds[0], inds[1:]
+    if len(inds) == 1 and isinstance(inds[0], (tuple, list)):
+        inds = tuple(inds[0])
+    one = 1.0 if np.dtype(dtype) != np.complex128 else complex(1.0)
+    return ITensor(_diag_storage(inds, flux, uniform=one, dtype=dtype), inds)
+
+
+def diag_itensor(x, *inds, flux=None, dtype=None, device="cuda"):
+    """diag_itensor(x, [flux,] is...): the diagonal set to x, a number or a
+    vector of every stored diagonal value in block order (DiagBlockSparse;
+    src/qn/qnitensor.jl:484-517, src/itensor.jl:479-600)."""
+    if inds and isinstance(inds[0], QN):
+        flux, inds = inds[0], inds[1:]
+    if len(inds) == 1 and isinstance(inds[0], (tuple, list)):
+        inds = tuple(inds[0])
+    if dtype is None:
+        dtype = np.complex128 if np.iscomplexobj(x) else np.float64
+    return ITensor(_diag_storage(inds, flux, values=x, dtype=dtype, device=device), inds)
+
+
+δ = delta
+
+
 def contract(A, B, *, prec=None, conjA=False, conjB=False):
     """A * B (src/tensor_operations/tensor_algebra.jl:1-77): contract every
     index A and B share; the result holds A's free indices (in A's order)
@@ -498,6 +570,10 @@ def contract(A, B, *, prec=None, conjA=False, conjB=False):
         warnings.warn("contraction of high-order tensors")
     lA, lB = compute_contraction_labels(A.inds, B.inds)
     lR = ND.contract_labels(lA, lB)
+    if A.isdiag() and not B.isdiag():
+        # contract(::DiagBlockSparse, l1, ::BlockSparse, l2) puts the other
+        # operand's labels first (diagblocksparse.jl:626-634)
+        lR = ND.contract_labels(lB, lA)
     indsR = tuple(A.inds[lA.index(x)] if x in lA else B.inds[lB.index(x)] for x in lR)
     if A.hasqns() != B.hasqns():
         raise ND.ItgUnsupported(2, "contracting a QN ITensor with a dense one")

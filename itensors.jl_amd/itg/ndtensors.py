@@ -223,6 +223,120 @@ def _out_dims(T1, l1, T2, l2, lR):
     return tuple(out)
 
 
+class DiagBlockSparseTensor:
+    """DiagBlockSparse storage (NDTensors/src/blocksparse/diagblocksparse.jl:
+    1-60): blocks (k, ..., k) only, each storing its diagonal, min(block
+    dims) values, at its offset in the flat device array `data`; or, when
+    `uniform` is a number, that one value on every diagonal element and no
+    array (UniformDiagBlockSparse, as `delta` builds). A tensor without QNs
+    (Diag storage) is the one-block case."""
+
+    def __init__(self, data, blockoffsets, blockdims, uniform=None, dtype=None):
+        self.data = data
+        self.uniform = uniform
+        self._dtype = data.dtype if data is not None else (dtype or torch.float64)
+        self.blockoffsets = OrderedDict((tuple(int(c) for c in b), int(o))
+                                        for b, o in blockoffsets.items())
+        self.blockdims = tuple(tuple(int(x) for x in bd) for bd in blockdims)
+        for b in self.blockoffsets:
+            if any(c != b[0] for c in b):
+                raise ValueError(f"block {b} is off the block diagonal")
+
+    @property
+    def eltype(self):
+        return self._dtype
+
+    @property
+    def ndims(self):
+        return len(self.blockdims)
+
+    @property
+    def dims(self):
+        return tuple(sum(bd) for bd in self.blockdims)
+
+    def blockdims_of(self, block):
+        return tuple(self.blockdims[i][c - 1] for i, c in enumerate(block))
+
+    def diaglen_of(self, block):
+        return min(self.blockdims_of(block)) if self.ndims else 1
+
+    def nnzblocks(self):
+        return len(self.blockoffsets)
+
+    def nnz(self):
+        return sum(self.diaglen_of(b) for b in self.blockoffsets)
+
+    def diag_numpy(self, block):
+        n = self.diaglen_of(block)
+        if self.uniform is not None:
+            return np.full(n, self.uniform, dtype=np.complex128 if self._dtype == torch.complex128
+                           else np.float64)
+        o = self.blockoffsets[block]
+        return self.data.detach().cpu().numpy()[o:o + n]
+
+    def to_numpy_dense(self):
+        """dense(T) on the host (for inspection and tests)."""
+        out = np.zeros(self.dims, dtype=np.complex128 if self._dtype == torch.complex128
+                       else np.float64)
+        starts = [np.concatenate([[0], np.cumsum(bd)]).astype(int) for bd in self.blockdims]
+        for b in self.blockoffsets:
+            d = self.diag_numpy(b)
+            for j in range(len(d)):
+                out[tuple(starts[i][c - 1] + j for i, c in enumerate(b))] = d[j]
+        return out
+
+    def as_complex(self):
+        if self._dtype == torch.complex128:
+            return self
+        if self.uniform is not None:
+            return DiagBlockSparseTensor(None, self.blockoffsets, self.blockdims,
+                                         uniform=complex(self.uniform), dtype=torch.complex128)
+        return DiagBlockSparseTensor(self.data.to(torch.complex128), self.blockoffsets,
+                                     self.blockdims)
+
+
+def _as_blocksparse(T):
+    """A DenseTensor viewed as a one-block BlockSparseTensor (same data)."""
+    if isinstance(T, DenseTensor):
+        return BlockSparseTensor(T.data, {(1,) * T.ndims: 0}, tuple((d,) for d in T.dims))
+    return T
+
+
+def _contract_diag(T1, labels1, T2, labels2, labelsR, alpha=1.0):
+    """contract(BlockSparse, DiagBlockSparse) in either order
+    (NDTensors/src/blocksparse/diagblocksparse.jl:596-634); Dense with a
+    one-block Diag goes the same way. The output labels default to the
+    non-diagonal operand's first (diagblocksparse.jl:626-634)."""
+    d1 = isinstance(T1, DiagBlockSparseTensor)
+    d2 = isinstance(T2, DiagBlockSparseTensor)
+    if d1 and d2:
+        raise ItgUnsupported(_lib.ITG_ERR_UNSUPPORTED, "contraction of two diagonal tensors")
+    if d1:
+        T1, labels1, T2, labels2 = T2, labels2, T1, labels1
+    if labelsR is None:
+        labelsR = contract_labels(labels1, labels2)
+    labelsR = tuple(labelsR)
+    dense_out = isinstance(T1, DenseTensor)
+    B, D = _as_blocksparse(T1), T2
+    if B.eltype != D.eltype:
+        if B.eltype == torch.float64:
+            B = BlockSparseTensor(B.data.to(torch.complex128), B.blockoffsets, B.blockdims)
+        else:
+            D = D.as_complex()
+    p = bs_plan(B, labels1, D, labels2, labelsR, diag=True)
+    blockdims = [B.blockdims[labels1.index(r)] if r in labels1 else D.blockdims[labels2.index(r)]
+                 for r in labelsR]
+    data = torch.empty(p.nnzR, dtype=B.eltype, device=B.data.device)
+    if p.npairs:
+        p.execute_diag(data, B.data, D, alpha=alpha)
+    if dense_out:
+        dims = tuple(sum(bd) for bd in blockdims)
+        if not p.npairs:
+            data = torch.zeros(math.prod(dims), dtype=B.eltype, device=B.data.device)
+        return DenseTensor(data, dims)
+    return BlockSparseTensor(data, OrderedDict(zip(p.blocksR, p.offsR)), blockdims)
+
+
 def _promote(T1, T2):
     """Mixed real/complex inputs are promoted by allocation, as
     dense/tensoralgebra/contract.jl:196-211 does."""
@@ -279,7 +393,7 @@ class BSPlan:
     reference's first-appearance order), its offsets, and the device tables of
     the grouped launches."""
 
-    def __init__(self, T1, labels1, T2, labels2, labelsR):
+    def __init__(self, T1, labels1, T2, labels2, labelsR, diag=False):
         dev = T1.data.device
         self.h = _handle(dev)
         L = _lib.lib()
@@ -303,9 +417,10 @@ class BSPlan:
             return ptr
 
         plan = ctypes.c_void_p()
-        st = L.itg_bs_plan_create(self.h.h, T1.ndims, n1, pp(b1), pp(o1), pp(ns1), pp(sd1), pp(l1),
-                                  T2.ndims, n2, pp(b2), pp(o2), pp(ns2), pp(sd2), pp(l2),
-                                  len(labelsR), pp(lR), ctypes.byref(plan))
+        create = L.itg_bs_diag_plan_create if diag else L.itg_bs_plan_create
+        st = create(self.h.h, T1.ndims, n1, pp(b1), pp(o1), pp(ns1), pp(sd1), pp(l1),
+                    T2.ndims, n2, pp(b2), pp(o2), pp(ns2), pp(sd2), pp(l2),
+                    len(labelsR), pp(lR), ctypes.byref(plan))
         self.h.check(st)
         self.plan = plan
         nblk, nnz, npairs = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
@@ -337,6 +452,16 @@ class BSPlan:
             pa = arr.ctypes.data_as(_lib.DP)
         st = _lib.lib().itg_bs_execute(h.h, self.plan, dt, _ptr(R_data), _ptr(T1_data), _ptr(T2_data),
                                        a, pa, _lib.PREC[prec or _precision[0]], int(flags))
+        h.check(st)
+
+    def execute_diag(self, R_data, T1_data, D, alpha=1.0):
+        """R = alpha * T1 * D for a diagonal plan; D a DiagBlockSparseTensor."""
+        h = _handle(R_data.device)
+        a, _ka = _lib.scalar2(alpha)
+        u, _ku = (_lib.scalar2(D.uniform) if D.uniform is not None else (None, None))
+        st = _lib.lib().itg_bs_diag_execute(h.h, self.plan, _dtype_code(R_data.dtype),
+                                            _ptr(R_data), _ptr(T1_data),
+                                            _ptr(D.data) if D.data is not None else None, u, a)
         h.check(st)
 
     def autotune(self, R_data, T1_data, T2_data, prec=None):
@@ -389,16 +514,17 @@ def _structure_key(T):
     return (tuple(T.blockoffsets.items()), T.blockdims)
 
 
-def bs_plan(T1, labels1, T2, labels2, labelsR):
+def bs_plan(T1, labels1, T2, labels2, labelsR, diag=False):
     """Plan of a block-sparse contraction, cached by block structure (the
-    structure repeats every time a network is applied)."""
+    structure repeats every time a network is applied). diag: T2 is a
+    DiagBlockSparseTensor."""
     key = (_structure_key(T1), tuple(labels1), _structure_key(T2), tuple(labels2), tuple(labelsR),
-           T1.data.device.index)
+           T1.data.device.index, diag)
     p = _plan_cache.get(key)
     if p is not None:
         _plan_cache.move_to_end(key)
         return p
-    p = BSPlan(T1, labels1, T2, labels2, labelsR)
+    p = BSPlan(T1, labels1, T2, labels2, labelsR, diag=diag)
     _plan_cache[key] = p
     while len(_plan_cache) > _PLAN_CACHE_MAX:
         _plan_cache.popitem(last=False)
@@ -422,6 +548,8 @@ def contract(T1, labels1, T2, labels2, labelsR=None, *, alpha=1.0, prec=None, co
     """NDTensors.contract(T1, labels1, T2, labels2[, labelsR]): allocate the
     (uninitialised) output and contract into it."""
     labels1, labels2 = tuple(labels1), tuple(labels2)
+    if isinstance(T1, DiagBlockSparseTensor) or isinstance(T2, DiagBlockSparseTensor):
+        return _contract_diag(T1, labels1, T2, labels2, labelsR, alpha)
     if labelsR is None:
         labelsR = contract_labels(labels1, labels2)
     labelsR = tuple(labelsR)

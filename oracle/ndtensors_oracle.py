@@ -40,6 +40,15 @@ Pinning status (what checks each function; see tests/test_oracle.py):
                                         (test_qnitensor.jl:666-784), nnz kept,
                                         per-block flux kept, the two-QN-name
                                         example (test_qnitensor.jl:786-828)
+  DiagBlockSparse / nzdiagblocks /
+  diagblocksparse_contract           -- closed forms: delta replaces an index,
+                                        U * diag(s) scales columns, the fully
+                                        contracted case is sum_j d_j A[j, j];
+                                        the reference's cases
+                                        (test_diagblocksparse.jl:31-78: square
+                                        and non-square blocks, the off-diagonal
+                                        error); densify-and-einsum on random QN
+                                        structures (tests/test_oracle_diag.py)
 """
 from __future__ import annotations
 
@@ -413,6 +422,74 @@ def blocksparse_contract(T1, l1, T2, l2, lR=None, alpha=1.0, exact=False, conj1=
         view[...] = dense_contract(view, lR, A, l1, B, l2, alpha, beta, exact=exact)
         written.add(bR)
     return R, plan
+
+
+# ------------------------------------------------------------ DiagBlockSparse
+def nzdiagblocks(flux, inds):
+    """src/qn/qnindexset.jl:20-29: the diagonal blocks (k, ..., k) (0-based
+    here) over k < min(nblocks), kept when their flux is `flux`."""
+    n = min(i.nblocks() for i in inds)
+    return [b for b in ((k,) * len(inds) for k in range(n)) if qn_eq(flux_of_block(inds, b), flux)]
+
+
+class DiagBlockSparse:
+    """DiagBlockSparse (NDTensors/src/blocksparse/diagblocksparse.jl:1-60):
+    blocks on the block diagonal only, each storing its diagonal (min of its
+    dims values) from its offset in `data`; `uniform` = one value on every
+    diagonal element (UniformDiagBlockSparse, what delta builds)."""
+
+    def __init__(self, data, offsets, inds, uniform=None):
+        self.data = data
+        self.offsets = offsets
+        self.inds = tuple(inds)
+        self.uniform = uniform
+
+    def diag(self, b):
+        n = min(blockdims(self.inds, b))
+        if self.uniform is not None:
+            return np.full(n, self.uniform)
+        o = self.offsets[b]
+        return self.data[o:o + n]
+
+    def dense_block(self, b):
+        """dense of one diagonal block (diag/diagtensor.jl: zeros with the
+        diagonal set)."""
+        d = self.diag(b)
+        out = np.zeros(blockdims(self.inds, b), dtype=np.result_type(d, np.float64))
+        for j, v in enumerate(d):
+            out[(j,) * len(self.inds)] = v
+        return out
+
+    def as_blocksparse(self):
+        """The same block list with every block made dense: the tensor the
+        reference contracts against (convert_to_dense)."""
+        offs, nnz = blockoffsets(list(self.offsets), self.inds)
+        dt = np.complex128 if (np.iscomplexobj(self.data) or np.iscomplexobj(self.uniform)) else np.float64
+        data = np.zeros(nnz, dtype=dt)
+        for b, o in offs.items():
+            blk = self.dense_block(b)
+            data[o:o + blk.size] = blk.reshape(-1, order="F")
+        return BlockSparse(data, offs, self.inds)
+
+    def dense(self):
+        return self.as_blocksparse().dense()
+
+
+def diagblocksparse_contract(T, lT, D, lD, lR=None, alpha=1.0):
+    """contract(::BlockSparseTensor, ::DiagBlockSparseTensor)
+    (NDTensors/src/blocksparse/diagblocksparse.jl:596-675): the output
+    structure from contract_blockoffsets, then per pair in plan order the
+    dense contract! of the T block with the dense form of the diagonal block
+    (diag/tensoralgebra/contract.jl:103-160 with convert_to_dense, :208-216
+    for the argument order), beta 0 for an output block's first pair and 1
+    after. A block off the block diagonal is an error
+    (diagblocksparse.jl:645-649). The output labels default to T's first
+    (contract_labels(lT, lD); diagonal-first calls swap, :626-634)."""
+    for b in D.offsets:
+        if any(c != b[0] for c in b):
+            raise ValueError("When contracting a BlockSparse tensor with a DiagBlockSparse "
+                             "tensor, the DiagBlockSparse tensor must be block diagonal")
+    return blocksparse_contract(T, lT, D.as_blocksparse(), lD, lR, alpha)
 
 
 # ------------------------------------------------------------------- adjoint
