@@ -26,6 +26,7 @@ METRIC = "GFLOP/s block-sparse contract on 1 GPU: split-prec vs FP64 at ≤1e-12
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 INT8_PEAK_TOPS = 5000.0
+HBM_PEAK_GBS = 8000.0
 
 
 def parse():
@@ -47,6 +48,9 @@ def parse():
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the other configs of the metric (dense Float64 chi=1024, "
                          "block-sparse ComplexF64 chi=4096, dense ComplexF64 chi=4096)")
+    ap.add_argument("--diag-only", action="store_true",
+                    help="run only the DiagBlockSparse x BlockSparse measurement (chi=4096 psi "
+                         "times a diagonal over one bond) and print its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="(internal) run step 1 in every mode a few times plus a 256 MiB copy "
                          "and exit; the workload of the counter passes")
@@ -199,6 +203,15 @@ def kernel_times(torch, fn, reps):
     return {k: (c / reps, ms / c) for k, (c, ms) in t.results.items()}
 
 
+def _blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return max([p.get("num_threads", 1) for p in threadpool_info()
+                    if p.get("internal_api") == "openblas"] or [os.cpu_count()])
+    except Exception:
+        return os.cpu_count()
+
+
 def cpu_baseline(args, net_tensors, gpu_outputs):
     """Time the oracle (numpy, BLAS-threaded) on one apply of the same
     network and return the figures plus the relative errors of the device
@@ -206,11 +219,7 @@ def cpu_baseline(args, net_tensors, gpu_outputs):
     import numpy as np
     from oracle import ndtensors_oracle as O
 
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([p.get("num_threads", 1) for p in threadpool_info() if p.get("internal_api") == "openblas"] or [os.cpu_count()])
-    except Exception:
-        threads = os.cpu_count()
+    threads = _blas_threads()
 
     def to_oracle(T):
         host = T.tensor.data.detach().cpu().numpy()
@@ -337,6 +346,67 @@ def secondary_configs(args, torch, dev):
     return out
 
 
+def diag_config(args, torch, dev):
+    """DiagBlockSparse x BlockSparse at chi = 4096: psi(a, s1, s2, b) times
+    a diagonal S(dag(b), b') (the singular values of a bond, random here),
+    one diag_contract_kernel launch. HBM roofline: algorithmic bytes =
+    (nnz psi read + nnz out written + diagonal read) * 8. CPU beside it:
+    the oracle's restatement (densified diagonal blocks, per-pair BLAS)."""
+    import numpy as np
+    import itg
+    from collections import OrderedDict
+    from itg.configs import heff_network
+    from oracle import ndtensors_oracle as O
+
+    L, psi, W1, W2, R, ix = heff_network(4096, qns=True, seed=0, device=dev)
+    del L, W1, W2, R
+    b = ix.b
+    bp = itg.Index(b.space, "b'", dir=b.dir)
+    S = itg.diag_itensor(0.0, b.dag(), bp, device=dev)
+    S.tensor.data.copy_(torch.from_numpy(np.random.default_rng(0).standard_normal(S.tensor.nnz())))
+    out = psi * S
+    torch.cuda.synchronize()
+    kt = kernel_times(torch, lambda: psi * S, 20)
+    launches, k_ms = kt["diag_contract_kernel"]
+    nbytes = (psi.tensor.nnz() + out.tensor.nnz() + S.tensor.nnz()) * 8
+    gbs = nbytes / (k_ms * 1e-3) / 1e9
+    r = {"workload": "psi(chi=4096 U(1)) * diag(s) over b", "nnz_in": psi.tensor.nnz(),
+         "nnz_out": out.tensor.nnz(), "kernel_ms": k_ms, "launches": launches,
+         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes}}
+    if not args.no_cpu:
+        def oi(inds):
+            return tuple(O.Index([(O.QN(*[(n, v, m) for n, (v, m) in q.vals.items()]), d)
+                                  for q, d in i.space], tags=i.tags, plev=i.plev, dir=i.dir,
+                                 id=i.id) for i in inds)
+        offs = OrderedDict((tuple(c - 1 for c in bb), o) for bb, o in psi.tensor.blockoffsets.items())
+        Op = O.BlockSparse(psi.tensor.data.cpu().numpy(), offs, oi(psi.inds))
+        soffs = OrderedDict((tuple(c - 1 for c in bb), o) for bb, o in S.tensor.blockoffsets.items())
+        Os = O.DiagBlockSparse(S.tensor.data.cpu().numpy(), soffs, oi(S.inds))
+        lA, lB = itg.compute_contraction_labels(psi.inds, S.inds)
+        t0 = time.perf_counter()
+        ref, _p = O.diagblocksparse_contract(Op, lA, Os, lB)
+        dt = time.perf_counter() - t0
+        r["cpu_baseline"] = {"value": nbytes / dt / 1e9, "unit": "GB/s", "seconds": dt,
+                             "kind": "port", "cores": _blas_threads(),
+                             "sample": "the same contraction by the numpy restatement"}
+        v = out.tensor.data.cpu().numpy()
+        r["rel_err_vs_oracle"] = float(np.linalg.norm(v - ref.data) / np.linalg.norm(ref.data))
+    log(args, f"[bench] diag: {json.dumps(r)}")
+    return r
+
+
+def _calib_copy(torch, dev, mib):
+    """Five device copies of a mib-MiB buffer: the known-size reference the
+    FETCH_SIZE / WRITE_SIZE readings are scaled by."""
+    n = mib * (1 << 20) // 8
+    src = torch.randn(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -345,6 +415,15 @@ def main():
         sys.exit(2)
     if args.traffic_probe:
         traffic_probe(args)
+        return
+    if args.diag_only:
+        import torch
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        r = diag_config(args, torch, dev)
+        if args.calib_copy_mib > 0:
+            _calib_copy(torch, dev, args.calib_copy_mib)
+        print(json.dumps(r))
         return
     # counter passes first: they run as child processes, before this process
     # initialises the GPU
@@ -474,13 +553,9 @@ def main():
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         out["secondary_configs"] = secondary_configs(args, torch, dev)
+        out["diag_blocksparse"] = diag_config(args, torch, dev)
     if args.calib_copy_mib > 0:
-        n = args.calib_copy_mib * (1 << 20) // 8
-        src = torch.randn(n, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+        _calib_copy(torch, dev, args.calib_copy_mib)
     print(json.dumps(out))
 
 
