@@ -17,7 +17,10 @@ UTF-8 NULLTERM, as HDF5.jl writes them):
 * TagSet (tagset.jl:4-9): "tags" = the tag string;
 * Dense{T} (NDTensors/ext/NDTensorsHDF5Ext/dense.jl:4-16): "data";
 * BlockSparse{T} (blocksparse.jl:37-52): "ndims", "data", "offsets" =
-  per block its N 1-based block coordinates then its 0-based offset.
+  per block its N 1-based block coordinates then its 0-based offset;
+* Diag{T,Vector{T}} / Diag{T,T} (diag.jl:4-16, tensors without QNs): "data",
+  the diagonal, or one value (delta) as a 0-d dataset. The reference has no
+  HDF5 form for DiagBlockSparse (QN) storage, so neither has this module.
 
 ComplexF64 data is an HDF5 compound {r, i} of doubles (HDF5.jl's complex
 type); read also accepts the C++ ITensor form, a (n, 2) double array
@@ -317,9 +320,16 @@ def _write_scalar(node, name, v, kind="i64"):
         H.H5Dwrite(d, t, _H5S_ALL, _H5S_ALL, _H5P_DEFAULT,
                    ctypes.create_string_buffer(_b(v), max(len(_b(v)), 1)))
         H.H5Tclose(t)
+    elif kind == "c128":
+        t = _complex_type()
+        x = (ctypes.c_double * 2)(complex(v).real, complex(v).imag)
+        d = H.H5Dcreate2(node.hid, _b(name), t, sp, _H5P_DEFAULT, _H5P_DEFAULT, _H5P_DEFAULT)
+        H.H5Dwrite(d, t, _H5S_ALL, _H5S_ALL, _H5P_DEFAULT, x)
+        H.H5Tclose(t)
     else:
-        t = H.UINT64 if kind == "u64" else H.INT64
-        x = (ctypes.c_uint64 if kind == "u64" else ctypes.c_int64)(int(v))
+        t = {"u64": H.UINT64, "f64": H.DOUBLE}.get(kind, H.INT64)
+        x = {"u64": ctypes.c_uint64, "f64": ctypes.c_double}.get(kind, ctypes.c_int64)(
+            float(v) if kind == "f64" else int(v))
         d = H.H5Dcreate2(node.hid, _b(name), t, sp, _H5P_DEFAULT, _H5P_DEFAULT, _H5P_DEFAULT)
         H.H5Dwrite(d, t, _H5S_ALL, _H5S_ALL, _H5P_DEFAULT, ctypes.byref(x))
     H.H5Dclose(d)
@@ -557,7 +567,11 @@ def _eltype_name(dtype):
 
 
 def _write_storage(parent, name, T, compress=None):
-    host = T.data.detach().cpu().numpy()
+    if isinstance(T, ND.DiagBlockSparseTensor) and T.blockdims and any(
+            len(bd) != 1 for bd in T.blockdims):
+        raise TypeError("HDF5 write of DiagBlockSparse (QN diagonal) storage: the reference "
+                        "defines no HDF5 form for it")
+    host = None if T.data is None else T.data.detach().cpu().numpy()
     with parent.create_group(name) as g:
         if isinstance(T, ND.BlockSparseTensor):
             _tag(g, f"BlockSparse{{{_eltype_name(T.data.dtype)}}}")
@@ -571,6 +585,14 @@ def _write_storage(parent, name, T, compress=None):
         elif isinstance(T, ND.DenseTensor):
             _tag(g, f"Dense{{{_eltype_name(T.data.dtype)}}}")
             _write_array(g, "data", host, compress)
+        elif isinstance(T, ND.DiagBlockSparseTensor):
+            el = _eltype_name(T.eltype)
+            if T.uniform is not None:
+                _tag(g, f"Diag{{{el},{el}}}")
+                _write_scalar(g, "data", T.uniform, "c128" if el == "ComplexF64" else "f64")
+            else:
+                _tag(g, f"Diag{{{el},Vector{{{el}}}}}")
+                _write_array(g, "data", host, compress)
         else:
             raise TypeError(f"HDF5 write of {type(T).__name__} storage is not supported")
 
@@ -588,6 +610,8 @@ def _read_storage(parent, name, inds, device):
     import torch
     with parent.group(name) as g:
         t = _attr(g, "type")
+        if t and t.startswith("Diag{"):
+            return _read_diag(g, t, inds, device)
         for kind in ("Dense", "BlockSparse"):
             for el, dt in (("Float64", np.float64), ("ComplexF64", np.complex128)):
                 if t == f"{kind}{{{el}}}":
@@ -618,6 +642,29 @@ def _read_storage(parent, name, inds, device):
             rec = off[k * (N + 1):(k + 1) * (N + 1)]
             boff[tuple(int(c) for c in rec[:N])] = int(rec[N])
         return ND.BlockSparseTensor(tdata, boff, tuple(i.blockdims() for i in inds))
+
+
+def _read_diag(g, t, inds, device):
+    """Diag{ElT,DataT} (diag.jl:18-44): one diagonal block over the whole
+    index set; DataT = ElT means one value on every diagonal element."""
+    import torch
+    el, datat = t[len("Diag{"):-1].split(",", 1)
+    if el not in ("Float64", "ComplexF64") or datat not in (el, f"Vector{{{el}}}"):
+        raise H5Error(f"HDF5 storage type {t!r} is not supported")
+    cplx = el == "ComplexF64"
+    if any(i.hasqns for i in inds):
+        raise H5Error("Diag storage over QN indices")
+    offs = {(1,) * len(inds): 0}
+    bdims = tuple((i.dim,) for i in inds)
+    v = _dataset(g, "data")
+    if datat == el:
+        return ND.DiagBlockSparseTensor(None, offs, bdims, uniform=complex(v) if cplx else float(v),
+                                        dtype=torch.complex128 if cplx else torch.float64)
+    a = np.ascontiguousarray(np.asarray(v).reshape(-1), dtype=np.complex128 if cplx else np.float64)
+    n = min(i.dim for i in inds) if inds else 1
+    if a.size != n:
+        raise H5Error(f"Diag data length {a.size} does not match the indices (diagonal {n})")
+    return ND.DiagBlockSparseTensor(torch.from_numpy(a).to(device), offs, bdims)
 
 
 # ---------------------------------------------------------------- dispatch

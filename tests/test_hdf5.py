@@ -136,6 +136,39 @@ def test_qn_itensor_round_trip(env, tmp_path, cplx):
     assert R.flux() == T.flux()
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_delta_and_diag_itensor_round_trip(env, tmp_path, cplx):
+    """test_readwrite.jl:108-150: delta (one value, Diag{T,T}) and a
+    diagonal ITensor (Diag{T,Vector{T}}) without QNs."""
+    itg, H5 = env
+    k = itg.Index(4, "k")
+    dt = np.complex128 if cplx else np.float64
+    v = np.random.default_rng(5).standard_normal(4) + (1j if cplx else 0) * 0.5
+    D = itg.diag_itensor(v.astype(dt), k, k.prime(), device="cpu")
+    d = itg.delta(k, k.prime(), dtype=dt)
+    p = str(tmp_path / "data.h5")
+    with H5.h5open(p, "w") as f:
+        H5.write(f, "diag_tensor", D)
+        H5.write(f, "delta_tensor", d)
+    with H5.h5open(p) as f:
+        rD = H5.read(f, "diag_tensor", itg.ITensor, device="cpu")
+        rd = H5.read(f, "delta_tensor", itg.ITensor, device="cpu")
+        assert H5._attr(f.group("delta_tensor").group("storage"), "type") == (
+            "Diag{ComplexF64,ComplexF64}" if cplx else "Diag{Float64,Float64}")
+    assert rD.isdiag() and rd.isdiag() and rd.tensor.uniform == d.tensor.uniform
+    assert np.array_equal(rD.array(), D.array()) and np.array_equal(rd.array(), d.array())
+    assert all(same_index(a, b) for a, b in zip(rD.inds, D.inds))
+
+
+def test_qn_diagonal_storage_is_refused(env, tmp_path):
+    """No HDF5 form exists for DiagBlockSparse in the reference."""
+    itg, H5 = env
+    i = ref_qn_inds(itg)[0]
+    with H5.h5open(str(tmp_path / "d.h5"), "w") as f:
+        with pytest.raises(TypeError):
+            H5.write(f, "d", itg.delta(i, i.prime().dag()))
+
+
 def test_reads_the_reference_v0_1_41_file(env):
     """test_readwrite.jl:182-192: an ITensor written by ITensors.jl v0.1.41
     (storage under the old key "store") reads back with the ids, dims,
