@@ -86,6 +86,35 @@ def test_fused_mpo_steps_bit_identical_and_match_oracle(env, chi, cplx):
         assert relerr(fused.result.tensor.data.cpu().numpy(), ref.data) < FP64_RTOL
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_fused_pair_with_wide_link_sectors(env, cplx):
+    """A 3-wide MPO link sector gives more than eight x / y values per
+    super-fiber (the 16-wide form of the kernel): still bit-identical to
+    the separate steps and equal to the oracle."""
+    torch, itg, ND, O = env
+    from itg.configs import HeffIndices, SZ
+    from itg.network import ContractionSequence
+    QN, Index = itg.QN, itg.Index
+    ix = HeffIndices(256, True)
+    lk = [(QN((SZ, 0)), 1), (QN((SZ, 2)), 1), (QN((SZ, -2)), 1), (QN((SZ, 0)), 3)]
+    ix.w, ix.wp, ix.wpp = (Index(lk, t, dir=itg.In) for t in ("w", "w'", "w''"))
+    rng = np.random.default_rng(21)
+    z = QN((SZ, 0))
+    dt = np.complex128 if cplx else np.float64
+    net = [itg.random_itensor(z, *inds, rng=rng, dtype=dt)
+           for inds in (ix.L(), ix.psi(), ix.W1(), ix.W2(), ix.R())]
+    fused = ContractionSequence(net)
+    plain = ContractionSequence(net, fuse=False)
+    assert list(fused.chains) == [1]
+    zf = fused.run("fp64").tensor.data.clone()
+    zp = plain.run("fp64").tensor.data.clone()
+    torch.cuda.synchronize()
+    assert torch.equal(zf, zp)
+    ref = oracle_apply(O, net)
+    same_blocks(fused.result, ref)
+    assert relerr(zf.cpu().numpy(), ref.data) < FP64_RTOL
+
+
 def test_fused_sequence_graph_replays(env):
     """Captured once, replayed several times with eager work in between:
     every replay equals the eager result."""
