@@ -55,6 +55,61 @@ def precision() -> str:
     return _precision[0]
 
 
+# The reference's algorithm switches, kept for API parity. On the device a
+# block-sparse plan is one grouped launch whatever the threading switch says,
+# there is no TBLIS, and the GEMM backend is libitg's own kernels, so these
+# record the setting and change nothing.
+_flags = {"threaded_blocksparse": False, "tblis": False, "gemm_backend": "Auto"}
+
+
+def using_threaded_blocksparse() -> bool:
+    """NDTensors/src/NDTensors.jl:106-184."""
+    return _flags["threaded_blocksparse"]
+
+
+def enable_threaded_blocksparse() -> bool:
+    """Returns the previous state (NDTensors.jl:109-130)."""
+    old = _flags["threaded_blocksparse"]
+    _flags["threaded_blocksparse"] = True
+    return old
+
+
+def disable_threaded_blocksparse() -> bool:
+    old = _flags["threaded_blocksparse"]
+    _flags["threaded_blocksparse"] = False
+    return old
+
+
+def using_tblis() -> bool:
+    """NDTensors.jl:243-255."""
+    return _flags["tblis"]
+
+
+def enable_tblis():
+    _flags["tblis"] = True
+
+
+def disable_tblis():
+    _flags["tblis"] = False
+
+
+def gemm_backend() -> str:
+    """abstractarray/tensoralgebra/contract.jl:15-24 (:Auto, :BLAS, :Generic)."""
+    return _flags["gemm_backend"]
+
+
+def backend_auto():
+    _flags["gemm_backend"] = "Auto"
+
+
+def backend_blas():
+    _flags["gemm_backend"] = "BLAS"
+
+
+def backend_generic():
+    _flags["gemm_backend"] = "Generic"
+
+
 # ------------------------------------------------------------------ types
 def _dtype_code(tdtype):
     if tdtype == torch.float64:

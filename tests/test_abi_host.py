@@ -147,3 +147,19 @@ def test_precision_flag():
     itg.set_precision(old)
     with pytest.raises(ValueError):
         itg.set_precision("fp32")
+
+
+def test_reference_algorithm_switches_are_accepted():
+    """NDTensors' threading / TBLIS / GEMM-backend switches exist and report
+    their state (NDTensors.jl:106-184,243-255; tensoralgebra/contract.jl:15-24)."""
+    from itg import ndtensors as ND
+    assert ND.enable_threaded_blocksparse() is False
+    assert ND.using_threaded_blocksparse() is True
+    assert ND.disable_threaded_blocksparse() is True
+    ND.enable_tblis()
+    assert ND.using_tblis()
+    ND.disable_tblis()
+    ND.backend_blas()
+    assert ND.gemm_backend() == "BLAS"
+    ND.backend_auto()
+    assert ND.gemm_backend() == "Auto"
