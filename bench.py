@@ -495,29 +495,32 @@ def main():
             head = "fp64"
 
     r = results[head]
-    # dominant kernel: the grouped GEMM of step 1 (L*psi), timed alone by the
-    # library's per-kernel events; algorithmic flops = sum over its block
-    # pairs of 2*m*n*k
+    # dominant kernel: the grouped GEMM, launched twice per apply (steps 1
+    # and 4, L*psi and Z*R, the same algorithmic flops); its average launch
+    # duration inside the apply, from the library's per-kernel events
     kname = "gemm_f64_kernel" if head == "fp64" else "split_gemm_kernel"
-    kt1 = kernel_times(torch, lambda: seq.run_step(0, head), max(5, args.steps))
-    launches, k_ms = kt1[kname]
-    achieved = step_flops[0] / (launches * k_ms * 1e-3) / 1e12
+    launches, k_ms = (r["kernels_per_apply"][kname]["launches"],
+                      r["kernels_per_apply"][kname]["ms_per_launch"])
+    gemm_flops = (step_flops[0] + step_flops[3]) / 2.0
+    achieved = gemm_flops / (k_ms * 1e-3) / 1e12
     tr = traffic.get(head)
     traffic_bytes = (tr[0] + tr[1]) if tr else None
     if head == "fp64":
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes,
-                "kernel": f"gemm_f64_kernel, step 1 (L*psi), {launches:g} launch(es) per step, "
-                          f"{k_ms:.4f} ms per launch"}
+                "kernel": f"gemm_f64_kernel, steps 1 and 4 of the apply ({launches:g} launches, "
+                          f"{gemm_flops / 1e9:.2f} GFLOP each), {k_ms:.4f} ms per launch inside an "
+                          f"apply; traffic: step 1's launch"}
     else:
         # the slice products run on INT8 MFMA: 28 products per FP64 product
         int8_ops = achieved * 28.0
         roof = {"bound": "mfma", "achieved": int8_ops, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                 "frac": int8_ops / INT8_PEAK_TOPS, "traffic": traffic_bytes,
                 "fp64_equivalent_tflops": achieved,
-                "kernel": f"split_gemm_kernel, step 1 (L*psi), {launches:g} launch(es) per step, "
-                          f"{k_ms:.4f} ms per launch; achieved = 28 int8 slice products per "
-                          f"algorithmic FP64 flop (padding not counted)"}
+                "kernel": f"split_gemm_kernel, steps 1 and 4 of the apply ({launches:g} launches, "
+                          f"{gemm_flops / 1e9:.2f} GFLOP each), {k_ms:.4f} ms per launch inside an "
+                          f"apply; achieved = 28 int8 slice products per algorithmic "
+                          f"FP64 flop (padding not counted); traffic: step 1's launch"}
     if tr:
         roof["traffic_read_write_bytes"] = [tr[0], tr[1]]
         roof["traffic_note"] = ("HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, "
