@@ -328,6 +328,15 @@ def dag(T):
     (NDTensors/src/combiner/combiner.jl:39-40)."""
     if iscombiner(T):
         return ITensor(T.tensor.conj(), tuple(i.dag() for i in T.inds))
+    if T.isdiag():
+        D = T.tensor
+        if D.uniform is not None:
+            S = ND.DiagBlockSparseTensor(None, D.blockoffsets, D.blockdims,
+                                         uniform=np.conj(D.uniform).item(), dtype=D.eltype)
+        else:
+            d = D.data.conj().resolve_conj() if D.data.is_complex() else D.data
+            S = ND.DiagBlockSparseTensor(d, D.blockoffsets, D.blockdims)
+        return ITensor(S, tuple(i.dag() for i in T.inds))
     data = T.tensor.data
     if data.is_complex():
         data = data.conj().resolve_conj()

@@ -120,3 +120,4 @@ def test_random_structures_match_dense_einsum(seed):
     R, _ = O.diagblocksparse_contract(A, (1, -1, 2), D, (-1, 3, 4))
     ref = np.einsum("ajk,jbc->akbc", A.dense(), D.dense())
     assert np.allclose(R.dense(), ref, rtol=0, atol=1e-13)
+
