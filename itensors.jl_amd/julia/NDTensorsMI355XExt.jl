@@ -24,6 +24,7 @@ module NDTensorsMI355XExt
 using NDTensors
 using NDTensors: DenseTensor, BlockSparseTensor, BlockOffsets, Block, Dense, BlockSparse,
     blockoffsets, inds, nnzblocks, nblocks, blockdim, data, tensor, contract_labels
+using NDTensors: DiagBlockSparseTensor
 using NDTensors.Expose: Exposed, unexpose
 using Dictionaries: Dictionary
 
@@ -200,6 +201,81 @@ function NDTensors.contract(
     finally
         ccall((:itg_bs_plan_destroy, libitg), Cint, (Ptr{Cvoid},), plan[])
     end
+end
+
+# --------------------------------------------------------- diagonal
+# BlockSparse x DiagBlockSparse (NDTensors/src/blocksparse/diagblocksparse.jl:
+# 596-675): the same block-pair plan with the diagonal tensor's blocks and
+# diagonal offsets, one grouped launch; a uniform diagonal (delta) passes its
+# value instead of data.
+function diagstructure(D::DiagBlockSparseTensor)
+    N = ndims(D)
+    blocks = Int64[]
+    offs = Int64[]
+    for (b, o) in pairs(NDTensors.diagblockoffsets(D))
+        append!(blocks, Int64.(Tuple(b)))
+        push!(offs, o)
+    end
+    nsec = Int64[nblocks(inds(D), i) for i in 1:N]
+    secd = Int64[blockdim(inds(D)[i], j) for i in 1:N for j in 1:nsec[i]]
+    return length(offs), blocks, offs, nsec, secd
+end
+
+function NDTensors.contract(
+        T1::BlockSparseTensor{<:Any, N1, <:Any, <:BlockSparse{<:Any, <:ItgVector}}, labels1,
+        T2::DiagBlockSparseTensor{<:Any, N2}, labels2,
+        labelsR = contract_labels(labels1, labels2)
+    ) where {N1, N2}
+    elt = eltype(T1)
+    eltype(T2) == elt || error("promote both operands to one element type first")
+    n1, b1, o1, ns1, sd1 = blockstructure(T1)
+    n2, b2, o2, ns2, sd2 = diagstructure(T2)
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:itg_bs_diag_plan_create, libitg), Cint,
+                (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Cint, Ptr{Int64}, Ref{Ptr{Cvoid}}),
+                handle(), N1, n1, b1, o1, ns1, sd1, collect(Int64, labels1),
+                N2, n2, b2, o2, ns2, sd2, collect(Int64, labels2),
+                length(labelsR), collect(Int64, labelsR), plan))
+    try
+        nblk, nnz, npairs = Ref{Int64}(), Ref{Int64}(), Ref{Int64}()
+        check(ccall((:itg_bs_plan_sizes, libitg), Cint,
+                    (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}), plan[], nblk, nnz, npairs))
+        NR = length(labelsR)
+        blocksR = Vector{Int64}(undef, NR * nblk[])
+        offsR = Vector{Int64}(undef, nblk[])
+        check(ccall((:itg_bs_plan_blocks, libitg), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}),
+                    plan[], blocksR, offsR))
+        boffsR = BlockOffsets{NR}()
+        for k in 1:nblk[]
+            insert!(boffsR, Block(ntuple(i -> UInt(blocksR[(k - 1) * NR + i]), NR)), Int(offsR[k]))
+        end
+        indsR = NDTensors.contract_inds(inds(T1), labels1, inds(T2), labels2, labelsR)
+        R = tensor(BlockSparse(ItgVector{elt}(undef, nnz[]), boffsR), indsR)
+        if npairs[] > 0
+            uniform = data(T2) isa Number
+            check(ccall((:itg_bs_diag_execute, libitg), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                         Ptr{ComplexF64}, Ref{ComplexF64}),
+                        handle(), plan[], eltcode(elt), data(R).ptr, data(T1).ptr,
+                        uniform ? C_NULL : data(T2).ptr,
+                        uniform ? Ref(ComplexF64(data(T2))) : C_NULL, ComplexF64(1)))
+        end
+        return R
+    finally
+        ccall((:itg_bs_plan_destroy, libitg), Cint, (Ptr{Cvoid},), plan[])
+    end
+end
+
+# the diagonal tensor first: the other operand's labels come first
+# (diagblocksparse.jl:626-634)
+function NDTensors.contract(
+        T1::DiagBlockSparseTensor, labels1,
+        T2::BlockSparseTensor{<:Any, N2, <:Any, <:BlockSparse{<:Any, <:ItgVector}}, labels2,
+        labelsR = contract_labels(labels2, labels1)
+    ) where {N2}
+    return NDTensors.contract(T2, labels2, T1, labels1, labelsR)
 end
 
 # --------------------------------------------------------------- combiner
