@@ -16,7 +16,9 @@ from .itensors import (  # noqa: F401
     QN, Index, ITensor, In, Neither, Out, compute_contraction_labels, contract,
     contract_network, dag, eachblock, flux_of_block, itensor_from_array, nzblocks, prime,
     random_itensor, blocksparse_itensor, combiner, combinedind, combineblocks, iscombiner,
-    delta, diag_itensor, nzdiagblocks,
+    delta, diag_itensor, nzdiagblocks, contraction_cost, optimal_contraction_sequence,
+    enable_contraction_sequence_optimization, disable_contraction_sequence_optimization,
+    using_contraction_sequence_optimization,
 )
 from .ndtensors import precision, set_precision  # noqa: F401
 from .itensors import tagstring  # noqa: F401

@@ -592,20 +592,128 @@ def contract(A, B, *, prec=None, conjA=False, conjB=False):
     return ITensor(T, indsR)
 
 
-def contract_network(As, sequence=None, *, prec=None):
-    """contract(As; sequence) with the default left-associative fold
-    (tensor_algebra.jl:121-134). `sequence` may be a nested list of
-    positions, e.g. [[0, 1], 2]."""
+_sequence_optimization = [False]
+
+
+def using_contraction_sequence_optimization():
+    """src/global_variables.jl (default off)."""
+    return _sequence_optimization[0]
+
+
+def enable_contraction_sequence_optimization():
+    _sequence_optimization[0] = True
+
+
+def disable_contraction_sequence_optimization():
+    _sequence_optimization[0] = False
+
+
+def left_associative_contraction_sequence(n):
+    """[[[1, 2], 3], ...] (src/tensor_operations/contraction_cost.jl:1-2)."""
+    seq = 1
+    for k in range(2, n + 1):
+        seq = [seq, k] if k > 2 else [1, 2]
+    return seq
+
+
+def _seq_inds(As, seq, costs):
+    if isinstance(seq, int):
+        return list(As[seq - 1].inds)
+    a = _seq_inds(As, seq[0], costs)
+    b = _seq_inds(As, seq[1], costs)
+    union = a + [i for i in b if i not in a]
+    costs.append(math.prod(i.dim for i in union))
+    return [i for i in a if i not in b] + [i for i in b if i not in a]
+
+
+def contraction_cost(As, sequence=None):
+    """Pairwise costs of contracting As along `sequence` (1-based, as the
+    reference's): each pair costs dim(union(inds1, inds2)), sparsity
+    ignored (src/tensor_operations/contraction_cost.jl:4-42)."""
+    costs = []
+    _seq_inds(list(As), sequence or left_associative_contraction_sequence(len(As)), costs)
+    return costs
+
+
+def optimal_contraction_sequence(As):
+    """A sequence of least total cost (the cost model of contraction_cost)
+    by dynamic programming over subsets of the tensors -- the search the
+    reference delegates to TensorOperations.optimaltree
+    (ext/ITensorsTensorOperationsExt/ITensorsTensorOperationsExt.jl:6-13).
+    Exact; meant for the few-tensor networks it is used on (up to ~14)."""
     As = list(As)
-    if sequence is None or sequence == "left_associative":
+    n = len(As)
+    if n == 1:
+        return 1
+    if n > 16:
+        raise ValueError("optimal_contraction_sequence: more than 16 tensors")
+    full = (1 << n) - 1
+    inds = [list(A.inds) for A in As]
+    # indices of the tensor a subset contracts to: each pairwise step keeps
+    # the symmetric difference, which is associative, so whatever the order
+    # they are the indices met an odd number of times in the subset
+    open_of, best, split = {}, {}, {}
+
+    def open_inds(S):
+        if S not in open_of:
+            seen = {}
+            for k in range(n):
+                if S >> k & 1:
+                    for i in inds[k]:
+                        seen[i] = seen.get(i, 0) + 1
+            open_of[S] = [i for i, c in seen.items() if c % 2 == 1]
+        return open_of[S]
+
+    for k in range(n):
+        best[1 << k] = 0.0
+    for S in range(1, full + 1):
+        if S & (S - 1) == 0:
+            continue
+        low = S & -S
+        T = (S - 1) & S
+        while T:
+            if T & low:  # each unordered split once
+                U = S ^ T
+                a, b = open_inds(T), open_inds(U)
+                union = a + [i for i in b if i not in a]
+                c = best[T] + best[U] + math.prod(i.dim for i in union)
+                if S not in best or c < best[S]:
+                    best[S], split[S] = c, (T, U)
+            T = (T - 1) & S
+
+    def build(S):
+        if S & (S - 1) == 0:
+            return S.bit_length()
+        T, U = split[S]
+        return [build(T), build(U)]
+
+    return build(full)
+
+
+def contract_network(As, sequence=None, *, prec=None):
+    """contract(As; sequence) (src/tensor_operations/tensor_algebra.jl:121-159):
+    "left_associative" (the default, unless sequence optimisation is
+    enabled), "right_associative", "automatic" (optimal_contraction_sequence)
+    or an explicit tree of 1-based positions, e.g. [[1, 2], 3]."""
+    As = list(As)
+    if sequence is None:
+        sequence = "automatic" if using_contraction_sequence_optimization() else "left_associative"
+    if sequence == "left_associative":
         out = As[0]
         for B in As[1:]:
             out = contract(out, B, prec=prec)
         return out
+    if sequence == "right_associative":
+        out = As[-1]
+        for B in reversed(As[:-1]):
+            out = contract(B, out, prec=prec)
+        return out
+    if sequence == "automatic":
+        sequence = optimal_contraction_sequence(As)
 
     def rec(s):
         if isinstance(s, int):
-            return As[s]
+            return As[s - 1]
         a, b = s
         return contract(rec(a), rec(b), prec=prec)
 
