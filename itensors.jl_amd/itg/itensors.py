@@ -574,6 +574,12 @@ def contract(A, B, *, prec=None, conjA=False, conjB=False):
     index A and B share; the result holds A's free indices (in A's order)
     then B's. QN indices must meet with opposite arrows."""
     if iscombiner(A) or iscombiner(B):
+        # a combiner holds no values: a conjugation flag acts on the other
+        # operand's data, conjugated before the data movement
+        if conjA and not iscombiner(A):
+            A = ITensor(ND._conj_storage(A.tensor), A.inds)
+        if conjB and not iscombiner(B):
+            B = ITensor(ND._conj_storage(B.tensor), B.inds)
         return _contract_combiner(A, B)
     if len(A.inds) + len(B.inds) >= _warn_order[0] * 2:
         warnings.warn("contraction of high-order tensors")

@@ -357,6 +357,22 @@ def _as_blocksparse(T):
     return T
 
 
+def _conj_storage(T):
+    """conj of a storage's values (a copy for complex data; real data and
+    real uniform values are returned as they are)."""
+    if isinstance(T, DiagBlockSparseTensor):
+        if T.uniform is not None:
+            u = complex(T.uniform).conjugate() if T.eltype == torch.complex128 else T.uniform
+            return DiagBlockSparseTensor(None, T.blockoffsets, T.blockdims, uniform=u,
+                                         dtype=T.eltype)
+        if not T.data.is_complex():
+            return T
+        return DiagBlockSparseTensor(T.data.conj().resolve_conj(), T.blockoffsets, T.blockdims)
+    if not T.data.is_complex():
+        return T
+    return _with_data(T, T.data.conj().resolve_conj())
+
+
 def _contract_diag(T1, labels1, T2, labels2, labelsR, alpha=1.0):
     """contract(BlockSparse, DiagBlockSparse) in either order
     (NDTensors/src/blocksparse/diagblocksparse.jl:596-634); Dense with a
@@ -604,6 +620,11 @@ def contract(T1, labels1, T2, labels2, labelsR=None, *, alpha=1.0, prec=None, co
     (uninitialised) output and contract into it."""
     labels1, labels2 = tuple(labels1), tuple(labels2)
     if isinstance(T1, DiagBlockSparseTensor) or isinstance(T2, DiagBlockSparseTensor):
+        # the diagonal kernel takes no conjugation flags: conjugate first
+        if conj1:
+            T1 = _conj_storage(T1)
+        if conj2:
+            T2 = _conj_storage(T2)
         return _contract_diag(T1, labels1, T2, labels2, labelsR, alpha)
     if labelsR is None:
         labelsR = contract_labels(labels1, labels2)

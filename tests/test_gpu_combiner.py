@@ -238,3 +238,17 @@ def test_block_copy_rejects_out_of_range_descriptors(env):
                 [((2, 2), 0, (1, 2), -1, (1, 2))]):  # negative destination
         with pytest.raises(itg.ItgError):
             CB.bs_copy(dst, src, bad)
+
+
+def test_conjugation_flag_with_a_combiner(env):
+    """contract(A, C; conjA=True) with a combiner C conjugates A's values
+    (the adjoint of a combiner contraction does this)."""
+    _, itg, ND, O = env
+    QN, Index = itg.QN, itg.Index
+    i = Index([(QN(("Sz", 0)), 2), (QN(("Sz", 1)), 3)], "i")
+    j = Index([(QN(("Sz", 0)), 1), (QN(("Sz", -1)), 2)], "j", dir=itg.In)
+    A = itg.random_itensor(QN(), i, j, rng=np.random.default_rng(13), dtype=complex)
+    C = itg.combiner(i, j)
+    X = itg.contract(A, C, conjA=True)
+    Y = A * C
+    assert np.array_equal(X.tensor.data.cpu().numpy(), np.conj(Y.tensor.data.cpu().numpy()))

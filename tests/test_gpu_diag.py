@@ -179,3 +179,20 @@ def test_off_block_diagonal_rejected_by_the_abi(env):
         ND.BSPlan(A.tensor, (1, -1), Fake(), (-1, 2), (1, 2), diag=True)
     with pytest.raises(ValueError):
         ND.DiagBlockSparseTensor(None, Fake.blockoffsets, Fake.blockdims, uniform=1.0)
+
+
+def test_conjugation_flags_reach_the_diagonal_path(env):
+    """contract(...; conjA / conjB) with a diagonal operand (the adjoint of
+    a contraction conjugates one side) equals contracting conj'd copies."""
+    torch, itg, ND, O = env
+    a, b = qidx(itg, SP, "a"), qidx(itg, SP, "b", dir=itg.In)
+    U = itg.random_itensor(a, b, rng=np.random.default_rng(11), dtype=np.complex128)
+    rng = np.random.default_rng(12)
+    s = rng.standard_normal(23) + 1j * rng.standard_normal(23)
+    bp = qidx(itg, SP, "bp", dir=itg.In)
+    S = itg.diag_itensor(s, b.dag(), bp)
+    ref = U.array(a, b).conj() * s.conj()[None, :]
+    C = itg.contract(U, S, conjA=True, conjB=True)
+    assert np.allclose(C.array(a, bp), ref, rtol=0, atol=1e-15)
+    C2 = itg.contract(S, U, conjA=True)
+    assert np.allclose(C2.array(a, bp), U.array(a, b) * s.conj()[None, :], rtol=0, atol=1e-15)
