@@ -569,10 +569,44 @@ def diag_itensor(x, *inds, flux=None, dtype=None, device="cuda"):
 δ = delta
 
 
+def _scaled(T, x):
+    """x * T, on the device through the library's permute (identity
+    permutation, alpha = x) over the flat data; the storage kind is kept."""
+    import torch
+    S = T.tensor
+    cplx = isinstance(x, (complex, np.complexfloating)) and complex(x).imag != 0
+    if isinstance(S, ND.DiagBlockSparseTensor) and S.uniform is not None:
+        u = S.uniform * x
+        dt = torch.complex128 if (cplx or S.eltype == torch.complex128) else torch.float64
+        return ITensor(ND.DiagBlockSparseTensor(None, S.blockoffsets, S.blockdims,
+                                                uniform=u if dt == torch.complex128 else float(np.real(u)),
+                                                dtype=dt), T.inds)
+    data = S.data
+    if cplx and not data.is_complex():
+        data = data.to(torch.complex128)
+    out = torch.empty_like(data)
+    if data.numel():
+        ND.permutedims_(ND.DenseTensor(out, (data.numel(),)), ND.DenseTensor(data, (data.numel(),)),
+                        (1,), alpha=x if data.is_complex() else float(np.real(x)))
+    if isinstance(S, ND.DiagBlockSparseTensor):
+        return ITensor(ND.DiagBlockSparseTensor(out, S.blockoffsets, S.blockdims), T.inds)
+    return ITensor(ND._with_data(S, out), T.inds)
+
+
 def contract(A, B, *, prec=None, conjA=False, conjB=False):
     """A * B (src/tensor_operations/tensor_algebra.jl:1-77): contract every
     index A and B share; the result holds A's free indices (in A's order)
     then B's. QN indices must meet with opposite arrows."""
+    if not iscombiner(A) and not iscombiner(B) and (len(A.inds) == 0 or len(B.inds) == 0):
+        # an order-0 operand is a number: A[] * B (tensor_algebra.jl:66-77)
+        if conjA:
+            A = ITensor(ND._conj_storage(A.tensor), A.inds)
+        if conjB:
+            B = ITensor(ND._conj_storage(B.tensor), B.inds)
+        if len(A.inds) == 0 and len(B.inds) == 0:
+            return itensor_from_array(np.asarray(A.scalar() * B.scalar()),
+                                      device=A.tensor.data.device)
+        return _scaled(B, A.scalar()) if len(A.inds) == 0 else _scaled(A, B.scalar())
     if iscombiner(A) or iscombiner(B):
         # a combiner holds no values: a conjugation flag acts on the other
         # operand's data, conjugated before the data movement

@@ -223,3 +223,25 @@ def test_stale_hip_error_from_the_host_is_not_reported(env):
                     (1, 2))
     torch.cuda.synchronize()
     assert relerr(T.to_numpy(), A @ B) < FP64_RTOL
+
+
+def test_order_zero_operands_scale(env):
+    """An order-0 ITensor is a number: A[] * B, B * A[], A[] * B[]
+    (tensor_algebra.jl:66-77), also against a QN tensor and a diagonal one."""
+    import itg
+    rng = np.random.default_rng(14)
+    i, j = itg.Index(4, "i"), itg.Index(3, "j")
+    B = itg.random_itensor(i, j, rng=rng)
+    s = itg.itensor_from_array(np.asarray(2.5))
+    assert np.array_equal((s * B).array(i, j), 2.5 * B.array(i, j))
+    assert np.array_equal((B * s).array(i, j), 2.5 * B.array(i, j))
+    assert (s * s).scalar() == 6.25
+    z = itg.itensor_from_array(np.asarray(1.0 - 2.0j))
+    assert np.allclose((z * B).array(i, j), (1.0 - 2.0j) * B.array(i, j), rtol=0, atol=1e-15)
+    q = itg.Index([(itg.QN(0), 2), (itg.QN(1), 3)], "q")
+    Q = itg.random_itensor(q, q.dag().prime(), rng=rng)
+    SQ = s * Q
+    assert SQ.hasqns() and list(SQ.tensor.blockoffsets) == list(Q.tensor.blockoffsets)
+    assert np.array_equal(SQ.tensor.data.cpu().numpy(), 2.5 * Q.tensor.data.cpu().numpy())
+    D = s * itg.delta(q, q.dag().prime())
+    assert D.isdiag() and D.tensor.uniform == 2.5
