@@ -499,7 +499,26 @@ def _contract_combiner(A, B):
 
 
 # ------------------------------------------------------------- contraction
-_warn_order = [14]
+_warn_order = [14]  # src/global_variables.jl: default_warn_order
+
+
+def get_warn_order():
+    return _warn_order[0]
+
+
+def set_warn_order(n):
+    """Returns the previous threshold; None disables the warning."""
+    old = _warn_order[0]
+    _warn_order[0] = n
+    return old
+
+
+def reset_warn_order():
+    return set_warn_order(14)
+
+
+def disable_warn_order():
+    return set_warn_order(None)
 
 
 # ------------------------------------------------------ diagonal ITensors
@@ -615,8 +634,6 @@ def contract(A, B, *, prec=None, conjA=False, conjB=False):
         if conjB and not iscombiner(B):
             B = ITensor(ND._conj_storage(B.tensor), B.inds)
         return _contract_combiner(A, B)
-    if len(A.inds) + len(B.inds) >= _warn_order[0] * 2:
-        warnings.warn("contraction of high-order tensors")
     lA, lB = compute_contraction_labels(A.inds, B.inds)
     lR = ND.contract_labels(lA, lB)
     if A.isdiag() and not B.isdiag():
@@ -626,8 +643,11 @@ def contract(A, B, *, prec=None, conjA=False, conjB=False):
     indsR = tuple(A.inds[lA.index(x)] if x in lA else B.inds[lB.index(x)] for x in lR)
     if A.hasqns() != B.hasqns():
         raise ND.ItgUnsupported(2, "contracting a QN ITensor with a dense one")
-    if len(lR) == 0 and not A.hasqns():
-        pass
+    if _warn_order[0] is not None and len(indsR) >= _warn_order[0]:
+        # the order warning of _contract (tensor_algebra.jl:8-22)
+        warnings.warn(f"Contraction resulted in ITensor with {len(indsR)} indices, which is "
+                      f"greater than or equal to the ITensor order warning threshold "
+                      f"{_warn_order[0]}.")
     T = ND.contract(A.tensor, lA, B.tensor, lB, lR, prec=prec, conj1=conjA, conj2=conjB)
     return ITensor(T, indsR)
 

@@ -91,3 +91,15 @@ def test_sequence_optimization_switch():
     assert itg.using_contraction_sequence_optimization()
     itg.disable_contraction_sequence_optimization()
     assert not itg.using_contraction_sequence_optimization()
+
+
+def test_warn_order_switches():
+    """src/global_variables.jl: the threshold can be read, set, reset and
+    disabled."""
+    import itg.itensors as I
+    assert I.get_warn_order() == 14
+    assert I.set_warn_order(5) == 14 and I.get_warn_order() == 5
+    I.disable_warn_order()
+    assert I.get_warn_order() is None
+    I.reset_warn_order()
+    assert I.get_warn_order() == 14
