@@ -196,3 +196,33 @@ def test_conjugation_flags_reach_the_diagonal_path(env):
     assert np.allclose(C.array(a, bp), ref, rtol=0, atol=1e-15)
     C2 = itg.contract(S, U, conjA=True)
     assert np.allclose(C2.array(a, bp), U.array(a, b) * s.conj()[None, :], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_structures(env, seed):
+    """Random U(1) sector layouts and arrows; a stored diagonal over one of
+    the block-sparse tensor's indices and an open one, either operand
+    order, real or complex: device vs the oracle."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(100 + seed)
+
+    def space():
+        qs = sorted(rng.choice(np.arange(-3, 4), size=int(rng.integers(2, 5)), replace=False))
+        return [(int(q), int(rng.integers(1, 6))) for q in qs]
+
+    sa, sb = space(), space()
+    a = qidx(itg, sa, "a")
+    b = qidx(itg, sb, "b", dir=itg.In)
+    c = qidx(itg, space(), "c", dir=itg.Out if seed % 2 else itg.In)
+    A = itg.random_itensor(a, b, c, rng=rng, dtype=np.complex128 if seed % 3 == 0 else np.float64)
+    if A.nnzblocks() == 0:
+        pytest.skip("no allowed blocks")
+    if seed % 2:
+        D = itg.diag_itensor(0.0, b.dag(), qidx(itg, sb, "bp", dir=itg.In))
+    else:
+        D = itg.diag_itensor(0.0, a.dag(), qidx(itg, sa, "x", dir=itg.Out))
+    if D.tensor.nnzblocks() == 0:
+        pytest.skip("no diagonal blocks with zero flux")
+    D.tensor.data.copy_(torch.from_numpy(rng.standard_normal(D.tensor.data.numel())))
+    check(env, A, D)
+    check(env, D, A)
