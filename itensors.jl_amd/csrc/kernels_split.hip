@@ -287,13 +287,6 @@ __global__ void __launch_bounds__(kThreads)
     KWalk kw;
     kw.init((uint32_t)(ch * 16), d);
     constexpr int NC = CPLX ? 3 : 1;  // re, im, -im
-    uint32_t packed[NC][S][4];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int i = 0; i < S; ++i)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) packed[c][i][w] = 0u;
     // the 16 offsets first (padding k >= K clamped onto the row's k = 0),
     // then the 16 loads back to back and unconditional, then the digits
     int64_t off[16];
@@ -317,30 +310,34 @@ __global__ void __launch_bounds__(kThreads)
         xi[j] = 0.0;
       }
     }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      double y = ldexp(xr[j], 6 - e), yi = ldexp(xi[j], 6 - e);
-#pragma unroll
-      for (int i = 0; i < S; ++i) {
-        const double v = rint(y);
-        y = (y - v) * 128.0;
-        packed[0][i][j >> 2] |= ((uint32_t)((int32_t)v & 0xff)) << (8 * (j & 3));
-        if (CPLX) {
-          const double w = rint(yi);
-          yi = (yi - w) * 128.0;
-          packed[1][i][j >> 2] |= ((uint32_t)((int32_t)w & 0xff)) << (8 * (j & 3));
-          packed[NC - 1][i][j >> 2] |= ((uint32_t)((int32_t)(-w) & 0xff)) << (8 * (j & 3));
-        }
-      }
-    }
+    // one slice set at a time (re, then im and -im for complex data: the
+    // digits of -im are those of im negated), so only one set's packed
+    // digits are live: with all three at once the complex kernel needed a
+    // full register file and ran one wave per SIMD
     int8_t* dst = planes + (int64_t)nsets * d.plane_off + ((int64_t)ch * d.R + gr) * 16;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if (c >= nsets) break;
+      uint32_t packed[S][4];
 #pragma unroll
       for (int i = 0; i < S; ++i)
-        *reinterpret_cast<v4i*>(dst + (c * S + i) * pstride) = v4i{
-            (int)packed[c][i][0], (int)packed[c][i][1], (int)packed[c][i][2], (int)packed[c][i][3]};
+#pragma unroll
+        for (int w = 0; w < 4; ++w) packed[i][w] = 0u;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        double y = ldexp(c == 0 ? xr[j] : xi[j], 6 - e);
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const double v = rint(y);
+          y = (y - v) * 128.0;
+          const int32_t dg = c == 2 ? -(int32_t)v : (int32_t)v;
+          packed[i][j >> 2] |= ((uint32_t)(dg & 0xff)) << (8 * (j & 3));
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+        *reinterpret_cast<v4i*>(dst + (c * S + i) * pstride) =
+            v4i{(int)packed[i][0], (int)packed[i][1], (int)packed[i][2], (int)packed[i][3]};
     }
   }
 }
