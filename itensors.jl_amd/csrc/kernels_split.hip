@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(kThreads)
   if (!kfast) {
     // lanes along the rows; thread part p walks its quarter of the K range,
     // 8 elements at a time: the 8 offsets first, then 8 loads in flight
-    constexpr int kQ = kSliceKPart / 4, kU = 8;
+    constexpr int kQ = kSliceKPart / 4, kU = 16;
     const int row = tid % kSliceRows, part = tid / kSliceRows;
     double m = 0.0;
     const int ka = k_lo + part * kQ, kb = min(k_hi, ka + kQ);
