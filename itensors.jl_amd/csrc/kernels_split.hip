@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(kThreads)
   __syncthreads();
   if (!kfast) {
     // lanes along the rows; thread part p walks its quarter of the K range,
-    // 8 elements at a time: the 8 offsets first, then 8 loads in flight
+    // 16 elements at a time: the 16 offsets first, then 16 loads in flight
     constexpr int kQ = kSliceKPart / 4, kU = 16;
     const int row = tid % kSliceRows, part = tid / kSliceRows;
     double m = 0.0;
