@@ -58,6 +58,9 @@ def parse():
 
 
 CALIB_MIB = 256
+# int8 slice products per FP64 product of the split form (S slices: the
+# products v_i w_j with i + j < S)
+SLICE_PRODUCTS = 28
 KERNELS = {"fp64": "gemm_f64_kernel", "split_int8": "split_gemm_kernel"}
 # dispatch-name prefixes of every tile variant of each mode's GEMM (the
 # library's timer reports them all under the KERNELS name)
@@ -114,9 +117,10 @@ def measure_traffic(args):
 
     prof = shutil.which("rocprofv3")
     if prof is None:
-        return {}
+        return {}, {}
     out = {m: [None, None] for m in KERNELS}
-    for j, counter in enumerate(("FETCH_SIZE", "WRITE_SIZE")):
+    mops = {}
+    for j, counter in enumerate(("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU_MFMA_MOPS_I8")):
         with tempfile.TemporaryDirectory() as d:
             cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--",
                    sys.executable, os.path.abspath(__file__), "--traffic-probe",
@@ -128,16 +132,25 @@ def measure_traffic(args):
             except subprocess.TimeoutExpired:
                 os.killpg(p.pid, signal.SIGKILL)
                 p.wait()
-                return {}
+                return {}, {}
             path = os.path.join(d, "t_counter_collection.csv")
             cpath = os.path.join(d, "probe_counts.json")
             if not os.path.exists(path) or not os.path.exists(cpath):
-                return {}
+                return {}, {}
             rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
             counts = json.load(open(cpath))
+        if j == 2:
+            # executed INT8 MFMA work of the split GEMM (the counter counts
+            # units of 512 ops), tile padding included
+            n = int(counts.get("split_int8", 0))
+            v = [float(r["Counter_Value"]) for r in rows
+                 if f"::{KERNEL_PREFIX['split_int8']}" in r["Kernel_Name"]]
+            if n > 0 and len(v) >= 3 * n:
+                mops["split_int8"] = sum(v[-3 * n:]) / (3 * n) * 512.0
+            continue
         copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
         if not copies or max(copies) <= 0:
-            return {}
+            return {}, {}
         scale = CALIB_MIB * 1024.0 / max(copies)  # counters read KiB
         for m, kname in KERNELS.items():
             n = int(counts.get(m, 0))
@@ -146,7 +159,7 @@ def measure_traffic(args):
             if n > 0 and len(v) >= 3 * n:
                 # the last 3 steps, after tuning: bytes per launch
                 out[m][j] = sum(v[-3 * n:]) / (3 * n) * scale * 1024.0
-    return {m: tuple(v) for m, v in out.items() if None not in v}
+    return {m: tuple(v) for m, v in out.items() if None not in v}, mops
 
 
 def log(args, *a):
@@ -212,14 +225,70 @@ def _blas_threads():
         return os.cpu_count()
 
 
-def cpu_baseline(args, net_tensors, gpu_outputs):
-    """Time the oracle (numpy, BLAS-threaded) on one apply of the same
-    network and return the figures plus the relative errors of the device
-    outputs against it."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _host_cores():
+    """Cores this process may use, capped at the 16-thread CPU share of one
+    GPU on the box."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _blocksparse_chain_threaded(O, ts, threads):
+    """The oracle's left-to-right block-sparse chain with the per-output-block
+    pair loops spread over `threads` host threads (BLAS at 1 thread inside
+    each), the reference's threaded block-sparse layout
+    (NDTensors/src/blocksparse/contract_threaded.jl:77-92 with Folds over
+    output blocks). Same pair order inside a block, so the values equal the
+    sequential chain's."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+
+    cur = ts[0]
+    with ThreadPoolExecutor(threads) as ex:
+        for B in ts[1:]:
+            lA, lB = O.compute_contraction_labels(cur.inds, B.inds)
+            lR = O.contract_labels(lA, lB)
+            offs, nnz, plan, indsR = O.contract_blockoffsets_sequential(cur, lA, B, lB, lR)
+            R = O.BlockSparse(np.empty(nnz, dtype=np.result_type(cur.data, B.data)), offs, indsR)
+            groups = {}
+            for b1, b2, bR in plan:
+                groups.setdefault(bR, []).append((b1, b2))
+
+            def one(item, A=cur, B=B, lA=lA, lB=lB, lR=lR, R=R):
+                bR, prs = item
+                view = R.block(bR)
+                for n, (b1, b2) in enumerate(prs):
+                    view[...] = O.dense_contract(view, lR, A.block(b1), lA, B.block(b2), lB,
+                                                 1.0, 1.0 if n else 0.0)
+
+            list(ex.map(one, groups.items()))
+            cur = R
+    return cur
+
+
+def cpu_baseline(args, net_tensors, gpu_outputs, flops):
+    """Time the oracle on one apply of the same network on this host: one
+    thread (sequential pair loop, BLAS at 1 thread) and all cores (output
+    blocks over the cores, BLAS at 1 thread each); one warm-up, then the
+    median of 5 runs each. Returns the baseline record and the relative
+    errors of the device outputs against the sequential result."""
+    import statistics
     import numpy as np
     from oracle import ndtensors_oracle as O
-
-    threads = _blas_threads()
+    from threadpoolctl import threadpool_limits
 
     def to_oracle(T):
         host = T.tensor.data.detach().cpu().numpy()
@@ -232,15 +301,29 @@ def cpu_baseline(args, net_tensors, gpu_outputs):
         return O.BlockSparse(host, OrderedDict(offs), tuple(inds))
 
     ts = [to_oracle(t) for t in net_tensors]
-    flops = 0
-    t0 = time.perf_counter()
-    cur = ts[0]
-    for B in ts[1:]:
-        lA, lB = O.compute_contraction_labels(cur.inds, B.inds)
-        lR = O.contract_labels(lA, lB)
-        cur, plan = O.blocksparse_contract(cur, lA, B, lB, lR)
-    dt = time.perf_counter() - t0
+
+    def sequential():
+        cur = ts[0]
+        for B in ts[1:]:
+            lA, lB = O.compute_contraction_labels(cur.inds, B.inds)
+            cur, _plan = O.blocksparse_contract(cur, lA, B, lB, O.contract_labels(lA, lB))
+        return cur
+
+    cores = _host_cores()
+    runs = {}
+    with threadpool_limits(limits=1, user_api="blas"):
+        for name, fn in (("1_thread", sequential),
+                         ("all_cores", lambda: _blocksparse_chain_threaded(O, ts, cores))):
+            out = fn()  # warm-up
+            times = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                fn()
+                times.append(time.perf_counter() - t0)
+            runs[name] = (statistics.median(times), out)
+    cur = runs["1_thread"][1]
     ref = cur.data
+    same = bool(np.array_equal(runs["all_cores"][1].data, ref))
     errs = {}
     for k, v in gpu_outputs.items():
         d = np.linalg.norm(v - ref)
@@ -251,7 +334,19 @@ def cpu_baseline(args, net_tensors, gpu_outputs):
             if nb > 0:
                 worst = max(worst, float(np.linalg.norm(v[o:o + n] - ref[o:o + n]) / nb))
         errs[k] = (float(d / np.linalg.norm(ref)), worst)
-    return dt, threads, errs, flops
+    t1, ta = runs["1_thread"][0], runs["all_cores"][0]
+    rec = {"value": flops / ta / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": f"one full apply of the chi={args.chi} network (4 contractions, "
+                     f"{flops / 1e9:.2f} GFLOP) by the numpy restatement: per-pair BLAS at 1 "
+                     f"thread, output blocks spread over {cores} host threads; median of 5 "
+                     f"runs after one warm-up",
+           "seconds": ta, "threaded_equals_sequential": same,
+           "single_thread": {"value": flops / t1 / 1e9, "unit": "GFLOP/s", "cores": 1,
+                             "seconds": t1,
+                             "sample": "the same apply, sequential pair loop, BLAS at 1 "
+                                       "thread; median of 5 runs after one warm-up"}}
+    return rec, errs
 
 
 def _time_network(torch, seq, mode, steps, warmup):
@@ -427,8 +522,9 @@ def main():
         return
     # counter passes first: they run as child processes, before this process
     # initialises the GPU
-    traffic = {} if args.no_traffic else measure_traffic(args)
-    log(args, f"[bench] HBM bytes per step-1 GEMM launch (read, write): {traffic}")
+    traffic, mfma_ops = ({}, {}) if args.no_traffic else measure_traffic(args)
+    log(args, f"[bench] HBM bytes per step-1 GEMM launch (read, write): {traffic}; "
+              f"executed INT8 MFMA ops per split launch: {mfma_ops}")
     import torch
     import itg
     from itg.configs import heff_network, heff_flops
@@ -483,11 +579,10 @@ def main():
     # the headline mode must meet the accuracy bar; otherwise fall back to FP64
     cpu = None
     if not args.no_cpu:
-        log(args, "[bench] CPU baseline: one apply of the same network with the numpy oracle")
-        dt, threads, errs, _ = cpu_baseline(args, net, outputs)
-        cpu = {"value": flops / dt / 1e9, "unit": "GFLOP/s", "cores": int(threads), "kind": "port",
-               "sample": f"one full apply of the chi={args.chi} network (4 contractions, "
-                         f"{flops / 1e9:.2f} GFLOP) by the numpy restatement, per-pair BLAS"}
+        log(args, "[bench] CPU baseline: one apply of the same network with the numpy oracle "
+                  "(1 thread and all cores, median of 5)")
+        cpu, errs = cpu_baseline(args, net, outputs, flops)
+        log(args, f"[bench] CPU: {json.dumps(cpu)}")
         for m in results:
             results[m]["rel_err_vs_oracle"] = errs[m][0]
             results[m]["max_block_rel_err_vs_oracle"] = errs[m][1]
@@ -512,15 +607,26 @@ def main():
                           f"{gemm_flops / 1e9:.2f} GFLOP each), {k_ms:.4f} ms per launch inside an "
                           f"apply; traffic: step 1's launch"}
     else:
-        # the slice products run on INT8 MFMA: 28 products per FP64 product
-        int8_ops = achieved * 28.0
-        roof = {"bound": "mfma", "achieved": int8_ops, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                "frac": int8_ops / INT8_PEAK_TOPS, "traffic": traffic_bytes,
-                "fp64_equivalent_tflops": achieved,
+        # the work runs on the INT8 matrix cores: achieved is the ALGORITHMIC
+        # rate (2*m*n*k FP64 flops of the pairs, no slices counted) over the
+        # INT8 dense peak; the slice products and the executed INT8 ops
+        # (MFMA counter, tile padding included) are reported beside it
+        nprod = SLICE_PRODUCTS
+        roof = {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic_bytes,
+                "fp64_equivalent_frac": achieved / FP64_PEAK_TFLOPS,
+                "slice_products_per_flop": nprod,
+                "slice_product_frac": achieved * nprod / INT8_PEAK_TOPS,
                 "kernel": f"split_gemm_kernel, steps 1 and 4 of the apply ({launches:g} launches, "
                           f"{gemm_flops / 1e9:.2f} GFLOP each), {k_ms:.4f} ms per launch inside an "
-                          f"apply; achieved = 28 int8 slice products per algorithmic "
-                          f"FP64 flop (padding not counted); traffic: step 1's launch"}
+                          f"apply; achieved = algorithmic FP64 flops per second, peak = dense "
+                          f"INT8 MFMA ops per second (the unit doing the work); "
+                          f"slice_product_frac counts {nprod} int8 slice products per flop; "
+                          f"traffic: step 1's launch"}
+        ops = mfma_ops.get(head)
+        if ops:
+            roof["executed_int8_ops_per_launch"] = ops
+            roof["executed_int8_frac"] = ops / (k_ms * 1e-3) / 1e12 / INT8_PEAK_TOPS
     if tr:
         roof["traffic_read_write_bytes"] = [tr[0], tr[1]]
         roof["traffic_note"] = ("HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, "

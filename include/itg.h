@@ -108,6 +108,28 @@ int64_t itg_timer_count(itg_handle* h);
 itg_status itg_timer_read(itg_handle* h, int64_t i, const char** name, int64_t* calls,
                           double* total_ms);
 
+/* ------------------------------------------------------------ HIP graphs */
+/* Record the handle's subsequent launches into a graph instead of running
+ * them, then replay the whole sequence with one launch: the captured form of
+ * an effective-Hamiltonian apply that an eigensolver repeats. A handle on
+ * the null stream records on a stream of its own (after waiting for the null
+ * stream); otherwise the handle's stream is captured. Every call recorded
+ * must have run once on this handle before (its plan tables, per-pair
+ * factor tables and workspace then exist: building them is not
+ * capturable, and a call that would needs makes itg_graph_end fail). The
+ * graph reads the caller's buffers at the addresses it was recorded with,
+ * and the plans it uses must outlive it; the library keeps every workspace
+ * and per-pair factor table a graph may reference until the handle / plan is
+ * destroyed. While a capture is open, calls on the handle only record.
+ * No NDTensors counterpart (its contractions run eagerly); the replacement
+ * for the hipGraph of the north-star batching pattern at the boundary. */
+typedef struct itg_graph itg_graph;
+itg_status itg_graph_begin(itg_handle* h);
+itg_status itg_graph_end(itg_handle* h, itg_graph** out);
+/* Launch the graph on the handle's stream (stream-ordered, returns at once). */
+itg_status itg_graph_launch(itg_handle* h, itg_graph* g);
+itg_status itg_graph_destroy(itg_graph* g);
+
 /* ---------------------------------------------------------- device memory */
 /* Device buffers for callers without their own GPU array package (the Julia
  * extension's ItgVector storage type, which stands where CuArray/ROCArray
@@ -275,6 +297,29 @@ itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
                           const int64_t* blockdims, const int64_t* offs_src,
                           const int64_t* offs_dst, const int32_t* perm,
                           const void* src, void* dst);
+
+/* The general block-sparse permutedims!(R, T, perm, f) with
+ * f(r, t) = beta * r + alpha * t over the UNION of R's blocks and T's
+ * permuted blocks (NDTensors/src/blocksparse/blocksparsetensor.jl:834-881):
+ * the caller lists every block of the union (R's blocks in R's order, then
+ * T's permuted blocks that R lacks, in T's order, as the reference iterates
+ * `union(blocks_R, perm_blocks_T)`), with, per block b:
+ *   blockdims[b*N ..]  the block's dims in T's mode order;
+ *   offs_src[b]        its offset in T's data, or -1 when T has no such
+ *                      block (t = 0: the block becomes beta * r);
+ *   offs_dst[b]        its offset in R's data (R's storage already holds
+ *                      room for the new blocks);
+ *   fresh[b]           (fresh may be NULL = none) 1 when R had no such block:
+ *                      r = 0 and the destination is written without being read.
+ * nonzero (NULL or nblk ints) returns 1 for every block whose written values
+ * are not all zero: the reference inserts a new block into R only when it is
+ * nonzero, so the caller drops the fresh blocks flagged 0. beta == 0 never
+ * reads R. Synchronous on the handle's stream. */
+itg_status itg_bs_permute_ex(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
+                             const int64_t* blockdims, const int64_t* offs_src,
+                             const int64_t* offs_dst, const uint8_t* fresh,
+                             const int32_t* perm, const void* src, void* dst,
+                             const double* alpha, const double* beta, int32_t* nonzero);
 
 /* Grouped strided block copy: for each block b (0 <= b < nblk) and every
  * multi-index x over dims[b*nmodes .. b*nmodes+nmodes),

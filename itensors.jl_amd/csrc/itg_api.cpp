@@ -35,6 +35,16 @@ struct itg_handle {
   // autotuned tile cfg per (bucket, cplx, prec)
   std::map<std::string, int> tuned;
   int split_tile = 0;  // itg_set_split_tile (0 = per plan)
+  // itg_graph_begin / _end: the stream being captured, and the caller's
+  // stream to restore (a handle on the null stream captures on its own)
+  hipStream_t cap_stream = nullptr;  // created on first use, destroyed with the handle
+  hipStream_t pre_capture = nullptr;
+  bool capturing = false;
+};
+
+struct itg_graph {
+  int device = 0;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct itg_bs_plan {
@@ -44,9 +54,11 @@ struct itg_bs_plan {
   BlockList b1, b2, bR;
   std::vector<int64_t> blocksR;  // NR x nblkR, 1-based
   std::vector<PairSpec> pairs;
-  std::unique_ptr<ExecPlan> ep;      // pair_alpha = NULL
-  std::unique_ptr<ExecPlan> ep_alpha;  // last per-pair alpha variant
-  std::vector<double> ep_alpha_key;
+  std::unique_ptr<ExecPlan> ep;  // pair_alpha = NULL
+  // one plan per set of per-pair factors used so far (the factors are baked
+  // into the device tables). None is freed before the plan is destroyed: a
+  // graph captured with one of them keeps reading its tables.
+  std::map<std::vector<double>, std::unique_ptr<ExecPlan>> ep_alpha;
   // T2 block diagonal (itg_bs_diag_plan_create): device tables of
   // diag_contract_kernel instead of ep
   bool diag = false;
@@ -141,7 +153,7 @@ itg_status run_plan(itg_handle* h, ExecPlan& ep, int cplx, int prec, const void*
                     const double* beta) {
   const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
   const double b_re = beta ? beta[0] : 0.0, b_im = (beta && cplx) ? beta[1] : 0.0;
-  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx, prec));
+  hipError_t e = ensure_ws(h, exec_workspace_bytes(ep, cplx, prec, a_im != 0.0));
   if (e != hipSuccess) return hip_fail(h, e, "workspace allocation");
   std::string err;
   const int cfg = pick_cfg(h, ep, cplx, prec);
@@ -175,13 +187,94 @@ itg_status itg_destroy(itg_handle* h) {
   if (!h) return ITG_OK;
   {
     DeviceGuard g(h->device);
+    if (h->capturing) {
+      hipGraph_t gr = nullptr;
+      (void)hipStreamEndCapture(h->stream, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      h->stream = h->pre_capture;
+    }
     if (h->ws || !h->retired.empty()) (void)hipStreamSynchronize(h->stream);
     if (h->ws) (void)hipFree(h->ws);
     for (void* p : h->retired) (void)hipFree(p);
     h->dense_map.clear();
     h->dense_lru.clear();
+    if (h->cap_stream) {
+      (void)hipStreamSynchronize(h->cap_stream);
+      (void)hipStreamDestroy(h->cap_stream);
+    }
   }
   delete h;
+  return ITG_OK;
+}
+
+itg_status itg_graph_begin(itg_handle* h) {
+  if (!h) return ITG_ERR_INVALID;
+  if (h->capturing) return fail(h, ITG_ERR_INVALID, "a capture is already open on this handle");
+  DeviceGuard g(h->device);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (h->stream && (hipStreamIsCapturing(h->stream, &cs) != hipSuccess ||
+                    cs != hipStreamCaptureStatusNone))
+    return fail(h, ITG_ERR_INVALID, "the handle's stream is already being captured");
+  h->pre_capture = h->stream;
+  if (!h->stream) {
+    // the null stream cannot be captured: record on the handle's own stream,
+    // ordered after the work already queued on the null stream
+    if (!h->cap_stream) {
+      hipError_t e = hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking);
+      if (e != hipSuccess) return hip_fail(h, e, "capture stream");
+    }
+    hipError_t e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
+    h->stream = h->cap_stream;
+  }
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    h->stream = h->pre_capture;
+    return hip_fail(h, e, "hipStreamBeginCapture");
+  }
+  h->capturing = true;
+  return ITG_OK;
+}
+
+itg_status itg_graph_end(itg_handle* h, itg_graph** out) {
+  if (!h || !out) return ITG_ERR_INVALID;
+  *out = nullptr;
+  if (!h->capturing) return fail(h, ITG_ERR_INVALID, "no capture is open on this handle");
+  DeviceGuard g(h->device);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &gr);
+  h->stream = h->pre_capture;
+  h->capturing = false;
+  if (e != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return hip_fail(h, e, "capture failed (a call inside it needed a new plan table, workspace "
+                          "or per-pair factors: run it once before capturing)");
+  }
+  std::unique_ptr<itg_graph> ig(new itg_graph);
+  ig->device = h->device;
+  e = hipGraphInstantiate(&ig->exec, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+  *out = ig.release();
+  return ITG_OK;
+}
+
+itg_status itg_graph_launch(itg_handle* h, itg_graph* gph) {
+  if (!h || !gph) return ITG_ERR_INVALID;
+  if (gph->device != h->device) return fail(h, ITG_ERR_INVALID, "graph built on another device");
+  if (h->capturing) return fail(h, ITG_ERR_INVALID, "cannot launch a graph while capturing");
+  DeviceGuard g(h->device);
+  hipError_t e = hipGraphLaunch(gph->exec, h->stream);
+  if (e != hipSuccess) return hip_fail(h, e, "hipGraphLaunch");
+  return ITG_OK;
+}
+
+itg_status itg_graph_destroy(itg_graph* gph) {
+  if (!gph) return ITG_OK;
+  DeviceGuard g(gph->device);
+  (void)hipDeviceSynchronize();
+  if (gph->exec) (void)hipGraphExecDestroy(gph->exec);
+  delete gph;
   return ITG_OK;
 }
 
@@ -189,6 +282,11 @@ const char* itg_last_error(const itg_handle* h) { return h ? h->err.c_str() : "n
 
 itg_status itg_set_stream(itg_handle* h, void* stream) {
   if (!h) return ITG_ERR_INVALID;
+  if (h->capturing) {
+    // naming the stream the capture started from keeps the capture's stream
+    if ((hipStream_t)stream == h->pre_capture) return ITG_OK;
+    return fail(h, ITG_ERR_INVALID, "cannot change the stream while a capture is open");
+  }
   h->stream = (hipStream_t)stream;
   return ITG_OK;
 }
@@ -768,15 +866,21 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* 
       pa[2 * idx[e]] = pair_alpha[2 * e];
       pa[2 * idx[e] + 1] = cplx ? pair_alpha[2 * e + 1] : 0.0;
     }
-    if (!p->ep_alpha || p->ep_alpha_key != pa) {
-      (void)hipStreamSynchronize(h->stream);
-      p->ep_alpha.reset(new ExecPlan);
+    auto it = p->ep_alpha.find(pa);
+    if (it == p->ep_alpha.end()) {
+      // building uploads tables on the stream, which a capture would record
+      // with host sources that do not outlive the call
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return fail(h, ITG_ERR_INVALID,
+                    "new per-pair factors during stream capture: execute once before capturing");
+      std::unique_ptr<ExecPlan> np(new ExecPlan);
       std::string err =
-          build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, pa.data(), h->stream, p->ep_alpha.get());
+          build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, pa.data(), h->stream, np.get());
       if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
-      p->ep_alpha_key = pa;
+      it = p->ep_alpha.emplace(pa, std::move(np)).first;
     }
-    ep = p->ep_alpha.get();
+    ep = it->second.get();
   }
   return run_plan(h, *ep, cplx, (int)prec, T1, T2, R, flags, alpha, nullptr);
 }
@@ -836,15 +940,17 @@ itg_status itg_bs_chain_destroy(itg_bs_chain* c) {
   return ITG_OK;
 }
 
-itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
-                          const int64_t* blockdims, const int64_t* offs_src,
-                          const int64_t* offs_dst, const int32_t* perm, const void* src,
-                          void* dst) {
+itg_status itg_bs_permute_ex(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
+                             const int64_t* blockdims, const int64_t* offs_src,
+                             const int64_t* offs_dst, const uint8_t* fresh, const int32_t* perm,
+                             const void* src, void* dst, const double* alpha, const double* beta,
+                             int32_t* nonzero) {
   if (!h) return ITG_ERR_INVALID;
   if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (N < 0 || N > kMaxModes) return fail(h, ITG_ERR_UNSUPPORTED, "too many modes");
+  if (nblk < 0) return fail(h, ITG_ERR_INVALID, "negative block count");
   if (nblk == 0) return ITG_OK;
-  if (!blockdims || !offs_src || !offs_dst || (N > 0 && !perm) || !src || !dst)
+  if (!blockdims || !offs_src || !offs_dst || (N > 0 && !perm) || !dst)
     return fail(h, ITG_ERR_INVALID, "null argument");
   int p0[kMaxModes];
   bool seen[kMaxModes] = {false};
@@ -856,6 +962,7 @@ itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
   }
   BlockList bl;
   bl.N = N;
+  bool any_src = false;
   for (int64_t b = 0; b < nblk; ++b) {
     int64_t sz = 1;
     for (int i = 0; i < N; ++i) {
@@ -865,21 +972,55 @@ itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
     }
     if (sz > INT32_MAX) return fail(h, ITG_ERR_UNSUPPORTED, "block too large");
     if (N == 0) bl.dims.push_back(1);
+    if (offs_dst[b] < 0) return fail(h, ITG_ERR_INVALID, "negative destination offset");
+    any_src |= offs_src[b] >= 0;
   }
+  if (any_src && !src) return fail(h, ITG_ERR_INVALID, "null source");
   bl.offs.assign(offs_src, offs_src + nblk);
   std::vector<int64_t> od(offs_dst, offs_dst + nblk);
+  std::vector<uint8_t> fr;
+  if (fresh) fr.assign(fresh, fresh + nblk);
   DeviceGuard g(h->device);
   PermuteLaunch pl;
-  hipError_t e = build_permute_launch(&pl, N, p0, bl, &od, h->stream);
+  hipError_t e = build_permute_launch(&pl, N, p0, bl, &od, h->stream, fresh ? &fr : nullptr);
   if (e != hipSuccess) return hip_fail(h, e, "permute tables");
+  DevBuf nz;
+  if (nonzero) {
+    std::vector<int32_t> zero((size_t)nblk, 0);
+    if ((e = nz.upload(zero.data(), zero.size() * sizeof(int32_t), h->stream)) != hipSuccess)
+      return hip_fail(h, e, "flag buffer");
+  }
   const int cplx = dtype == ITG_C128;
+  const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
+  const double b_re = beta ? beta[0] : 0.0, b_im = (beta && cplx) ? beta[1] : 0.0;
+  KernelTimer* kt = active_timer(h);
+  if (kt) kt->begin("permute_kernel", h->stream);
   e = launch_permute(h->stream, cplx, pl.nm, pl.perm, (const PermBlock*)pl.blocks.p,
-                     (const int64_t*)pl.prefix.p, pl.nblk, pl.tiles, pl.kind, src, dst, 1.0, 0.0,
-                     0.0, 0.0);
+                     (const int64_t*)pl.prefix.p, pl.nblk, pl.tiles, pl.kind, src, dst, a_re,
+                     a_im, b_re, b_im, nonzero ? (int32_t*)nz.p : nullptr);
+  if (kt) kt->end(h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "permute launch");
+  if (nonzero) {
+    e = hipMemcpyAsync(nonzero, nz.p, (size_t)nblk * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "flag download");
+  }
+  // the tables (and the flags) are freed when the locals go out of scope
   e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return hip_fail(h, e, "permute");
   return ITG_OK;
+}
+
+itg_status itg_bs_permute(itg_handle* h, itg_dtype dtype, int N, int64_t nblk,
+                          const int64_t* blockdims, const int64_t* offs_src,
+                          const int64_t* offs_dst, const int32_t* perm, const void* src,
+                          void* dst) {
+  if (!h) return ITG_ERR_INVALID;
+  if (nblk > 0 && offs_src)
+    for (int64_t b = 0; b < nblk; ++b)
+      if (offs_src[b] < 0) return fail(h, ITG_ERR_INVALID, "negative source offset");
+  return itg_bs_permute_ex(h, dtype, N, nblk, blockdims, offs_src, offs_dst, nullptr, perm, src,
+                           dst, nullptr, nullptr, nullptr);
 }
 
 itg_status itg_bs_copy(itg_handle* h, itg_dtype dtype, int64_t nblk, int nmodes,

@@ -18,14 +18,30 @@ constexpr int kMaxKModes = 8;
 // ------------------------------------------------------------ descriptors
 // Column-major tensors; offsets and strides are in ELEMENTS of the dtype.
 
-// One block of a grouped permutation: dst = alpha*perm(src) + beta*dst.
-// dims are the src block's dims after mode merging (nm modes, uniform per
-// launch); the permutation itself is a launch argument.
+// One block of a grouped permutation: dst = alpha*perm(src) + beta*dst,
+// described in the DST block's mode order (dst mode 0 fastest) after mode
+// merging (nm modes, uniform per launch): the dst dims with their division
+// constants and the src stride of every dst mode, all computed on the host,
+// so the kernels walk the modes with compile-time indices only.
 struct PermBlock {
-  int64_t off_src;
+  int64_t off_src;            // < 0: no source block (its values are zero)
   int64_t off_dst;
-  int32_t dims[kMaxModes];
+  int64_t sstr[kMaxModes];    // src stride of dst mode i
+  uint32_t ddim[kMaxModes];   // dst dims
+  uint32_t dmul[kMaxModes];   // x / ddim[i] == (umulhi(x, dmul[i]) + x) >> dshr[i], x < 2^31
+  uint32_t dshr[kMaxModes];
+  int32_t nelem;
+  int32_t fresh;              // 1: the dst block is new (written, never read)
 };
+// Division constants of d >= 1 for dividends below 2^31 (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994,
+// fig. 4.1): q = (umulhi(x, mul) + x) >> shr.
+inline void fastdiv_consts(uint32_t d, uint32_t* mul, uint32_t* shr) {
+  uint32_t l = 0;
+  while (l < 32 && (uint64_t(1) << l) < d) ++l;
+  *shr = l;
+  *mul = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+}
 
 // Small-K direct contraction: every output element of a group (output block)
 // sums, over the group's pairs in plan order, alpha_p * sum_k A[..]*B[..].
@@ -111,17 +127,23 @@ inline hipError_t launch_kernel(void (*kernel)(KArgs...), dim3 grid, dim3 block,
 // All launchers enqueue on `stream` and return hipSuccess or the launch error.
 // `esize` is 1 for Float64 and 2 for ComplexF64.
 
+// Element types of the data-movement kernels.
+enum EltCode { kEltF64 = 0, kEltC128 = 1, kEltF32 = 2, kEltC64 = 3 };
 // Grouped permutation. perm[i] = src mode feeding dst mode i (0-based).
-// tile_prefix has nblk+1 entries (workgroups before block b).
-hipError_t launch_permute(hipStream_t s, int cplx, int nm, const int32_t* perm_host,
+// tile_prefix has nblk+1 entries (workgroups before block b). elt: EltCode
+// (0 / 1 = the cplx flag of the FP64 paths). nzflags (optional, nblk ints,
+// cleared by the launcher): set to 1 for every block with a written value
+// that is not zero.
+hipError_t launch_permute(hipStream_t s, int elt, int nm, const int32_t* perm_host,
                           const PermBlock* blocks_dev, const int64_t* tile_prefix_dev,
                           int64_t nblk, int64_t total_tiles, int mode_kind,
                           const void* src, void* dst, double a_re, double a_im,
-                          double b_re, double b_im);
+                          double b_re, double b_im, int32_t* nzflags = nullptr);
 // Number of workgroups a block of the permutation needs (host helper that
-// must agree with the kernel's tiling). mode_kind: 0 = inner mode kept,
-// 1 = 2-D tile transpose of src mode 0 and src mode perm[0].
-int64_t permute_block_tiles(int nm, const int32_t* perm, const int32_t* dims, int mode_kind);
+// must agree with the kernel's tiling). ddim: the block's dst dims; q: the
+// dst mode fed by src mode 0. mode_kind: 0 = src mode 0 stays dst mode 0
+// (streamed), 1 = 2-D tile transpose of dst mode 0 and dst mode q.
+int64_t permute_block_tiles(int nm, int q, const uint32_t* ddim, int mode_kind);
 
 // One block of a grouped strided copy (element offsets and strides; modes
 // in traversal order, mode 0 fastest; nelem = product of dims < 2^31).
@@ -324,7 +346,7 @@ struct SplitPair {
   int64_t a_pstride, b_pstride;  // bytes between consecutive slice planes
   int32_t a_exp, b_exp;          // first row exponent of the A block / B block
   int32_t kpad, pad;
-  double alpha;
+  double alpha, alpha_im;        // per-pair factor (the kernel takes one part per launch)
 };
 
 int split_slices(int prec);  // slices per value for a precision mode (0: none)
@@ -350,12 +372,18 @@ hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const Slic
 // packs the slice sets (0 re, 1 im, 2 -im) of A and B for the two K halves:
 // bits 0-1 A half 0, 2-3 B half 0, 4-5 A half 1, 6-7 B half 1. Both complex
 // operands are sliced with 3 sets.
+// aim: take each pair's factor from alpha_im instead of alpha (the second
+// pass of a product with complex per-pair factors).
 hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int comp, int sets,
                              const SplitPair* pairs, const GemmGroup* groups, const int2* tiles,
                              int64_t grid, const void* planesA, const void* planesB,
                              const void* expA, const void* expB, void* C, double ga,
-                             double beta);
+                             double beta, int aim = 0);
 // x[i] = beta * x[i] for i < n (complex when cplx).
 hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re, double b_im);
+// Complex y[i] = alpha * (x[i] + i * x2[i]) + beta * y[i] for i < n (x2 may
+// be null: taken as 0; beta == 0 does not read y).
+hipError_t launch_axpby_join(hipStream_t s, int64_t n, double a_re, double a_im, const void* x,
+                             const void* x2, double b_re, double b_im, void* y);
 
 }  // namespace itg

@@ -161,7 +161,8 @@ Merged merge_modes(int n, const int* perm) {
 }
 
 hipError_t build_permute(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
-                         const std::vector<int64_t>* dst_offs, hipStream_t s) {
+                         const std::vector<int64_t>* dst_offs, hipStream_t s,
+                         const std::vector<uint8_t>* fresh = nullptr) {
   Merged m = merge_modes(n, perm);
   if (m.nm == 0) {  // 0-dim tensor: one mode of extent 1
     m.nm = 1;
@@ -172,20 +173,38 @@ hipError_t build_permute(PermuteLaunch* pl, int n, const int* perm, const BlockL
   pl->nm = m.nm;
   for (int i = 0; i < kMaxModes; ++i) pl->perm[i] = i < m.nm ? m.perm[i] : 0;
   pl->kind = (m.nm <= 1 || m.perm[0] == 0) ? 0 : 1;
+  int q = 0;
+  for (int i = 0; i < m.nm; ++i)
+    if (m.perm[i] == 0) q = i;
   std::vector<PermBlock> blocks(bl.nblk());
   std::vector<int64_t> prefix(bl.nblk() + 1, 0);
   for (int64_t b = 0; b < bl.nblk(); ++b) {
     PermBlock& pb = blocks[b];
+    std::memset(&pb, 0, sizeof(pb));
     pb.off_src = bl.offs[b];
     pb.off_dst = dst_offs ? (*dst_offs)[b] : bl.offs[b];
-    for (int i = 0; i < kMaxModes; ++i) pb.dims[i] = 1;
+    pb.fresh = fresh ? (*fresh)[b] : 0;
     const int32_t* d = bl.d(b);
-    for (int i = 0; i < m.nm; ++i) {
+    int64_t sdim[kMaxModes], sstr[kMaxModes];
+    int64_t acc = 1;
+    for (int i = 0; i < m.nm; ++i) {  // merged src modes
       int64_t prod = 1;
       for (int k = m.first[i]; k <= m.last[i]; ++k) prod *= d[k];
-      pb.dims[i] = (int32_t)prod;
+      sdim[i] = prod;
+      sstr[i] = acc;
+      acc *= prod;
     }
-    prefix[b + 1] = prefix[b] + permute_block_tiles(m.nm, pl->perm, pb.dims, pl->kind);
+    pb.nelem = (int32_t)acc;
+    for (int i = 0; i < kMaxModes; ++i) {
+      if (i < m.nm) {
+        pb.ddim[i] = (uint32_t)sdim[m.perm[i]];
+        pb.sstr[i] = sstr[m.perm[i]];
+      } else {
+        pb.ddim[i] = 1;
+      }
+      fastdiv_consts(std::max<uint32_t>(pb.ddim[i], 1), &pb.dmul[i], &pb.dshr[i]);
+    }
+    prefix[b + 1] = prefix[b] + permute_block_tiles(m.nm, q, pb.ddim, pl->kind);
   }
   pl->nblk = bl.nblk();
   pl->tiles = prefix.back();
@@ -630,8 +649,9 @@ bool build_fiber_tables(const Structure& st, const BlockList& b1, const BlockLis
 }  // namespace
 
 hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
-                                const std::vector<int64_t>* dst_offs, hipStream_t s) {
-  return build_permute(pl, n, perm, bl, dst_offs, s);
+                                const std::vector<int64_t>* dst_offs, hipStream_t s,
+                                const std::vector<uint8_t>* fresh) {
+  return build_permute(pl, n, perm, bl, dst_offs, s, fresh);
 }
 
 // ----------------------------------------------------------- exec plan
@@ -1055,6 +1075,7 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
     x.b_exp = (int32_t)dB[bq].exp_off;
     x.kpad = dA[bp].K == 0 ? 0 : dA[bp].Kpad;
     x.alpha = ep->h_pairs[p].alpha_re;
+    x.alpha_im = ep->h_pairs[p].alpha_im;
     if (ep->h_pairs[p].alpha_im != 0.0) ep->cplx_pair_alpha = true;
     ep->split_kmax = std::max<int64_t>(ep->split_kmax, dA[bp].K);
   }
@@ -1191,7 +1212,13 @@ namespace {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 }  // namespace
 
-size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec) {
+// Complex split products whose scale is not real (a complex alpha or
+// complex per-pair factors) are formed in separate buffers and combined.
+static bool split_cplx_scale(const ExecPlan& ep, int cplx, bool alpha_cplx) {
+  return cplx && (alpha_cplx || ep.cplx_pair_alpha);
+}
+
+size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec, bool alpha_cplx) {
   if (ep.strategy != ExecPlan::GEMM) return 0;
   const size_t es = cplx ? 16 : 8;
   size_t b = 0;
@@ -1200,6 +1227,12 @@ size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec) {
     b += align256((size_t)ep.sA_plane_bytes * ns) + align256((size_t)ep.sB_plane_bytes * ns);
     b += align256((size_t)ep.sA_rows * 4) + align256((size_t)ep.sB_rows * 4);
     b += align256((size_t)ep.sA_pmax * 4) + align256((size_t)ep.sB_pmax * 4);
+    if (split_cplx_scale(ep, cplx, alpha_cplx)) {
+      // sum of the real-factor parts, then (complex pair factors) the
+      // imaginary-factor parts, each in C's (pre-permutation) layout
+      b += align256((size_t)ep.nnzC * es) * (ep.cplx_pair_alpha ? 2 : 1);
+      if (ep.permC) return b + 1024;  // the combined sum is permuted into R from there
+    }
   } else {
     if (ep.permA) b += align256((size_t)ep.nnzA * es);
     if (ep.permB) b += align256((size_t)ep.nnzB * es);
@@ -1247,10 +1280,6 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
   }
   if (split_slices(prec) == 0) {
     *err = "this split-precision form is not available";
-    return hipErrorNotSupported;
-  }
-  if (cplx && (a_im != 0.0 || ep.cplx_pair_alpha)) {
-    *err = "complex split precision takes a real alpha";
     return hipErrorNotSupported;
   }
   if (cplx && ep.split_kmax > 32768) {  // the two K halves share one INT32 sum
@@ -1303,6 +1332,47 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
     return e;
   }
   GemmLaunch& gl = *ep.launches[tcfg];
+  if (split_cplx_scale(ep, cplx, a_im != 0.0)) {
+    // R = alpha * sum_p (a_p + i b_p) P_p + beta * R: the split kernel scales
+    // each pair by a real factor, so form S1 = sum_p a_p P_p and S2 =
+    // sum_p b_p P_p (unit global scale), then R = alpha (S1 + i S2) + beta R
+    // (through the output permutation when there is one)
+    const size_t cbytes = align256((size_t)ep.nnzC * 16);
+    void* S1 = w;
+    void* S2 = ep.cplx_pair_alpha ? (void*)(w + cbytes) : nullptr;
+    for (int part = 0; part < (S2 ? 2 : 1) && e == hipSuccess; ++part)
+      for (int comp = 0; comp < 2 && e == hipSuccess; ++comp) {
+        TimeScope ts(kt, "split_gemm_kernel", s);
+        e = launch_split_gemm(s, prec, cplx, wmb, comp, split_sets(comp, conja, conjb),
+                              (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.g_groups.p,
+                              (const int2*)gl.tiles.p, gl.grid, planesA, planesB, expA, expB,
+                              part ? S2 : S1, 1.0, 0.0, part);
+      }
+    if (e != hipSuccess) {
+      *err = std::string("split gemm launch: ") + hipGetErrorString(e);
+      return e;
+    }
+    if (ep.permC) {
+      if (S2) {
+        TimeScope ts(kt, "axpby_join_kernel", s);
+        e = launch_axpby_join(s, ep.nnzC, 1.0, 0.0, S1, S2, 0.0, 0.0, S1);
+        if (e != hipSuccess) {
+          *err = std::string("axpby launch: ") + hipGetErrorString(e);
+          return e;
+        }
+      }
+      TimeScope ts(kt, "permute_kernel", s);
+      e = launch_permute(s, cplx, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,
+                         (const int64_t*)ep.pC.prefix.p, ep.pC.nblk, ep.pC.tiles, ep.pC.kind, S1,
+                         R, a_re, a_im, b_re, b_im);
+      if (e != hipSuccess) *err = std::string("permute launch: ") + hipGetErrorString(e);
+      return e;
+    }
+    TimeScope ts(kt, "axpby_join_kernel", s);
+    e = launch_axpby_join(s, ep.nnzC, a_re, a_im, S1, S2, b_re, b_im, R);
+    if (e != hipSuccess) *err = std::string("axpby launch: ") + hipGetErrorString(e);
+    return e;
+  }
   void* C = ep.permC ? (void*)w : R;
   double beta_k = ep.permC ? 0.0 : b_re;
   if (cplx && !ep.permC && b_im != 0.0) {

@@ -172,12 +172,16 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
                                const std::vector<PairSpec>& pairs, bool swap, hipStream_t s);
 
 // Build the device tables of a grouped permutation (dst offsets default to
-// the src offsets). perm: new mode i = old mode perm[i], 0-based.
+// the src offsets; a negative src offset marks an absent source block, whose
+// values are zero). perm: new mode i = old mode perm[i], 0-based. fresh
+// (optional, per block): 1 = the dst block is new, written without being read.
 hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const BlockList& bl,
-                                const std::vector<int64_t>* dst_offs, hipStream_t s);
+                                const std::vector<int64_t>* dst_offs, hipStream_t s,
+                                const std::vector<uint8_t>* fresh = nullptr);
 
 // Launch the plan. ws: workspace of at least exec_workspace_bytes().
-size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec);
+// alpha_cplx: the call's alpha has a nonzero imaginary part.
+size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec, bool alpha_cplx = false);
 // kt (optional) brackets each launch with timer events.
 hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                     const void* T1, const void* T2, void* R, unsigned flags, double a_re,

@@ -25,11 +25,6 @@ constexpr int kElemsPerThread = 4;
 constexpr int kElemsPerWG = kThreads * kElemsPerThread;
 constexpr int kTile = 32;
 
-struct PermArgs {
-  int nm;
-  int32_t perm[kMaxModes];
-};
-
 template <int CPLX>
 struct Elem;
 template <>
@@ -58,6 +53,20 @@ __device__ inline double2 axpby(double a_re, double a_im, double2 x, double b_re
   }
   return r;
 }
+// Float32 / ComplexF32: formed in FP64, rounded once on the store.
+__device__ inline float axpby(double a_re, double a_im, float x, double b_re, double b_im,
+                              float y, bool use_beta) {
+  return (float)axpby(a_re, a_im, (double)x, b_re, b_im, (double)y, use_beta);
+}
+__device__ inline float2 axpby(double a_re, double a_im, float2 x, double b_re, double b_im,
+                               float2 y, bool use_beta) {
+  const double2 r = axpby(a_re, a_im, double2{x.x, x.y}, b_re, b_im, double2{y.x, y.y}, use_beta);
+  return float2{(float)r.x, (float)r.y};
+}
+__device__ inline bool nonzero(double x) { return x != 0.0; }
+__device__ inline bool nonzero(double2 x) { return x.x != 0.0 || x.y != 0.0; }
+__device__ inline bool nonzero(float x) { return x != 0.0f; }
+__device__ inline bool nonzero(float2 x) { return x.x != 0.0f || x.y != 0.0f; }
 
 __device__ inline int64_t find_block(const int64_t* prefix, int64_t nblk, int64_t wg) {
   // largest b with prefix[b] <= wg
@@ -72,120 +81,159 @@ __device__ inline int64_t find_block(const int64_t* prefix, int64_t nblk, int64_
   return lo;
 }
 
-template <int CPLX>
-__global__ void __launch_bounds__(kThreads)
-    permute_inner_kernel(PermArgs pa, const PermBlock* __restrict__ blocks,
-                         const int64_t* __restrict__ prefix, int64_t nblk,
-                         const typename Elem<CPLX>::T* __restrict__ src,
-                         typename Elem<CPLX>::T* __restrict__ dst, double a_re, double a_im,
-                         double b_re, double b_im) {
-  using T = typename Elem<CPLX>::T;
-  const int64_t wg = blockIdx.x;
-  const int64_t b = find_block(prefix, nblk, wg);
-  const PermBlock blk = blocks[b];
-  const int nm = pa.nm;
-  // src strides and dst dims
-  int64_t sstr[kMaxModes];
-  int64_t ddim[kMaxModes];
-  int64_t nelem = 1;
-  {
-    int64_t st = 1;
-    for (int i = 0; i < nm; ++i) {
-      sstr[i] = st;
-      st *= blk.dims[i];
-    }
-    nelem = st;
-    for (int i = 0; i < nm; ++i) ddim[i] = blk.dims[pa.perm[i]];
-  }
-  const bool use_beta = (b_re != 0.0 || b_im != 0.0);
-  const int64_t e0 = (wg - prefix[b]) * kElemsPerWG;
-  const int64_t d0 = ddim[0];
-  for (int r = 0; r < kElemsPerThread; ++r) {
-    const int64_t e = e0 + r * kThreads + threadIdx.x;
-    if (e >= nelem) break;
-    int64_t rem = e / d0;
-    int64_t so = (e - rem * d0);  // mode 0 of dst == mode 0 of src (stride 1)
-    for (int i = 1; i < nm; ++i) {
-      const int64_t di = ddim[i];
-      const int64_t q = rem / di;
-      so += (rem - q * di) * sstr[pa.perm[i]];
-      rem = q;
-    }
-    T x = src[blk.off_src + so];
-    T y{};
-    if (use_beta) y = dst[blk.off_dst + e];
-    dst[blk.off_dst + e] = axpby(a_re, a_im, x, b_re, b_im, y, use_beta);
-  }
+__device__ inline uint32_t fdiv(uint32_t x, uint32_t mul, uint32_t shr) {
+  return (__umulhi(x, mul) + x) >> shr;
 }
 
-template <int CPLX>
+// Src offset of dst element e of a block (dst modes 0..NM-1, mode 0 fastest).
+template <int NM>
+__device__ inline int64_t src_offset(const PermBlock& blk, uint32_t e) {
+  int64_t so = 0;
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    if (i + 1 < NM) {
+      const uint32_t q = fdiv(e, blk.dmul[i], blk.dshr[i]);
+      so += (int64_t)(e - q * blk.ddim[i]) * blk.sstr[i];
+      e = q;
+    } else {
+      so += (int64_t)e * blk.sstr[i];
+    }
+  }
+  return so;
+}
+
+// Block-level "some written value is not zero" flag.
+__device__ inline void mark_nonzero(int32_t* nzflags, int64_t b, bool nz) {
+  if (!nzflags) return;
+  if (__syncthreads_or(nz ? 1 : 0) && threadIdx.x == 0) nzflags[b] = 1;
+}
+
+// Kind 0: src mode 0 stays the fastest dst mode. A workgroup streams
+// kElemsPerWG consecutive dst elements of one block: the four loads of a
+// thread are issued back to back (offsets clamped into the block), then the
+// stores; reads and writes run along mode 0.
+template <typename T, int NM>
 __global__ void __launch_bounds__(kThreads)
-    permute_tile_kernel(PermArgs pa, const PermBlock* __restrict__ blocks,
-                        const int64_t* __restrict__ prefix, int64_t nblk,
-                        const typename Elem<CPLX>::T* __restrict__ src,
-                        typename Elem<CPLX>::T* __restrict__ dst, double a_re, double a_im,
-                        double b_re, double b_im) {
-  using T = typename Elem<CPLX>::T;
+    permute_inner_kernel(const PermBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
+                         int64_t nblk, const T* __restrict__ src, T* __restrict__ dst,
+                         double a_re, double a_im, double b_re, double b_im,
+                         int32_t* __restrict__ nzflags) {
+  const int64_t wg = blockIdx.x;
+  const int64_t b = find_block(prefix, nblk, wg);
+  const PermBlock& blk = blocks[b];
+  const int64_t off_src = blk.off_src, off_dst = blk.off_dst;
+  const uint32_t n = (uint32_t)blk.nelem;
+  const bool has_src = off_src >= 0;
+  const bool use_beta = (b_re != 0.0 || b_im != 0.0) && !blk.fresh;
+  const uint32_t e0 = (uint32_t)(wg - prefix[b]) * kElemsPerWG + threadIdx.x;
+  T x[kElemsPerThread], y[kElemsPerThread];
+#pragma unroll
+  for (int r = 0; r < kElemsPerThread; ++r) {
+    const uint32_t e = min(e0 + r * kThreads, n - 1);
+    x[r] = has_src ? src[off_src + src_offset<NM>(blk, e)] : T{};
+    y[r] = use_beta ? dst[off_dst + e] : T{};
+  }
+  bool nz = false;
+#pragma unroll
+  for (int r = 0; r < kElemsPerThread; ++r) {
+    const uint32_t e = e0 + r * kThreads;
+    if (e < n) {
+      const T v = axpby(a_re, a_im, x[r], b_re, b_im, y[r], use_beta);
+      nz |= nonzero(v);
+      dst[off_dst + e] = v;
+    }
+  }
+  mark_nonzero(nzflags, b, nz);
+}
+
+// Kind 1: dst mode 0 (src mode p) and dst mode q (src mode 0) form 32x32
+// tiles staged through LDS: read along src mode 0, written along dst mode 0,
+// both coalesced; the other dst modes are a batch index.
+template <typename T, int NM>
+__global__ void __launch_bounds__(kThreads)
+    permute_tile_kernel(const PermBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
+                        int64_t nblk, int q, const T* __restrict__ src, T* __restrict__ dst,
+                        double a_re, double a_im, double b_re, double b_im,
+                        int32_t* __restrict__ nzflags) {
   __shared__ T tile[kTile][kTile + 1];
   const int64_t wg = blockIdx.x;
   const int64_t b = find_block(prefix, nblk, wg);
-  const PermBlock blk = blocks[b];
-  const int nm = pa.nm;
-  const int p = pa.perm[0];  // src mode that becomes dst's fastest mode
-  int64_t sstr[kMaxModes];
-  int64_t dstr_of_src[kMaxModes];  // dst stride of each src mode
+  const PermBlock& blk = blocks[b];
+  // dst strides (column-major over ddim); dst mode q is src mode 0
+  int64_t dstr[NM];
   {
-    int64_t st = 1;
-    for (int i = 0; i < nm; ++i) {
-      sstr[i] = st;
-      st *= blk.dims[i];
-    }
-    int64_t dt = 1;
-    for (int i = 0; i < nm; ++i) {
-      dstr_of_src[pa.perm[i]] = dt;
-      dt *= blk.dims[pa.perm[i]];
+    int64_t acc = 1;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+      dstr[i] = acc;
+      acc *= blk.ddim[i];
     }
   }
-  const int64_t D0 = blk.dims[0];
-  const int64_t Dp = blk.dims[p];
-  const int64_t ti_n = (D0 + kTile - 1) / kTile;
-  const int64_t tj_n = (Dp + kTile - 1) / kTile;
-  int64_t t = wg - prefix[b];
-  const int64_t ti = t % ti_n;
-  t /= ti_n;
-  const int64_t tj = t % tj_n;
-  int64_t bidx = t / tj_n;
-  // batch coordinates: every src mode except 0 and p, in src order
+  uint32_t Dq = 1, Mq = 1, Sq = 0;
+  int64_t dstr_q = 1;
+#pragma unroll
+  for (int i = 1; i < NM; ++i)
+    if (i == q) {
+      Dq = blk.ddim[i];
+      Mq = blk.dmul[i];
+      Sq = blk.dshr[i];
+      dstr_q = dstr[i];
+    }
+  (void)Mq;
+  (void)Sq;
+  const uint32_t D0 = blk.ddim[0];
+  const int64_t sstr0 = blk.sstr[0];  // src stride of dst mode 0 (src mode p)
+  const uint32_t tq_n = (Dq + kTile - 1) / kTile, t0_n = (D0 + kTile - 1) / kTile;
+  uint32_t t = (uint32_t)(wg - prefix[b]);
+  const uint32_t tq = t % tq_n;
+  t /= tq_n;
+  const uint32_t t0 = t % t0_n;
+  uint32_t bidx = t / t0_n;
+  // batch coordinates: every dst mode except 0 and q
   int64_t sbase = 0, dbase = 0;
-  for (int i = 1; i < nm; ++i) {
-    if (i == p) continue;
-    const int64_t di = blk.dims[i];
-    const int64_t q = bidx / di;
-    const int64_t c = bidx - q * di;
-    sbase += c * sstr[i];
-    dbase += c * dstr_of_src[i];
-    bidx = q;
+#pragma unroll
+  for (int i = 1; i < NM; ++i) {
+    if (i == q) continue;
+    const uint32_t qq = fdiv(bidx, blk.dmul[i], blk.dshr[i]);
+    const uint32_t c = bidx - qq * blk.ddim[i];
+    sbase += (int64_t)c * blk.sstr[i];
+    dbase += (int64_t)c * dstr[i];
+    bidx = qq;
   }
-  const int64_t i0 = ti * kTile, j0 = tj * kTile;
+  const uint32_t iq0 = tq * kTile, j0 = t0 * kTile;
   const int lx = threadIdx.x % kTile;
   const int ly = threadIdx.x / kTile;  // 0..7
-  // read: i along src mode 0 (contiguous)
-  for (int r = 0; r < kTile; r += kThreads / kTile) {
-    const int64_t i = i0 + lx, j = j0 + ly + r;
-    if (i < D0 && j < Dp) tile[ly + r][lx] = src[blk.off_src + sbase + i + j * sstr[p]];
-  }
-  __syncthreads();
-  const bool use_beta = (b_re != 0.0 || b_im != 0.0);
-  const int64_t dstr0 = dstr_of_src[0];
-  for (int r = 0; r < kTile; r += kThreads / kTile) {
-    const int64_t j = j0 + lx, i = i0 + ly + r;
-    if (i < D0 && j < Dp) {
-      const int64_t o = blk.off_dst + dbase + j + i * dstr0;
-      T y{};
-      if (use_beta) y = dst[o];
-      dst[o] = axpby(a_re, a_im, tile[lx][ly + r], b_re, b_im, y, use_beta);
+  const bool has_src = blk.off_src >= 0;
+  // read: iq (dst mode q = src mode 0, contiguous in src) along lanes
+  if (has_src) {
+#pragma unroll
+    for (int r = 0; r < kTile; r += kThreads / kTile) {
+      const uint32_t iq = iq0 + lx, j = j0 + ly + r;
+      if (iq < Dq && j < D0) tile[ly + r][lx] = src[blk.off_src + sbase + iq + j * sstr0];
     }
   }
+  __syncthreads();
+  const bool use_beta = (b_re != 0.0 || b_im != 0.0) && !blk.fresh;
+  bool nz = false;
+#pragma unroll
+  for (int r = 0; r < kTile; r += kThreads / kTile) {
+    const uint32_t j = j0 + lx, iq = iq0 + ly + r;
+    if (iq < Dq && j < D0) {
+      const int64_t o = blk.off_dst + dbase + j + iq * dstr_q;
+      T y{};
+      if (use_beta) y = dst[o];
+      const T x = has_src ? tile[lx][ly + r] : T{};
+      const T v = axpby(a_re, a_im, x, b_re, b_im, y, use_beta);
+      nz |= nonzero(v);
+      dst[o] = v;
+    }
+  }
+  mark_nonzero(nzflags, b, nz);
+}
+
+__global__ void __launch_bounds__(kThreads) fill_i32_kernel(int32_t* __restrict__ x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) x[i] = 0;
 }
 
 template <int CPLX>
@@ -195,6 +243,25 @@ __global__ void __launch_bounds__(kThreads)
   if (i >= n) return;
   typename Elem<CPLX>::T y{};
   x[i] = axpby(b_re, b_im, x[i], 0.0, 0.0, y, false);
+}
+
+// y = alpha * (x + i*x2) + beta * y over complex elements; x2 may be null.
+__global__ void __launch_bounds__(kThreads)
+    axpby_join_kernel(const double2* __restrict__ x, const double2* __restrict__ x2,
+                      double2* __restrict__ y, int64_t n, double a_re, double a_im, double b_re,
+                      double b_im) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  double2 v = x[i];
+  if (x2) {
+    const double2 w = x2[i];
+    v.x -= w.y;
+    v.y += w.x;
+  }
+  const bool use_beta = b_re != 0.0 || b_im != 0.0;
+  double2 old{};
+  if (use_beta) old = y[i];
+  y[i] = axpby(a_re, a_im, v, b_re, b_im, old, use_beta);
 }
 
 // Grouped strided copy: dst[off_dst + sum_m x_m dstr[m]] = src[off_src +
@@ -243,44 +310,98 @@ hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re
   return hipGetLastError();
 }
 
-int64_t permute_block_tiles(int nm, const int32_t* perm, const int32_t* dims, int mode_kind) {
-  int64_t nelem = 1;
-  for (int i = 0; i < nm; ++i) nelem *= dims[i];
-  if (nelem == 0) return 0;
-  if (mode_kind == 0) return (nelem + kElemsPerWG - 1) / kElemsPerWG;
-  const int p = perm[0];
-  const int64_t D0 = dims[0], Dp = dims[p];
-  const int64_t nb = nelem / (D0 * Dp);
-  return ((D0 + kTile - 1) / kTile) * ((Dp + kTile - 1) / kTile) * nb;
+hipError_t launch_axpby_join(hipStream_t s, int64_t n, double a_re, double a_im, const void* x,
+                             const void* x2, double b_re, double b_im, void* y) {
+  if (n <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + kThreads - 1) / kThreads)), block(kThreads);
+  ITG_LAUNCH(axpby_join_kernel, grid, block, 0, s, (const double2*)x, (const double2*)x2,
+             (double2*)y, n, a_re, a_im, b_re, b_im);
+  return hipGetLastError();
 }
 
-hipError_t launch_permute(hipStream_t s, int cplx, int nm, const int32_t* perm_host,
+int64_t permute_block_tiles(int nm, int q, const uint32_t* ddim, int mode_kind) {
+  int64_t nelem = 1;
+  for (int i = 0; i < nm; ++i) nelem *= ddim[i];
+  if (nelem == 0) return 0;
+  if (mode_kind == 0) return (nelem + kElemsPerWG - 1) / kElemsPerWG;
+  const int64_t D0 = ddim[0], Dq = ddim[q];
+  const int64_t nb = nelem / (D0 * Dq);
+  return ((D0 + kTile - 1) / kTile) * ((Dq + kTile - 1) / kTile) * nb;
+}
+
+namespace {
+template <typename T, int NM>
+void launch_perm_nm(hipStream_t s, int kind, int q, dim3 grid, const PermBlock* blocks,
+                    const int64_t* prefix, int64_t nblk, const void* src, void* dst, double a_re,
+                    double a_im, double b_re, double b_im, int32_t* nz) {
+  if (kind == 0)
+    ITG_LAUNCH((permute_inner_kernel<T, NM>), grid, kThreads, 0, s, blocks, prefix, nblk,
+               (const T*)src, (T*)dst, a_re, a_im, b_re, b_im, nz);
+  else
+    ITG_LAUNCH((permute_tile_kernel<T, NM>), grid, kThreads, 0, s, blocks, prefix, nblk, q,
+               (const T*)src, (T*)dst, a_re, a_im, b_re, b_im, nz);
+}
+
+template <typename T>
+void launch_perm_t(hipStream_t s, int nm, int kind, int q, dim3 grid, const PermBlock* blocks,
+                   const int64_t* prefix, int64_t nblk, const void* src, void* dst, double a_re,
+                   double a_im, double b_re, double b_im, int32_t* nz) {
+  switch (nm) {
+#define ITG_PERM_NM(N)                                                                        \
+  case N:                                                                                     \
+    launch_perm_nm<T, N>(s, kind, q, grid, blocks, prefix, nblk, src, dst, a_re, a_im, b_re, \
+                         b_im, nz);                                                           \
+    break;
+    ITG_PERM_NM(1)
+    ITG_PERM_NM(2)
+    ITG_PERM_NM(3)
+    ITG_PERM_NM(4)
+    ITG_PERM_NM(5)
+    ITG_PERM_NM(6)
+    ITG_PERM_NM(7)
+    ITG_PERM_NM(8)
+#undef ITG_PERM_NM
+  }
+}
+static_assert(kMaxModes == 8, "permute kernels are instantiated for 1..8 modes");
+}  // namespace
+
+hipError_t launch_permute(hipStream_t s, int elt, int nm, const int32_t* perm_host,
                           const PermBlock* blocks_dev, const int64_t* tile_prefix_dev,
                           int64_t nblk, int64_t total_tiles, int mode_kind, const void* src,
-                          void* dst, double a_re, double a_im, double b_re, double b_im) {
+                          void* dst, double a_re, double a_im, double b_re, double b_im,
+                          int32_t* nzflags) {
+  if (nm < 1 || nm > kMaxModes) return hipErrorInvalidValue;
+  if (nzflags && nblk > 0) {
+    ITG_LAUNCH(fill_i32_kernel, dim3((unsigned)((nblk + kThreads - 1) / kThreads)), kThreads, 0,
+               s, nzflags, nblk);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (total_tiles == 0) return hipSuccess;
-  PermArgs pa;
-  pa.nm = nm;
-  for (int i = 0; i < kMaxModes; ++i) pa.perm[i] = i < nm ? perm_host[i] : 0;
-  dim3 grid((unsigned)total_tiles), block(kThreads);
-  if (mode_kind == 0) {
-    if (cplx)
-      ITG_LAUNCH(permute_inner_kernel<1>, grid, block, 0, s, pa, blocks_dev,
-                         tile_prefix_dev, nblk, (const double2*)src, (double2*)dst, a_re, a_im,
-                         b_re, b_im);
-    else
-      ITG_LAUNCH(permute_inner_kernel<0>, grid, block, 0, s, pa, blocks_dev,
-                         tile_prefix_dev, nblk, (const double*)src, (double*)dst, a_re, a_im,
-                         b_re, b_im);
-  } else {
-    if (cplx)
-      ITG_LAUNCH(permute_tile_kernel<1>, grid, block, 0, s, pa, blocks_dev,
-                         tile_prefix_dev, nblk, (const double2*)src, (double2*)dst, a_re, a_im,
-                         b_re, b_im);
-    else
-      ITG_LAUNCH(permute_tile_kernel<0>, grid, block, 0, s, pa, blocks_dev,
-                         tile_prefix_dev, nblk, (const double*)src, (double*)dst, a_re, a_im,
-                         b_re, b_im);
+  int q = 0;  // dst mode fed by src mode 0
+  for (int i = 0; i < nm; ++i)
+    if (perm_host[i] == 0) q = i;
+  const dim3 grid((unsigned)total_tiles);
+  switch (elt) {
+    case kEltF64:
+      launch_perm_t<double>(s, nm, mode_kind, q, grid, blocks_dev, tile_prefix_dev, nblk, src,
+                            dst, a_re, a_im, b_re, b_im, nzflags);
+      break;
+    case kEltC128:
+      launch_perm_t<double2>(s, nm, mode_kind, q, grid, blocks_dev, tile_prefix_dev, nblk, src,
+                             dst, a_re, a_im, b_re, b_im, nzflags);
+      break;
+    case kEltF32:
+      launch_perm_t<float>(s, nm, mode_kind, q, grid, blocks_dev, tile_prefix_dev, nblk, src,
+                           dst, a_re, a_im, b_re, b_im, nzflags);
+      break;
+    case kEltC64:
+      launch_perm_t<float2>(s, nm, mode_kind, q, grid, blocks_dev, tile_prefix_dev, nblk, src,
+                            dst, a_re, a_im, b_re, b_im, nzflags);
+      break;
+    default:
+      return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
                       const int2* __restrict__ tiles, const int8_t* __restrict__ planesA,
                       const int8_t* __restrict__ planesB, const int32_t* __restrict__ expA,
                       const int32_t* __restrict__ expB, double* __restrict__ Cg, double ga,
-                      double beta, int comp, int sets, int nsetsA, int nsetsB) {
+                      double beta, int comp, int sets, int nsetsA, int nsetsB, int aim) {
   constexpr int BM = 32 * WM;
   constexpr int NW = 2 * WM;                    // waves
   constexpr int NSTAGE = WM == 2 ? 2 : 3;
@@ -551,7 +551,7 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
       for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[L][r]);
       double v = ldexp(h, ea + eb - 12);
       if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
-      acc[r] = fma(ga * pr.alpha, v, acc[r]);
+      acc[r] = fma(ga * (aim ? pr.alpha_im : pr.alpha), v, acc[r]);
     }
   }
 
@@ -771,12 +771,12 @@ template <int CPLX, int WM>
 hipError_t launch_sg(hipStream_t s, int comp, int sets, const SplitPair* pairs,
                      const GemmGroup* groups, const int2* tiles, int64_t grid,
                      const void* planesA, const void* planesB, const void* expA,
-                     const void* expB, void* C, double ga, double beta) {
+                     const void* expB, void* C, double ga, double beta, int aim) {
   const int ns = CPLX ? 3 : 1;
   ITG_LAUNCH((split_gemm_kernel<kSliceCount, CPLX, WM>), dim3((unsigned)grid),
                      dim3(128 * WM), 0, s, pairs, groups, tiles, (const int8_t*)planesA,
                      (const int8_t*)planesB, (const int32_t*)expA, (const int32_t*)expB,
-                     (double*)C, ga, beta, CPLX ? comp : 0, CPLX ? sets : 0, ns, ns);
+                     (double*)C, ga, beta, CPLX ? comp : 0, CPLX ? sets : 0, ns, ns, aim);
   return hipGetLastError();
 }
 }  // namespace
@@ -785,11 +785,11 @@ hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int com
                              const SplitPair* pairs, const GemmGroup* groups, const int2* tiles,
                              int64_t grid, const void* planesA, const void* planesB,
                              const void* expA, const void* expB, void* C, double ga,
-                             double beta) {
+                             double beta, int aim) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
   if (wmb == 3) {  // A in registers (real data; 64 x 64 tile table)
-    if (cplx) return hipErrorInvalidValue;
+    if (cplx || aim) return hipErrorInvalidValue;
     ITG_LAUNCH((split_gemm_areg_kernel<kSliceCount>), dim3((unsigned)grid), dim3(128), 0,
                        s, pairs, groups, tiles, (const int8_t*)planesA, (const int8_t*)planesB,
                        (const int32_t*)expA, (const int32_t*)expB, (double*)C, ga, beta);
@@ -798,13 +798,13 @@ hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int com
   if (wmb != 1 && wmb != 2) return hipErrorInvalidValue;
   if (cplx)
     return wmb == 2 ? launch_sg<1, 4>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
-                                      expA, expB, C, ga, beta)
+                                      expA, expB, C, ga, beta, aim)
                     : launch_sg<1, 2>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
-                                      expA, expB, C, ga, beta);
+                                      expA, expB, C, ga, beta, aim);
   return wmb == 2 ? launch_sg<0, 4>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
-                                    expA, expB, C, ga, beta)
+                                    expA, expB, C, ga, beta, aim)
                   : launch_sg<0, 2>(s, comp, sets, pairs, groups, tiles, grid, planesA, planesB,
-                                    expA, expB, C, ga, beta);
+                                    expA, expB, C, ga, beta, aim);
 }
 
 }  // namespace itg

@@ -41,6 +41,8 @@ EXPORTS = (
     "itg_timer_enable", "itg_timer_count", "itg_timer_read", "itg_set_split_tile",
     "itg_bs_copy", "itg_bs_chain_create", "itg_bs_chain_execute", "itg_bs_chain_destroy",
     "itg_bs_diag_plan_create", "itg_bs_diag_execute",
+    "itg_graph_begin", "itg_graph_end", "itg_graph_launch", "itg_graph_destroy",
+    "itg_bs_permute_ex",
 )
 
 
@@ -98,6 +100,8 @@ def lib():
         L.itg_bs_execute.argtypes = [P, P, I, P, P, P, DP, DP, I, ctypes.c_uint]
         L.itg_bs_plan_destroy.argtypes = [P]
         L.itg_bs_permute.argtypes = [P, I, I, I64, I64P, I64P, I64P, I32P, P, P]
+        L.itg_bs_permute_ex.argtypes = [P, I, I, I64, I64P, I64P, I64P, P, I32P, P, P, DP, DP,
+                                        I32P]
         L.itg_bs_autotune.argtypes = [P, P, I, P, P, P, I]
         L.itg_bs_copy.argtypes = [P, I, I64, I, I64P, I64P, I64P, I64P, I64P, P, I64, P, I64]
         L.itg_bs_chain_create.argtypes = [P, P, P, ctypes.POINTER(P)]
@@ -115,6 +119,10 @@ def lib():
         L.itg_free.argtypes = [P, P]
         for n in ("itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod"):
             getattr(L, n).argtypes = [P, P, P, SZ]
+        L.itg_graph_begin.argtypes = [P]
+        L.itg_graph_end.argtypes = [P, ctypes.POINTER(P)]
+        L.itg_graph_launch.argtypes = [P, P]
+        L.itg_graph_destroy.argtypes = [P]
         L.itg_timer_enable.argtypes = [P, I]
         L.itg_set_split_tile.argtypes = [P, I]
         L.itg_timer_count.argtypes = [P]

@@ -148,7 +148,7 @@ def _shape_only_contract(A, B, lA, lB, lR):
         p = ND.bs_plan(A.tensor, lA, B.tensor, lB, lR)
         bd = tuple(i.blockdims() for i in indsR)
         T = ND.BlockSparseTensor(torch.empty(0, dtype=A.eltype, device=A.tensor.data.device),
-                                 dict(zip(p.blocksR, p.offsR)), bd)
+                                 dict(zip(p.blocksR, p.offsR)), bd, structure_only=True)
         return ITensor(T, indsR)
     dims = tuple(i.dim for i in indsR)
     return ITensor(_FakeDense(dims), indsR)

@@ -312,7 +312,13 @@ class ITensor:
     def __mul__(self, other):
         if isinstance(other, ITensor):
             return contract(self, other)
-        return ITensor(ND._with_data(self.tensor, self.tensor.data * other), self.inds)
+        # a number: scale the stored values, keeping the storage kind (Diag,
+        # uniform Diag, BlockSparse, Dense), as NDTensors' scalar * tensor does
+        if not isinstance(other, (int, float, complex, np.number)):
+            return NotImplemented
+        if iscombiner(self):
+            raise TypeError("a combiner holds no values to scale")
+        return _scaled(self, other)
 
     __rmul__ = __mul__
 

@@ -179,6 +179,47 @@ class timer:
         return False
 
 
+class Graph:
+    """A HIP graph of library launches, recorded through the C ABI
+    (itg_graph_begin / itg_graph_end) and replayed with launch() on the
+    device's current stream. Everything run inside the recording must have
+    run once before (plans, factor tables and workspace exist).
+
+        g = Graph.capture(lambda: seq.run("split_int8"))
+        g.launch()
+    """
+
+    def __init__(self, dev, graph):
+        self.dev = dev
+        self.graph = graph
+
+    @staticmethod
+    def capture(fn, device=None):
+        dev = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        h = _handle(dev)
+        L = _lib.lib()
+        h.check(L.itg_graph_begin(h.h))
+        graph = ctypes.c_void_p()
+        try:
+            fn()
+        finally:
+            st = L.itg_graph_end(h.h, ctypes.byref(graph))
+        h.check(st)
+        return Graph(dev, graph)
+
+    def launch(self):
+        h = _handle(self.dev)
+        h.check(_lib.lib().itg_graph_launch(h.h, self.graph))
+
+    def __del__(self):
+        try:
+            if getattr(self, "graph", None) and _lib._lib is not None:
+                _lib._lib.itg_graph_destroy(self.graph)
+        except Exception:
+            pass
+
+
 class DenseTensor:
     """Dense storage + dims (NDTensors/src/dense/dense.jl:5-16): a flat device
     vector holding a column-major array."""
@@ -217,10 +258,26 @@ class BlockSparseTensor:
     (NDTensors/src/blocksparse/blocksparse.jl:5-13, blockoffsets.jl:7-11,
     blockdims.jl:8-19)."""
 
-    def __init__(self, data, blockoffsets, blockdims):
+    def __init__(self, data, blockoffsets, blockdims, *, structure_only=False):
         self.data = data
         self.blockoffsets = OrderedDict((tuple(int(c) for c in b), int(o)) for b, o in blockoffsets.items())
         self.blockdims = tuple(tuple(int(x) for x in bd) for bd in blockdims)
+        if structure_only:  # block structure for planning; holds no values
+            return
+        # every block must lie inside the data: the device kernels index the
+        # flat vector through these offsets without further checks
+        if data is None or data.dim() != 1:
+            raise ValueError("BlockSparseTensor data must be a flat vector")
+        end = 0
+        for b, o in self.blockoffsets.items():
+            if len(b) != len(self.blockdims) or any(
+                    not 1 <= c <= len(self.blockdims[i]) for i, c in enumerate(b)):
+                raise ValueError(f"block {b} is outside the block dimensions")
+            if o < 0:
+                raise ValueError(f"negative offset {o} of block {b}")
+            end = max(end, o + math.prod(self.blockdims_of(b)))
+        if data.numel() < end:
+            raise ValueError(f"data holds {data.numel()} elements, the blocks reach {end}")
 
     @property
     def eltype(self):
@@ -296,6 +353,19 @@ class DiagBlockSparseTensor:
         for b in self.blockoffsets:
             if any(c != b[0] for c in b):
                 raise ValueError(f"block {b} is off the block diagonal")
+            if len(b) != len(self.blockdims) or any(
+                    not 1 <= c <= len(self.blockdims[i]) for i, c in enumerate(b)):
+                raise ValueError(f"block {b} is outside the block dimensions")
+        if data is None:
+            if uniform is None:
+                raise ValueError("a DiagBlockSparseTensor needs data or a uniform value")
+        else:
+            if data.dim() != 1:
+                raise ValueError("DiagBlockSparseTensor data must be a flat vector")
+            end = max([o + self.diaglen_of(b) for b, o in self.blockoffsets.items()] or [0])
+            if min(self.blockoffsets.values(), default=0) < 0 or data.numel() < end:
+                raise ValueError(f"data holds {data.numel()} elements, the diagonal blocks "
+                                 f"reach {end}")
 
     @property
     def eltype(self):
@@ -497,6 +567,9 @@ class BSPlan:
         nblk, nnz, npairs = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         L.itg_bs_plan_sizes(plan, ctypes.byref(nblk), ctypes.byref(nnz), ctypes.byref(npairs))
         self.nblkR, self.nnzR, self.npairs = nblk.value, nnz.value, npairs.value
+        # elements each operand's data must hold (the kernels read through
+        # the offsets unchecked; execute() verifies the lengths first)
+        self.need1, self.need2 = _extent(T1), _extent(T2)
         NR = len(labelsR)
         blocks = np.zeros(max(NR * self.nblkR, 1), dtype=np.int64)
         offs = np.zeros(max(self.nblkR, 1), dtype=np.int64)
@@ -510,7 +583,14 @@ class BSPlan:
         L.itg_bs_plan_pairs(self.plan, arr.ctypes.data_as(_lib.I64P))
         return [tuple(int(x) for x in arr[3 * i:3 * i + 3]) for i in range(self.npairs)]
 
+    def _check_lengths(self, R_data, T1_data, T2_data):
+        for name, t, need in (("R", R_data, self.nnzR), ("T1", T1_data, self.need1),
+                              ("T2", T2_data, self.need2)):
+            if t is not None and t.numel() < need:
+                raise ValueError(f"{name} holds {t.numel()} elements, the plan needs {need}")
+
     def execute(self, R_data, T1_data, T2_data, alpha=1.0, prec=None, flags=0, pair_alpha=None):
+        self._check_lengths(R_data, T1_data, T2_data)
         h = _handle(R_data.device)
         dt = _dtype_code(R_data.dtype)
         a, _ka = _lib.scalar2(alpha)
@@ -527,6 +607,7 @@ class BSPlan:
 
     def execute_diag(self, R_data, T1_data, D, alpha=1.0):
         """R = alpha * T1 * D for a diagonal plan; D a DiagBlockSparseTensor."""
+        self._check_lengths(R_data, T1_data, D.data if D.uniform is None else None)
         h = _handle(R_data.device)
         a, _ka = _lib.scalar2(alpha)
         u, _ku = (_lib.scalar2(D.uniform) if D.uniform is not None else (None, None))
@@ -536,6 +617,7 @@ class BSPlan:
         h.check(st)
 
     def autotune(self, R_data, T1_data, T2_data, prec=None):
+        self._check_lengths(R_data, T1_data, T2_data)
         h = _handle(R_data.device)
         st = _lib.lib().itg_bs_autotune(h.h, self.plan, _dtype_code(R_data.dtype), _ptr(R_data),
                                         _ptr(T1_data), _ptr(T2_data), _lib.PREC[prec or _precision[0]])
@@ -562,8 +644,13 @@ class BSChain:
         self.h.check(_lib.lib().itg_bs_chain_create(self.h.h, p1.plan, p2.plan,
                                                     ctypes.byref(chain)))
         self.chain = chain
+        self.need = (p2.nnzR, p1.need1, p1.need2, p2.need2)
 
     def execute(self, Z_data, X_data, B1_data, B2_data):
+        for name, t, need in zip(("Z", "X", "B1", "B2"), (Z_data, X_data, B1_data, B2_data),
+                                 self.need):
+            if t.numel() < need:
+                raise ValueError(f"{name} holds {t.numel()} elements, the chain needs {need}")
         h = _handle(Z_data.device)
         h.check(_lib.lib().itg_bs_chain_execute(h.h, self.chain, _dtype_code(Z_data.dtype),
                                                 _ptr(Z_data), _ptr(X_data), _ptr(B1_data),
@@ -575,6 +662,15 @@ class BSChain:
                 _lib._lib.itg_bs_chain_destroy(self.chain)
         except Exception:
             pass
+
+
+def _extent(T):
+    """One past the last element any stored block of T occupies."""
+    if isinstance(T, DiagBlockSparseTensor):
+        if T.uniform is not None:
+            return 0
+        return max([o + T.diaglen_of(b) for b, o in T.blockoffsets.items()] or [0])
+    return max([o + math.prod(T.blockdims_of(b)) for b, o in T.blockoffsets.items()] or [0])
 
 
 _plan_cache: "OrderedDict[tuple, BSPlan]" = OrderedDict()
@@ -652,8 +748,91 @@ def contract(T1, labels1, T2, labels2, labelsR=None, *, alpha=1.0, prec=None, co
     raise ItgUnsupported(_lib.ITG_ERR_UNSUPPORTED, "mixed Dense / BlockSparse contraction")
 
 
+def _bs_permutedims_(R, T, perm, alpha, beta):
+    """permutedims!(R::BlockSparseTensor, T::BlockSparseTensor, perm, f) with
+    f(r, t) = beta*r + alpha*t (NDTensors/src/blocksparse/blocksparsetensor.jl:
+    834-881): over union(R's blocks, T's permuted blocks), R's block (zero
+    if absent) becomes f(r, perm(t)) (t zero if T lacks the block); blocks R
+    lacks are appended to R's storage, and kept only when nonzero. R is
+    updated in place (data and blockoffsets) and returned."""
+    perm = tuple(int(x) for x in perm)
+    N = len(perm)
+    if T.ndims != N or R.ndims != N:
+        raise ValueError("permutation length does not match the tensors")
+    inv = [0] * N
+    for i, p in enumerate(perm):
+        inv[p - 1] = i + 1
+    if tuple(R.blockdims) != tuple(T.blockdims[p - 1] for p in perm):
+        raise ValueError("R's block dimensions are not T's permuted ones")
+    if R.eltype != T.eltype:
+        raise TypeError("R and T must share one element type")
+    dev = T.data.device
+    h = _handle(dev)
+    L = _lib.lib()
+    tb = {tuple(b[p - 1] for p in perm): (b, o) for b, o in T.blockoffsets.items()}
+    order = list(R.blockoffsets)
+    new = [bp for bp in tb if bp not in R.blockoffsets]
+    extent = _extent(R)
+    offs_dst = list(R.blockoffsets.values())
+    acc = extent
+    for bp in new:
+        offs_dst.append(acc)
+        acc += math.prod(R.blockdims_of(bp))
+    order += new
+    data = torch.empty(acc, dtype=R.eltype, device=dev)
+    if complex(beta) != 0 and extent:
+        h.check(L.itg_memcpy_dtod(h.h, _ptr(data), _ptr(R.data), extent * R.data.element_size()))
+    dims, offs_src = [], []
+    for bp in order:
+        bt = tb.get(bp)
+        dims.extend(R.blockdims_of(bp)[i - 1] for i in inv)  # T's mode order
+        offs_src.append(bt[1] if bt else -1)
+    fresh = np.array([0] * len(R.blockoffsets) + [1] * len(new), dtype=np.uint8)
+    nz = np.zeros(len(order), dtype=np.int32)
+    keep = []
+
+    def pp(a):
+        ptr, k = _lib.ptr64(a)
+        keep.append(k)
+        return ptr
+
+    a, _ka = _lib.scalar2(alpha)
+    b, _kb = _lib.scalar2(beta)
+    p32, _k32 = _lib.ptr32(list(perm))
+    if order:
+        h.check(L.itg_bs_permute_ex(
+            h.h, _dtype_code(T.eltype), N, len(order), pp(_lib.i64(dims)), pp(_lib.i64(offs_src)),
+            pp(_lib.i64(offs_dst)), fresh.ctypes.data_as(ctypes.c_void_p), p32, _ptr(T.data),
+            _ptr(data), a, b, nz.ctypes.data_as(_lib.I32P)))
+    nR = len(R.blockoffsets)
+    kept = [i for i in range(nR, len(order)) if nz[i]]
+    if len(kept) < len(new):  # drop the new blocks that came out all zero
+        total = extent + sum(math.prod(R.blockdims_of(order[i])) for i in kept)
+        final = torch.empty(total, dtype=R.eltype, device=dev)
+        es = data.element_size()
+        if extent:
+            h.check(L.itg_memcpy_dtod(h.h, _ptr(final), _ptr(data), extent * es))
+        o = extent
+        new_offs = offs_dst[:nR]
+        for i in kept:
+            n = math.prod(R.blockdims_of(order[i]))
+            h.check(L.itg_memcpy_dtod(h.h, ctypes.c_void_p(final.data_ptr() + o * es),
+                                      ctypes.c_void_p(data.data_ptr() + offs_dst[i] * es), n * es))
+            new_offs.append(o)
+            o += n
+        data = final
+        order = order[:nR] + [order[i] for i in kept]
+        offs_dst = new_offs
+    R.data = data
+    R.blockoffsets = OrderedDict(zip(order, offs_dst))
+    return R
+
+
 def permutedims_(R, T, perm, alpha=1.0, beta=0.0):
-    """R = alpha*permutedims(T, perm) + beta*R (perm 1-based, Julia's)."""
+    """R = alpha*permutedims(T, perm) + beta*R (perm 1-based, Julia's); for
+    block-sparse tensors the union-of-blocks form of permutedims!."""
+    if isinstance(T, BlockSparseTensor):
+        return _bs_permutedims_(R, T, perm, alpha, beta)
     h = _handle(T.data.device)
     d, _k = _lib.ptr64(_lib.i64(T.dims))
     pp, _kp = _lib.ptr32([int(x) for x in perm])

@@ -26,8 +26,18 @@ Pinning status (what checks each function; see tests/test_oracle.py):
                                         (test_dense.jl:237-287)
   blocksparse_contract               -- densify(result) == dense_contract(
                                         densify(A), densify(B)) on random
-                                        structures
+                                        structures; per-pair factors: unit
+                                        factors change nothing, one-hot
+                                        factors give one pair's product
+                                        (densified, einsum) and their sum
+                                        is the whole contraction
   rrule_contract                     -- central finite differences
+  blocksparse_permutedims_           -- densified: dense(R') == beta*dense(R) +
+                                        alpha*transpose(dense(T)) on random QN
+                                        structures (R missing some of T's
+                                        blocks and holding blocks T lacks);
+                                        block order = R's then T's new ones;
+                                        all-zero new blocks dropped
   combined_index /
   combined_position_map              -- hand-derived known answer (mixed arrows,
                                         uneven sectors: order, merge, sizes,
@@ -395,22 +405,29 @@ def contract_blockoffsets_sequential(T1, l1, T2, l2, lR):
     return offsR, nnzR, plan, indsR
 
 
-def blocksparse_contract(T1, l1, T2, l2, lR=None, alpha=1.0, exact=False, conj1=False, conj2=False):
+def blocksparse_contract(T1, l1, T2, l2, lR=None, alpha=1.0, exact=False, conj1=False, conj2=False,
+                         pair_alpha=None):
     """NDTensors.contract(::BlockSparseTensor, ...) (blocksparse/contract.jl:3-76,
     contract_generic.jl:37-129): plan, allocate R, then for each output block
     the pairs in plan order with beta = 0 for the first and 1 afterwards.
+    pair_alpha: None, or one factor per pair of the plan (enumeration order)
+    multiplying alpha -- the value compute_alpha returns for that pair
+    (contract_utilities.jl:2-15, called per pair at contract_generic.jl:91).
     Returns (R, plan)."""
     if lR is None:
         lR = contract_labels(l1, l2)
     offsR, nnzR, plan, indsR = contract_blockoffsets_sequential(T1, l1, T2, l2, lR)
-    cplx = np.iscomplexobj(T1.data) or np.iscomplexobj(T2.data) or np.iscomplexobj(alpha)
+    if pair_alpha is not None and len(pair_alpha) != len(plan):
+        raise ValueError(f"pair_alpha has {len(pair_alpha)} factors for {len(plan)} pairs")
+    cplx = (np.iscomplexobj(T1.data) or np.iscomplexobj(T2.data) or np.iscomplexobj(alpha)
+            or (pair_alpha is not None and np.iscomplexobj(np.asarray(pair_alpha))))
     dt = np.complex128 if cplx else np.float64
     if exact:
         dt = np.clongdouble if cplx else np.longdouble
     data = np.full(nnzR, np.nan, dtype=dt)  # uninitialised output (similar.jl:5-8)
     R = BlockSparse(data, offsR, indsR)
     written = set()
-    for b1, b2, bR in plan:
+    for n, (b1, b2, bR) in enumerate(plan):
         A = T1.block(b1)
         B = T2.block(b2)
         if conj1:
@@ -419,9 +436,48 @@ def blocksparse_contract(T1, l1, T2, l2, lR=None, alpha=1.0, exact=False, conj1=
             B = np.conj(B)
         view = R.block(bR)
         beta = 1.0 if bR in written else 0.0
-        view[...] = dense_contract(view, lR, A, l1, B, l2, alpha, beta, exact=exact)
+        a = alpha if pair_alpha is None else alpha * pair_alpha[n]
+        view[...] = dense_contract(view, lR, A, l1, B, l2, a, beta, exact=exact)
         written.add(bR)
     return R, plan
+
+
+# ------------------------------------------------ block-sparse permutedims!
+def blocksparse_permutedims_(R, T, perm, alpha=1.0, beta=0.0):
+    """permutedims!(R, T, perm, f) with f(r, t) = beta*r + alpha*t
+    (NDTensors/src/blocksparse/blocksparsetensor.jl:834-881): over
+    union(R's blocks, T's permuted blocks) in that order, R's block (zeros if
+    absent) becomes f.(r, permutedims(T's block or zeros, perm)); a block R
+    did not have is inserted (appended, blockoffsets.jl insertion order) only
+    when the result is nonzero. beta == 0 does not read R (NaN-safe).
+    perm is 0-based here. Returns the new R (new data and offsets)."""
+    perm = tuple(perm)
+    inv = tuple(np.argsort(perm))
+    order = list(R.offsets)
+    for b in T.offsets:
+        pb = tuple(b[p] for p in perm)
+        if pb not in R.offsets and pb not in order:
+            order.append(pb)
+    dt = np.result_type(R.data, T.data, np.asarray(alpha), np.asarray(beta))
+    blocks, offs, n = [], OrderedDict(), 0
+    for bR in order:
+        bT = tuple(bR[i] for i in inv)
+        shape = blockdims(R.inds, bR)
+        t = (np.transpose(T.block(bT), perm) if bT in T.offsets
+             else np.zeros(shape, dtype=dt))
+        if bR in R.offsets:
+            r = R.block(bR)
+            v = alpha * t + (beta * r if beta != 0 else 0)
+        else:
+            v = alpha * t
+            if not np.any(v != 0):
+                continue  # a new all-zero block is not inserted
+        v = np.asarray(v, dtype=dt)
+        offs[bR] = n
+        n += v.size
+        blocks.append(v.reshape(-1, order="F"))
+    data = np.concatenate(blocks) if blocks else np.zeros(0, dtype=dt)
+    return BlockSparse(data, offs, R.inds)
 
 
 # ------------------------------------------------------------ DiagBlockSparse

@@ -8,6 +8,7 @@ oracle (a dropped pair, a wrong label sign, a transposed operand) fails at
 least one of them.
 """
 import itertools
+from collections import OrderedDict
 
 import numpy as np
 import pytest
@@ -160,6 +161,54 @@ def test_blocksparse_values_vs_dense(cplx):
         done += 1
 
 
+def _only_block(T, b):
+    """T with every stored block but b zeroed (same structure)."""
+    data = np.zeros_like(T.data)
+    o = T.offsets[b]
+    n = T.block(b).size
+    data[o:o + n] = T.data[o:o + n]
+    return O.BlockSparse(data, T.offsets, T.inds)
+
+
+def test_blocksparse_pair_alpha():
+    """Per-pair factors (compute_alpha's slot, contract_generic.jl:91): unit
+    factors change nothing; a one-hot factor on pair n gives exactly that
+    pair's product (densified operands holding only its two blocks, einsum);
+    complex factors c_n give sum_n c_n * (pair n's product)."""
+    rng = np.random.default_rng(12)
+    done = 0
+    while done < 8:
+        try:
+            A, B = _random_case(rng, 3, 3, int(rng.integers(1, 3)))
+        except ValueError:
+            continue
+        la, lb = O.compute_contraction_labels(A.inds, B.inds)
+        lr = O.contract_labels(la, lb)
+        full, plan = O.blocksparse_contract(A, la, B, lb)
+        if len(plan) < 2:
+            continue
+        ones, _ = O.blocksparse_contract(A, la, B, lb, pair_alpha=[1.0] * len(plan))
+        assert np.array_equal(ones.data, full.data)
+        spec = O._einsum_spec(la, lb, lr)
+        c = rng.standard_normal(len(plan)) + 1j * rng.standard_normal(len(plan))
+        want = np.zeros(full.dense().shape, dtype=np.complex128)
+        total = np.zeros_like(full.dense())
+        for n, (b1, b2, _bR) in enumerate(plan):
+            hot = [0.0] * len(plan)
+            hot[n] = 1.0
+            got, _ = O.blocksparse_contract(A, la, B, lb, pair_alpha=hot)
+            ref = np.einsum(spec, _only_block(A, b1).dense(), _only_block(B, b2).dense())
+            assert np.allclose(got.dense(), ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
+            total += ref
+            want += c[n] * ref
+        assert np.allclose(full.dense(), total, rtol=0, atol=1e-12 * max(1.0, np.abs(total).max()))
+        got, _ = O.blocksparse_contract(A, la, B, lb, alpha=0.5, pair_alpha=list(c))
+        assert np.allclose(got.dense(), 0.5 * want, rtol=0, atol=1e-12 * max(1.0, np.abs(want).max()))
+        with pytest.raises(ValueError):
+            O.blocksparse_contract(A, la, B, lb, pair_alpha=[1.0])
+        done += 1
+
+
 # ------------------------------------------------------------ dense contract
 def test_dense_contract_vs_loops():
     rng = np.random.default_rng(4)
@@ -262,3 +311,91 @@ def test_heff_sector_sizes_and_counts():
         # stand-in with the right structure for the next step
         cur = O.BlockSparse(np.zeros(nnz), offs, indsR)
     assert pairs == [372, 940, 952, 372]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_heff_apply_dense_vs_brute_force(cplx):
+    """heff_apply_dense (the left-to-right fold of the two-site network) against
+    the network's defining sum written as seven nested loops on tiny
+    dimensions: O[x,k,l,y] = sum L[x,w,a] psi[a,i,j,b] W1[w,i,k,u]
+    W2[u,j,l,v] R[y,v,b]. A transposed operand, a wrong contracted pair or a
+    dropped step changes the result."""
+    rng = np.random.default_rng(11)
+    chi, d, w = 3, 2, 2
+
+    def draw(*shape):
+        x = rng.standard_normal(shape)
+        return x + 1j * rng.standard_normal(shape) if cplx else x
+
+    L, psi = draw(chi, w, chi), draw(chi, d, d, chi)
+    W1, W2, R = draw(w, d, d, w), draw(w, d, d, w), draw(chi, w, chi)
+    want = np.zeros((chi, d, d, chi), dtype=psi.dtype)
+    rs, ds, ws = range(chi), range(d), range(w)
+    for x, k, l, y in itertools.product(rs, ds, ds, rs):
+        acc = 0.0
+        for wl, a, i, j, b, u, v in itertools.product(ws, rs, ds, ds, rs, ws, ws):
+            acc += L[x, wl, a] * psi[a, i, j, b] * W1[wl, i, k, u] * W2[u, j, l, v] * R[y, v, b]
+        want[x, k, l, y] = acc
+    for exact in (False, True):
+        got = O.heff_apply_dense(L, psi, W1, W2, R, exact=exact)
+        assert got.shape == want.shape
+        assert relerr(got, want) < 1e-13
+    # a row subset of L gives the same rows of the output (the sampled-row
+    # oracle of the large dense configurations relies on it)
+    rows = [2, 0]
+    assert relerr(O.heff_apply_dense(L[rows], psi, W1, W2, R), want[rows]) < 1e-13
+
+
+def test_blocksparse_permutedims_union():
+    """blocksparse_permutedims_ (blocksparsetensor.jl:834-881) against its
+    dense meaning: dense(R') = beta*dense(R) + alpha*permutedims(dense(T));
+    R' holds R's blocks in R's order, then the permuted blocks of T that R
+    lacks in T's order, except new blocks that come out all zero."""
+    rng = np.random.default_rng(13)
+    done = 0
+    while done < 10:
+        try:
+            A, _B = _random_case(rng, 3, 3, 1)
+        except ValueError:
+            continue
+        perm = tuple(rng.permutation(3))
+        pinds = tuple(A.inds[p] for p in perm)
+        blocks = [tuple(b[p] for p in perm) for b in A.offsets]
+        if len(blocks) < 3:
+            continue
+        # R: some of T's permuted blocks missing, R's own block order shuffled
+        keep = [blocks[i] for i in rng.permutation(len(blocks))[:len(blocks) - 1]]
+        # and a block T does not have (another flux), so that r is kept as beta*r
+        others = [b for b in itertools.product(*[range(i.nblocks()) for i in pinds])
+                  if b not in blocks]
+        if not others:
+            continue
+        keep.insert(int(rng.integers(0, len(keep) + 1)), others[int(rng.integers(0, len(others)))])
+        offs, n = OrderedDict(), 0
+        for b in keep:
+            offs[b] = n
+            n += int(np.prod(O.blockdims(pinds, b)))
+        R = O.BlockSparse(rng.standard_normal(n), offs, pinds)
+        T = A
+        if rng.random() < 0.5:  # an all-zero block of T that R lacks is not inserted
+            miss = [b for b in blocks if b not in R.offsets][0]
+            inv = tuple(np.argsort(perm))
+            bt = tuple(miss[i] for i in inv)
+            o = T.offsets[bt]
+            T = O.BlockSparse(T.data.copy(), T.offsets, T.inds)
+            T.data[o:o + T.block(bt).size] = 0.0
+        for alpha, beta in ((1.0, 0.0), (0.5, -2.0), (0.0, 1.5)):
+            out = O.blocksparse_permutedims_(R, T, perm, alpha, beta)
+            want = beta * R.dense() + alpha * np.transpose(T.dense(), perm)
+            assert np.allclose(out.dense(), want, rtol=0, atol=1e-14)
+            assert list(out.offsets)[:len(R.offsets)] == list(R.offsets)
+            new = list(out.offsets)[len(R.offsets):]
+            tnew = [b for b in blocks if b not in R.offsets]
+            assert new == [b for b in tnew if alpha != 0 and
+                           np.any(np.transpose(T.block(tuple(b[i] for i in np.argsort(perm))),
+                                               perm) != 0)]
+        # beta == 0 does not read R
+        Rn = O.BlockSparse(np.full(R.data.size, np.nan), R.offsets, R.inds)
+        out = O.blocksparse_permutedims_(Rn, T, perm, 1.0, 0.0)
+        assert not np.isnan(out.data).any()
+        done += 1
