@@ -45,9 +45,17 @@ enum {
   ITG_ERR_NOMEM = 4        /* device allocation of workspace failed         */
 };
 
-/* Element types (Julia eltypes). */
+/* Element types (Julia eltypes). Float32 / ComplexF32 are accepted by
+ * itg_permute, itg_bs_permute(_ex), itg_contract_dense(_ex), itg_bs_execute
+ * and itg_bs_diag_execute: permutations move them as they are; contractions
+ * run in FP64 (MFMA) on widened copies of the operands, whatever the
+ * precision mode, and round once when R is written (R = round(alpha*T1*T2 +
+ * beta*R), the sum formed in FP64). The other entry points take FP64 data
+ * and return ITG_ERR_UNSUPPORTED for these. */
 typedef enum {
+  ITG_F32 = 0,  /* Float32    */
   ITG_F64 = 1,  /* Float64    */
+  ITG_C64 = 2,  /* ComplexF32 */
   ITG_C128 = 3  /* ComplexF64 */
 } itg_dtype;
 
@@ -141,6 +149,15 @@ itg_status itg_free(itg_handle* h, void* ptr);
 itg_status itg_memcpy_htod(itg_handle* h, void* dst, const void* src, size_t bytes);
 itg_status itg_memcpy_dtoh(itg_handle* h, void* dst, const void* src, size_t bytes);
 itg_status itg_memcpy_dtod(itg_handle* h, void* dst, const void* src, size_t bytes);
+/* dst[i] = src[i] for i < n, converted from element type `from` to the wider
+ * type `to` (Float32 -> Float64 / ComplexF32 / ComplexF64, Float64 ->
+ * ComplexF64, ComplexF32 -> ComplexF64; the same type copies). The device
+ * side of the promotion by allocation the dense contract! does for mixed
+ * element types (NDTensors/src/dense/tensoralgebra/contract.jl:196-211,
+ * NDTensors/ext/NDTensorscuTENSORExt/contract.jl:34-41). ITG_ERR_INVALID
+ * for a narrowing pair. Stream-ordered. */
+itg_status itg_convert(itg_handle* h, itg_dtype from, const void* src, itg_dtype to, void* dst,
+                       int64_t n);
 
 /* --------------------------------------------------------------- permute */
 

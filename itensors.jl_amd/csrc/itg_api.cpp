@@ -25,6 +25,8 @@ struct itg_handle {
   std::string err;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  void* ws32 = nullptr;        // FP64 copies of Float32 / ComplexF32 operands
+  size_t ws32_bytes = 0;
   std::vector<void*> retired;  // earlier workspaces, freed by itg_destroy
   bool timing = false;         // itg_timer_enable
   std::unique_ptr<KernelTimer> timer;
@@ -94,21 +96,74 @@ itg_status hip_fail(itg_handle* h, hipError_t e, const char* what) {
 }
 
 bool dtype_ok(itg_dtype dt) { return dt == ITG_F64 || dt == ITG_C128; }
+// Float32 / ComplexF32: accepted at the boundary; contractions run in FP64
+// on widened copies and round once when R is written.
+bool dtype32(itg_dtype dt) { return dt == ITG_F32 || dt == ITG_C64; }
+bool dtype_any(itg_dtype dt) { return dtype_ok(dt) || dtype32(dt); }
+itg_dtype wide_of(itg_dtype dt) { return dt == ITG_C64 ? ITG_C128 : ITG_F64; }
+int elt_code(itg_dtype dt) {
+  return dt == ITG_F64 ? kEltF64 : dt == ITG_C128 ? kEltC128 : dt == ITG_F32 ? kEltF32 : kEltC64;
+}
 
-hipError_t ensure_ws(itg_handle* h, size_t bytes) {
-  if (bytes <= h->ws_bytes) return hipSuccess;
-  // A captured graph keeps the address of the workspace it was recorded
-  // with, so a workspace that has been handed to kernels is never freed
-  // before itg_destroy: it is retired and the new one at least doubles, which
-  // keeps the retired total below the live size.
-  size_t b = std::max({bytes, (size_t)1 << 20, 2 * h->ws_bytes});
+// A grow-only device buffer: a captured graph keeps the address it was
+// recorded with, so a buffer that has been handed to kernels is never freed
+// before itg_destroy: it is retired and the new one at least doubles, which
+// keeps the retired total below the live size.
+hipError_t grow(itg_handle* h, void** buf, size_t* cur, size_t bytes) {
+  if (bytes <= *cur) return hipSuccess;
+  size_t b = std::max({bytes, (size_t)1 << 20, 2 * *cur});
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, b);
   if (e != hipSuccess) return e;
-  if (h->ws) h->retired.push_back(h->ws);
-  h->ws = p;
-  h->ws_bytes = b;
+  if (*buf) h->retired.push_back(*buf);
+  *buf = p;
+  *cur = b;
   return hipSuccess;
+}
+
+hipError_t ensure_ws(itg_handle* h, size_t bytes) { return grow(h, &h->ws, &h->ws_bytes, bytes); }
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// FP64 copies of up to two Float32 / ComplexF32 operands (n1, n2 elements)
+// and an FP64 output of nR elements, carved from the handle's ws32.
+struct Wide {
+  void* t1 = nullptr;
+  void* t2 = nullptr;
+  void* r = nullptr;
+};
+itg_status widen_operands(itg_handle* h, itg_dtype dt, int64_t n1, const void* T1, int64_t n2,
+                          const void* T2, int64_t nR, Wide* w) {
+  const int cplx = dt == ITG_C64;
+  const size_t es = cplx ? 16 : 8;
+  const size_t b1 = al256((size_t)n1 * es), b2 = al256((size_t)n2 * es);
+  hipError_t e = grow(h, &h->ws32, &h->ws32_bytes, b1 + b2 + al256((size_t)nR * es) + 256);
+  if (e != hipSuccess) return hip_fail(h, e, "workspace allocation");
+  char* p = (char*)h->ws32;
+  w->t1 = p;
+  w->t2 = p + b1;
+  w->r = p + b1 + b2;
+  if (T1 && (e = launch_widen(h->stream, cplx, n1, T1, w->t1)) != hipSuccess)
+    return hip_fail(h, e, "widen launch");
+  if (T2 && (e = launch_widen(h->stream, cplx, n2, T2, w->t2)) != hipSuccess)
+    return hip_fail(h, e, "widen launch");
+  return ITG_OK;
+}
+
+// One past the last element a block list's blocks occupy (diag: each block
+// stores min(dims) elements).
+int64_t extent_of(const BlockList& b, bool diag) {
+  int64_t end = 0;
+  for (int64_t k = 0; k < b.nblk(); ++k) {
+    int64_t n = 1, mn = INT64_MAX;
+    for (int i = 0; i < b.N; ++i) {
+      n *= b.d(k)[i];
+      mn = std::min<int64_t>(mn, b.d(k)[i]);
+    }
+    if (diag) n = b.N ? mn : 1;
+    end = std::max(end, b.offs[k] + n);
+  }
+  return end;
 }
 
 struct DeviceGuard {
@@ -195,6 +250,7 @@ itg_status itg_destroy(itg_handle* h) {
     }
     if (h->ws || !h->retired.empty()) (void)hipStreamSynchronize(h->stream);
     if (h->ws) (void)hipFree(h->ws);
+    if (h->ws32) (void)hipFree(h->ws32);
     for (void* p : h->retired) (void)hipFree(p);
     h->dense_map.clear();
     h->dense_lru.clear();
@@ -380,6 +436,27 @@ itg_status itg_memcpy_htod(itg_handle* h, void* dst, const void* src, size_t byt
 itg_status itg_memcpy_dtoh(itg_handle* h, void* dst, const void* src, size_t bytes) {
   return copy_bytes(h, dst, src, bytes, hipMemcpyDeviceToHost, true);
 }
+itg_status itg_convert(itg_handle* h, itg_dtype from, const void* src, itg_dtype to, void* dst,
+                       int64_t n) {
+  if (!h) return ITG_ERR_INVALID;
+  if (!dtype_any(from) || !dtype_any(to)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (n < 0) return fail(h, ITG_ERR_INVALID, "negative length");
+  if (n == 0) return ITG_OK;
+  if (!src || !dst) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  DeviceGuard g(h->device);
+  if (from == to) {
+    const size_t es = from == ITG_C128 ? 16 : (from == ITG_F32 ? 4 : 8);
+    hipError_t e = hipMemcpyAsync(dst, src, (size_t)n * es, hipMemcpyDeviceToDevice, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMemcpyAsync");
+    return ITG_OK;
+  }
+  hipError_t e = launch_convert(h->stream, elt_code(from), elt_code(to), n, src, dst);
+  if (e == hipErrorInvalidValue)
+    return fail(h, ITG_ERR_INVALID, "not a widening conversion (Float32 < Float64, real < complex)");
+  if (e != hipSuccess) return hip_fail(h, e, "convert launch");
+  return ITG_OK;
+}
+
 itg_status itg_memcpy_dtod(itg_handle* h, void* dst, const void* src, size_t bytes) {
   return copy_bytes(h, dst, src, bytes, hipMemcpyDeviceToDevice, false);
 }
@@ -388,7 +465,7 @@ itg_status itg_permute(itg_handle* h, itg_dtype dtype, int ndim, const int64_t* 
                        const int32_t* perm, const void* src, void* dst, const double* alpha,
                        const double* beta) {
   if (!h) return ITG_ERR_INVALID;
-  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (!dtype_any(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (ndim < 0 || ndim > kMaxModes) return fail(h, ITG_ERR_UNSUPPORTED, "too many modes");
   if (ndim > 0 && (!dims || !perm)) return fail(h, ITG_ERR_INVALID, "null dims/perm");
   int p0[kMaxModes];
@@ -417,10 +494,10 @@ itg_status itg_permute(itg_handle* h, itg_dtype dtype, int ndim, const int64_t* 
     hipError_t e = build_permute_launch(&pl, ndim, p0, bl, nullptr, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "permute tables");
   }
-  const int cplx = dtype == ITG_C128;
+  const int cplx = dtype == ITG_C128 || dtype == ITG_C64;
   const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
   const double b_re = beta ? beta[0] : 0.0, b_im = (beta && cplx) ? beta[1] : 0.0;
-  hipError_t e = launch_permute(h->stream, cplx, pl.nm, pl.perm, (const PermBlock*)pl.blocks.p,
+  hipError_t e = launch_permute(h->stream, elt_code(dtype), pl.nm, pl.perm, (const PermBlock*)pl.blocks.p,
                                 (const int64_t*)pl.prefix.p, pl.nblk, pl.tiles, pl.kind, src, dst,
                                 a_re, a_im, b_re, b_im);
   if (e != hipSuccess) return hip_fail(h, e, "permute launch");
@@ -437,7 +514,7 @@ itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype, int NR, const i
                                  const double* alpha, const double* beta, itg_prec prec,
                                  unsigned flags) {
   if (!h) return ITG_ERR_INVALID;
-  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (!dtype_any(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
     return fail(h, ITG_ERR_INVALID, "unknown precision mode");
   if ((NR > 0 && (!dimsR || !labelsR)) || (N1 > 0 && (!dims1 || !labels1)) ||
@@ -469,6 +546,23 @@ itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype, int NR, const i
     return fail(h, ITG_ERR_UNSUPPORTED, "tensor with more than 2^31-1 elements");
   if (nR == 0) return ITG_OK;
   if (!R || (n1 > 0 && !T1) || (n2 > 0 && !T2)) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  if (dtype32(dtype)) {
+    // FP64 (MFMA) on widened copies, whatever the precision mode: Float32
+    // values are exact in FP64, so the split form has nothing to emulate
+    DeviceGuard g(h->device);
+    Wide w;
+    itg_status st = widen_operands(h, dtype, n1, T1, n2, T2, nR, &w);
+    if (st != ITG_OK) return st;
+    st = itg_contract_dense_ex(h, wide_of(dtype), NR, dimsR, labelsR, w.r, N1, dims1, labels1,
+                               w.t1, N2, dims2, labels2, w.t2, alpha, nullptr, ITG_PREC_FP64,
+                               flags);
+    if (st != ITG_OK) return st;
+    const int c = dtype == ITG_C64;
+    hipError_t e = launch_narrow(h->stream, c, nR, w.r, R, beta ? beta[0] : 0.0,
+                                 (beta && c) ? beta[1] : 0.0);
+    if (e != hipSuccess) return hip_fail(h, e, "narrow launch");
+    return ITG_OK;
+  }
   const int cplx = dtype == ITG_C128;
   DeviceGuard g(h->device);
 
@@ -785,10 +879,24 @@ itg_status itg_bs_diag_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, v
                                const double* alpha) {
   if (!h || !p) return ITG_ERR_INVALID;
   if (!p->diag) return fail(h, ITG_ERR_INVALID, "not a diagonal plan (use itg_bs_execute)");
-  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (!dtype_any(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (p->pairs.empty()) return ITG_OK;
   if (!R || !T1 || (!D && !uniform)) return fail(h, ITG_ERR_INVALID, "null data pointer");
   if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
+  if (dtype32(dtype)) {
+    DeviceGuard g(h->device);
+    Wide w;
+    itg_status st = widen_operands(h, dtype, extent_of(p->b1, false), T1,
+                                   uniform ? 0 : extent_of(p->b2, true), uniform ? nullptr : D,
+                                   p->bR.nnz, &w);
+    if (st != ITG_OK) return st;
+    st = itg_bs_diag_execute(h, p, wide_of(dtype), w.r, w.t1, uniform ? nullptr : w.t2, uniform,
+                             alpha);
+    if (st != ITG_OK) return st;
+    hipError_t e = launch_narrow(h->stream, dtype == ITG_C64, p->bR.nnz, w.r, R, 0.0, 0.0);
+    if (e != hipSuccess) return hip_fail(h, e, "narrow launch");
+    return ITG_OK;
+  }
   const int cplx = dtype == ITG_C128;
   DeviceGuard g(h->device);
   const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
@@ -841,13 +949,26 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* 
                           const void* T1, const void* T2, const double* alpha,
                           const double* pair_alpha, itg_prec prec, unsigned flags) {
   if (!h || !p) return ITG_ERR_INVALID;
-  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (!dtype_any(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
     return fail(h, ITG_ERR_INVALID, "unknown precision mode");
   if (p->diag) return fail(h, ITG_ERR_INVALID, "diagonal plan (use itg_bs_diag_execute)");
   if (p->pairs.empty()) return ITG_OK;
   if (!R || !T1 || !T2) return fail(h, ITG_ERR_INVALID, "null data pointer");
   if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
+  if (dtype32(dtype)) {
+    DeviceGuard g(h->device);
+    Wide w;
+    itg_status st = widen_operands(h, dtype, extent_of(p->b1, false), T1,
+                                   extent_of(p->b2, false), T2, p->bR.nnz, &w);
+    if (st != ITG_OK) return st;
+    st = itg_bs_execute(h, p, wide_of(dtype), w.r, w.t1, w.t2, alpha, pair_alpha,
+                        ITG_PREC_FP64, flags);
+    if (st != ITG_OK) return st;
+    hipError_t e = launch_narrow(h->stream, dtype == ITG_C64, p->bR.nnz, w.r, R, 0.0, 0.0);
+    if (e != hipSuccess) return hip_fail(h, e, "narrow launch");
+    return ITG_OK;
+  }
   const int cplx = dtype == ITG_C128;
   DeviceGuard g(h->device);
   ExecPlan* ep = p->ep.get();
@@ -946,7 +1067,7 @@ itg_status itg_bs_permute_ex(itg_handle* h, itg_dtype dtype, int N, int64_t nblk
                              const void* src, void* dst, const double* alpha, const double* beta,
                              int32_t* nonzero) {
   if (!h) return ITG_ERR_INVALID;
-  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (!dtype_any(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (N < 0 || N > kMaxModes) return fail(h, ITG_ERR_UNSUPPORTED, "too many modes");
   if (nblk < 0) return fail(h, ITG_ERR_INVALID, "negative block count");
   if (nblk == 0) return ITG_OK;
@@ -990,12 +1111,12 @@ itg_status itg_bs_permute_ex(itg_handle* h, itg_dtype dtype, int N, int64_t nblk
     if ((e = nz.upload(zero.data(), zero.size() * sizeof(int32_t), h->stream)) != hipSuccess)
       return hip_fail(h, e, "flag buffer");
   }
-  const int cplx = dtype == ITG_C128;
+  const int cplx = dtype == ITG_C128 || dtype == ITG_C64;
   const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
   const double b_re = beta ? beta[0] : 0.0, b_im = (beta && cplx) ? beta[1] : 0.0;
   KernelTimer* kt = active_timer(h);
   if (kt) kt->begin("permute_kernel", h->stream);
-  e = launch_permute(h->stream, cplx, pl.nm, pl.perm, (const PermBlock*)pl.blocks.p,
+  e = launch_permute(h->stream, elt_code(dtype), pl.nm, pl.perm, (const PermBlock*)pl.blocks.p,
                      (const int64_t*)pl.prefix.p, pl.nblk, pl.tiles, pl.kind, src, dst, a_re,
                      a_im, b_re, b_im, nonzero ? (int32_t*)nz.p : nullptr);
   if (kt) kt->end(h->stream);

@@ -381,6 +381,15 @@ hipError_t launch_split_gemm(hipStream_t s, int prec, int cplx, int wmb, int com
                              double beta, int aim = 0);
 // x[i] = beta * x[i] for i < n (complex when cplx).
 hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re, double b_im);
+// Float32 / ComplexF32 at the boundary: x64[i] = x32[i] (n elements; cplx:
+// interleaved pairs), and y32[i] = round(x64[i] + beta * y32[i]) with the sum
+// formed in FP64 (beta == 0 does not read y32).
+hipError_t launch_widen(hipStream_t s, int cplx, int64_t n, const void* x32, void* x64);
+hipError_t launch_narrow(hipStream_t s, int cplx, int64_t n, const void* x64, void* y32,
+                         double b_re, double b_im);
+// y = x converted from EltCode `from` to the wider EltCode `to` (n elements;
+// hipErrorInvalidValue for a pair that is not a widening).
+hipError_t launch_convert(hipStream_t s, int from, int to, int64_t n, const void* x, void* y);
 // Complex y[i] = alpha * (x[i] + i * x2[i]) + beta * y[i] for i < n (x2 may
 // be null: taken as 0; beta == 0 does not read y).
 hipError_t launch_axpby_join(hipStream_t s, int64_t n, double a_re, double a_im, const void* x,

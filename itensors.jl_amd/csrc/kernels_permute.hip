@@ -245,6 +245,58 @@ __global__ void __launch_bounds__(kThreads)
   x[i] = axpby(b_re, b_im, x[i], 0.0, 0.0, y, false);
 }
 
+// x64 = x32 elementwise (n scalars: complex data counts both parts).
+__global__ void __launch_bounds__(kThreads)
+    widen_kernel(const float* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) y[i] = (double)x[i];
+}
+
+// dst[i] = src[i] converted to a wider element type (real -> complex puts
+// zero imaginary parts; Float32 -> Float64 is exact).
+template <typename S, typename D>
+__device__ inline D widen_elem(S x);
+template <> __device__ inline double widen_elem<float, double>(float x) { return x; }
+template <> __device__ inline float2 widen_elem<float, float2>(float x) { return float2{x, 0.0f}; }
+template <> __device__ inline double2 widen_elem<float, double2>(float x) { return double2{x, 0.0}; }
+template <> __device__ inline double2 widen_elem<double, double2>(double x) { return double2{x, 0.0}; }
+template <> __device__ inline double2 widen_elem<float2, double2>(float2 x) {
+  return double2{x.x, x.y};
+}
+template <typename S, typename D>
+__global__ void __launch_bounds__(kThreads)
+    convert_kernel(const S* __restrict__ x, D* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) y[i] = widen_elem<S, D>(x[i]);
+}
+
+// y32 = round(x64 + beta * y32), the sum in FP64.
+template <int CPLX>
+__global__ void __launch_bounds__(kThreads)
+    narrow_kernel(const typename Elem<CPLX>::T* __restrict__ x, void* __restrict__ y, int64_t n,
+                  double b_re, double b_im) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const bool use_beta = b_re != 0.0 || b_im != 0.0;
+  if (CPLX) {
+    float2* yp = (float2*)y;
+    const double2 v = ((const double2*)x)[i];
+    float2 old{};
+    if (use_beta) old = yp[i];
+    double2 r{v.x, v.y};
+    if (use_beta) {
+      r.x += b_re * (double)old.x - b_im * (double)old.y;
+      r.y += b_re * (double)old.y + b_im * (double)old.x;
+    }
+    yp[i] = float2{(float)r.x, (float)r.y};
+  } else {
+    float* yp = (float*)y;
+    double r = ((const double*)x)[i];
+    if (use_beta) r = fma(b_re, (double)yp[i], r);
+    yp[i] = (float)r;
+  }
+}
+
 // y = alpha * (x + i*x2) + beta * y over complex elements; x2 may be null.
 __global__ void __launch_bounds__(kThreads)
     axpby_join_kernel(const double2* __restrict__ x, const double2* __restrict__ x2,
@@ -307,6 +359,42 @@ hipError_t launch_scale(hipStream_t s, int cplx, void* x, int64_t n, double b_re
     ITG_LAUNCH(scale_kernel<1>, grid, block, 0, s, (double2*)x, n, b_re, b_im);
   else
     ITG_LAUNCH(scale_kernel<0>, grid, block, 0, s, (double*)x, n, b_re, b_im);
+  return hipGetLastError();
+}
+
+hipError_t launch_widen(hipStream_t s, int cplx, int64_t n, const void* x32, void* x64) {
+  const int64_t m = cplx ? 2 * n : n;
+  if (m <= 0) return hipSuccess;
+  ITG_LAUNCH(widen_kernel, dim3((unsigned)((m + kThreads - 1) / kThreads)), kThreads, 0, s,
+             (const float*)x32, (double*)x64, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(hipStream_t s, int from, int to, int64_t n, const void* x, void* y) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((n + kThreads - 1) / kThreads));
+#define ITG_CONV(F, T, S, D)                                                              \
+  if (from == F && to == T) {                                                            \
+    ITG_LAUNCH((convert_kernel<S, D>), grid, kThreads, 0, s, (const S*)x, (D*)y, n);     \
+    return hipGetLastError();                                                            \
+  }
+  ITG_CONV(kEltF32, kEltF64, float, double)
+  ITG_CONV(kEltF32, kEltC64, float, float2)
+  ITG_CONV(kEltF32, kEltC128, float, double2)
+  ITG_CONV(kEltF64, kEltC128, double, double2)
+  ITG_CONV(kEltC64, kEltC128, float2, double2)
+#undef ITG_CONV
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_narrow(hipStream_t s, int cplx, int64_t n, const void* x64, void* y32,
+                         double b_re, double b_im) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((n + kThreads - 1) / kThreads));
+  if (cplx)
+    ITG_LAUNCH(narrow_kernel<1>, grid, kThreads, 0, s, (const double2*)x64, y32, n, b_re, b_im);
+  else
+    ITG_LAUNCH(narrow_kernel<0>, grid, kThreads, 0, s, (const double*)x64, y32, n, b_re, b_im);
   return hipGetLastError();
 }
 

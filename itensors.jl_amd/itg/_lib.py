@@ -22,7 +22,9 @@ ITG_ERR_UNSUPPORTED = 2
 ITG_ERR_HIP = 3
 ITG_ERR_NOMEM = 4
 
+ITG_F32 = 0
 ITG_F64 = 1
+ITG_C64 = 2
 ITG_C128 = 3
 
 PREC = {"fp64": 0, "split_bf16": 1, "split_int8": 2}
@@ -42,7 +44,7 @@ EXPORTS = (
     "itg_bs_copy", "itg_bs_chain_create", "itg_bs_chain_execute", "itg_bs_chain_destroy",
     "itg_bs_diag_plan_create", "itg_bs_diag_execute",
     "itg_graph_begin", "itg_graph_end", "itg_graph_launch", "itg_graph_destroy",
-    "itg_bs_permute_ex",
+    "itg_bs_permute_ex", "itg_convert",
 )
 
 
@@ -119,6 +121,7 @@ def lib():
         L.itg_free.argtypes = [P, P]
         for n in ("itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod"):
             getattr(L, n).argtypes = [P, P, P, SZ]
+        L.itg_convert.argtypes = [P, I, P, I, P, I64]
         L.itg_graph_begin.argtypes = [P]
         L.itg_graph_end.argtypes = [P, ctypes.POINTER(P)]
         L.itg_graph_launch.argtypes = [P, P]

@@ -608,7 +608,7 @@ def _scaled(T, x):
                                                 dtype=dt), T.inds)
     data = S.data
     if cplx and not data.is_complex():
-        data = data.to(torch.complex128)
+        data = ND.convert(data, torch.complex128)
     out = torch.empty_like(data)
     if data.numel():
         ND.permutedims_(ND.DenseTensor(out, (data.numel(),)), ND.DenseTensor(data, (data.numel(),)),

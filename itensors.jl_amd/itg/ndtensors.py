@@ -116,7 +116,12 @@ def _dtype_code(tdtype):
         return _lib.ITG_F64
     if tdtype == torch.complex128:
         return _lib.ITG_C128
-    raise ItgUnsupported(_lib.ITG_ERR_UNSUPPORTED, f"dtype {tdtype} (Float64 and ComplexF64 only)")
+    if tdtype == torch.float32:
+        return _lib.ITG_F32
+    if tdtype == torch.complex64:
+        return _lib.ITG_C64
+    raise ItgUnsupported(_lib.ITG_ERR_UNSUPPORTED,
+                         f"dtype {tdtype} (Float32, Float64, ComplexF32, ComplexF64 only)")
 
 
 def _ptr(t):
@@ -416,7 +421,7 @@ class DiagBlockSparseTensor:
         if self.uniform is not None:
             return DiagBlockSparseTensor(None, self.blockoffsets, self.blockdims,
                                          uniform=complex(self.uniform), dtype=torch.complex128)
-        return DiagBlockSparseTensor(self.data.to(torch.complex128), self.blockoffsets,
+        return DiagBlockSparseTensor(convert(self.data, torch.complex128), self.blockoffsets,
                                      self.blockdims)
 
 
@@ -461,7 +466,7 @@ def _contract_diag(T1, labels1, T2, labels2, labelsR, alpha=1.0):
     B, D = _as_blocksparse(T1), T2
     if B.eltype != D.eltype:
         if B.eltype == torch.float64:
-            B = BlockSparseTensor(B.data.to(torch.complex128), B.blockoffsets, B.blockdims)
+            B = BlockSparseTensor(convert(B.data, torch.complex128), B.blockoffsets, B.blockdims)
         else:
             D = D.as_complex()
     p = bs_plan(B, labels1, D, labels2, labelsR, diag=True)
@@ -479,15 +484,29 @@ def _contract_diag(T1, labels1, T2, labels2, labelsR, alpha=1.0):
 
 
 def _promote(T1, T2):
-    """Mixed real/complex inputs are promoted by allocation, as
+    """Mixed element types are promoted by allocation to their common type
+    (Float32 < Float64, real < complex; promote_type), as
     dense/tensoralgebra/contract.jl:196-211 does."""
     if T1.eltype == T2.eltype:
         return T1, T2
-    if T1.eltype == torch.float64:
-        T1 = _with_data(T1, T1.data.to(torch.complex128))
-    else:
-        T2 = _with_data(T2, T2.data.to(torch.complex128))
+    dt = torch.promote_types(T1.eltype, T2.eltype)
+    if T1.eltype != dt:
+        T1 = _with_data(T1, convert(T1.data, dt))
+    if T2.eltype != dt:
+        T2 = _with_data(T2, convert(T2.data, dt))
     return T1, T2
+
+
+def convert(data, dtype):
+    """A widened copy of a flat device vector (itg_convert): Float32 ->
+    Float64 / complex, real -> complex with zero imaginary parts."""
+    if data.dtype == dtype:
+        return data
+    out = torch.empty(data.numel(), dtype=dtype, device=data.device)
+    h = _handle(data.device)
+    h.check(_lib.lib().itg_convert(h.h, _dtype_code(data.dtype), _ptr(data), _dtype_code(dtype),
+                                   _ptr(out), data.numel()))
+    return out
 
 
 def _with_data(T, data):

@@ -16,9 +16,12 @@
 #                grouped launch
 #   AD           ChainRulesCore.rrule(::typeof(contract), ::ITensor, ::ITensor)
 #                -- ext/ITensorsChainRulesCoreExt/itensor.jl:94-103, conj in-kernel
-# On ITG_ERR_UNSUPPORTED the methods fall back to the stock path after moving
-# the operands to the host, as the cuTENSOR extension falls back on
-# CUTENSORError (NDTensorscuTENSORExt/contract.jl:45-53).
+# On ITG_ERR_UNSUPPORTED every method falls back to the stock NDTensors path
+# on host copies of the operands and copies the result back, as the cuTENSOR
+# extension falls back to the generic path on CUTENSORError
+# (NDTensorscuTENSORExt/contract.jl:45-53). Mixed element types are promoted
+# on the device first (itg_convert), as NDTensorscuTENSORExt/contract.jl:34-41
+# and dense/tensoralgebra/contract.jl:196-211 do.
 module NDTensorsMI355XExt
 
 using NDTensors
@@ -33,7 +36,9 @@ const libitg = get(ENV, "ITG_LIBRARY", "libitg")
 # ------------------------------------------------------------------ handle
 const ITG_OK = Cint(0)
 const ITG_ERR_UNSUPPORTED = Cint(2)
+const ITG_F32 = Cint(0)
 const ITG_F64 = Cint(1)
+const ITG_C64 = Cint(2)
 const ITG_C128 = Cint(3)
 const PRECISION = Ref{Cint}(0)  # 0 FP64 MFMA, 2 split (int8 slices)
 const HANDLE = Ref{Ptr{Cvoid}}(C_NULL)
@@ -61,7 +66,9 @@ end
 
 using_mi355x_split_precision(on::Bool = true) = (PRECISION[] = on ? Cint(2) : Cint(0))
 
+eltcode(::Type{Float32}) = ITG_F32
 eltcode(::Type{Float64}) = ITG_F64
+eltcode(::Type{ComplexF32}) = ITG_C64
 eltcode(::Type{ComplexF64}) = ITG_C128
 eltcode(T) = nothing
 
@@ -96,15 +103,59 @@ function Base.Array(v::ItgVector{T}) where {T}
 end
 function ItgVector(a::Vector{T}) where {T}
     v = ItgVector{T}(undef, length(a))
+    copyto!(v, a)
+    return v
+end
+function Base.setindex!(v::ItgVector{T}, x, i::Int) where {T}  # scalar access: slow
+    r = Ref{T}(convert(T, x))
+    check(ccall((:itg_memcpy_htod, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ref{T}, Csize_t),
+                handle(), v.ptr + (i - 1) * sizeof(T), r, sizeof(T)))
+    return v
+end
+function Base.copyto!(v::ItgVector{T}, a::Vector{T}) where {T}  # host -> device
+    length(a) <= v.len || throw(DimensionMismatch("source longer than destination"))
     check(ccall((:itg_memcpy_htod, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Csize_t),
                 handle(), v.ptr, a, length(a) * sizeof(T)))
     return v
 end
+function Base.copyto!(a::Vector{T}, v::ItgVector{T}) where {T}  # device -> host
+    v.len <= length(a) || throw(DimensionMismatch("source longer than destination"))
+    check(ccall((:itg_memcpy_dtoh, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Csize_t),
+                handle(), a, v.ptr, v.len * sizeof(T)))
+    return a
+end
+function Base.copyto!(w::ItgVector{T}, v::ItgVector{T}) where {T}  # device -> device
+    v.len <= w.len || throw(DimensionMismatch("source longer than destination"))
+    check(ccall((:itg_memcpy_dtod, libitg), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Csize_t),
+                handle(), w.ptr, v.ptr, v.len * sizeof(T)))
+    return w
+end
+
+# A widened device copy (Float32 -> Float64, real -> complex): the promotion
+# by allocation of dense/tensoralgebra/contract.jl:196-211, on the device.
+function promote_data(v::ItgVector{S}, ::Type{T}) where {S, T}
+    S === T && return v
+    w = ItgVector{T}(undef, v.len)
+    check(ccall((:itg_convert, libitg), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Int64),
+                handle(), eltcode(S), v.ptr, eltcode(T), w.ptr, v.len))
+    return w
+end
+promote_tensor(T::DenseTensor, ::Type{E}) where {E} =
+    eltype(T) === E ? T : tensor(Dense(promote_data(data(T), E)), inds(T))
+promote_tensor(T::BlockSparseTensor, ::Type{E}) where {E} =
+    eltype(T) === E ? T : tensor(BlockSparse(promote_data(data(T), E), blockoffsets(T)), inds(T))
 
 mi355x(T::DenseTensor) = tensor(Dense(ItgVector(Vector(data(T)))), inds(T))
 mi355x(T::BlockSparseTensor) = tensor(BlockSparse(ItgVector(Vector(data(T))), blockoffsets(T)), inds(T))
+# a stored diagonal moves to the device; a uniform one (delta) is a number and stays
+mi355x(T::DiagBlockSparseTensor) = data(T) isa Number ? T :
+    tensor(NDTensors.DiagBlockSparse(ItgVector(Vector(data(T))), NDTensors.diagblockoffsets(T)), inds(T))
 cpu(T::DenseTensor{<:Any, <:Any, <:Dense{<:Any, <:ItgVector}}) = tensor(Dense(Array(data(T))), inds(T))
+cpu(T::DenseTensor) = T
 cpu(T::BlockSparseTensor) = tensor(BlockSparse(Array(data(T)), blockoffsets(T)), inds(T))
+cpu(T::DiagBlockSparseTensor) = data(T) isa ItgVector ?
+    tensor(NDTensors.DiagBlockSparse(Array(data(T)), NDTensors.diagblockoffsets(T)), inds(T)) : T
 
 # --------------------------------------------------------------- dense
 function NDTensors.contract!(
@@ -115,27 +166,34 @@ function NDTensors.contract!(
     )
     R, T1, T2 = unexpose.((exposedR, exposedT1, exposedT2))
     elt = eltype(R)
-    code = eltcode(elt)
-    if code === nothing || eltype(T1) != elt || eltype(T2) != elt
-        error("MI355X contraction needs Float64/ComplexF64 operands of one element type")
+    # promote the inputs to R's element type (NDTensorscuTENSORExt/contract.jl:34-41)
+    if promote_type(elt, eltype(T1), eltype(T2)) !== elt
+        error("In MI355X contraction, input tensors have element types `$(eltype(T1))` and " *
+              "`$(eltype(T2))` while the output has element type `$(elt)`.")
     end
-    a = ComplexF64(α)
-    b = ComplexF64(β)
-    st = ccall((:itg_contract_dense, libitg), Cint,
-               (Ptr{Cvoid}, Cint,
-                Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
-                Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
-                Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
-                Ref{ComplexF64}, Ref{ComplexF64}, Cint),
-               handle(), code,
-               ndims(R), collect(Int64, size(R)), collect(Int64, labelsR), data(R).ptr,
-               ndims(T1), collect(Int64, size(T1)), collect(Int64, labelsT1), data(T1).ptr,
-               ndims(T2), collect(Int64, size(T2)), collect(Int64, labelsT2), data(T2).ptr,
-               a, b, PRECISION[])
+    code = eltcode(elt)
+    st = ITG_ERR_UNSUPPORTED
+    if code !== nothing
+        T1 = promote_tensor(T1, elt)
+        T2 = promote_tensor(T2, elt)
+        st = ccall((:itg_contract_dense, libitg), Cint,
+                   (Ptr{Cvoid}, Cint,
+                    Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
+                    Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
+                    Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
+                    Ref{ComplexF64}, Ref{ComplexF64}, Cint),
+                   handle(), code,
+                   ndims(R), collect(Int64, size(R)), collect(Int64, labelsR), data(R).ptr,
+                   ndims(T1), collect(Int64, size(T1)), collect(Int64, labelsT1), data(T1).ptr,
+                   ndims(T2), collect(Int64, size(T2)), collect(Int64, labelsT2), data(T2).ptr,
+                   ComplexF64(α), ComplexF64(β), PRECISION[])
+    end
     if st == ITG_ERR_UNSUPPORTED
+        # the stock path on host copies (the cuTENSOR extension's fallback,
+        # NDTensorscuTENSORExt/contract.jl:45-53), then the result back
         Rh = cpu(R)
         NDTensors.contract!(Rh, labelsR, cpu(T1), labelsT1, cpu(T2), labelsT2, α, β)
-        copyto!(R, mi355x(Rh))
+        copyto!(data(R), data(Rh))
         return R
     end
     check(st)
@@ -157,45 +215,61 @@ function blockstructure(T::BlockSparseTensor)
     return length(offs), blocks, offs, nsec, secd
 end
 
+function plan_output(plan, labelsR, indsR, elt)
+    nblk, nnz, npairs = Ref{Int64}(), Ref{Int64}(), Ref{Int64}()
+    check(ccall((:itg_bs_plan_sizes, libitg), Cint,
+                (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}), plan, nblk, nnz, npairs))
+    NR = length(labelsR)
+    blocksR = Vector{Int64}(undef, NR * nblk[])
+    offsR = Vector{Int64}(undef, nblk[])
+    check(ccall((:itg_bs_plan_blocks, libitg), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}),
+                plan, blocksR, offsR))
+    boffsR = BlockOffsets{NR}()
+    for k in 1:nblk[]
+        blk = Block(ntuple(i -> UInt(blocksR[(k - 1) * NR + i]), NR))
+        insert!(boffsR, blk, Int(offsR[k]))
+    end
+    R = tensor(BlockSparse(ItgVector{elt}(undef, nnz[]), boffsR), indsR)
+    return R, npairs[]
+end
+
+# the stock block-sparse path on host copies, result moved back
+fallback_contract(T1, labels1, T2, labels2, labelsR) =
+    mi355x(NDTensors.contract(cpu(T1), labels1, cpu(T2), labels2, labelsR))
+
 function NDTensors.contract(
         T1::BlockSparseTensor{<:Any, N1, <:Any, <:BlockSparse{<:Any, <:ItgVector}}, labels1,
         T2::BlockSparseTensor{<:Any, N2, <:Any, <:BlockSparse{<:Any, <:ItgVector}}, labels2,
         labelsR = contract_labels(labels1, labels2)
     ) where {N1, N2}
     elt = promote_type(eltype(T1), eltype(T2))
-    eltype(T1) == eltype(T2) == elt || error("promote both operands to one element type first")
+    eltcode(elt) === nothing && return fallback_contract(T1, labels1, T2, labels2, labelsR)
+    T1 = promote_tensor(T1, elt)
+    T2 = promote_tensor(T2, elt)
     n1, b1, o1, ns1, sd1 = blockstructure(T1)
     n2, b2, o2, ns2, sd2 = blockstructure(T2)
     plan = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ccall((:itg_bs_plan_create, libitg), Cint,
-                (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
-                 Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
-                 Cint, Ptr{Int64}, Ref{Ptr{Cvoid}}),
-                handle(), N1, n1, b1, o1, ns1, sd1, collect(Int64, labels1),
-                N2, n2, b2, o2, ns2, sd2, collect(Int64, labels2),
-                length(labelsR), collect(Int64, labelsR), plan))
+    st = ccall((:itg_bs_plan_create, libitg), Cint,
+               (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                Cint, Ptr{Int64}, Ref{Ptr{Cvoid}}),
+               handle(), N1, n1, b1, o1, ns1, sd1, collect(Int64, labels1),
+               N2, n2, b2, o2, ns2, sd2, collect(Int64, labels2),
+               length(labelsR), collect(Int64, labelsR), plan)
+    st == ITG_ERR_UNSUPPORTED && return fallback_contract(T1, labels1, T2, labels2, labelsR)
+    check(st)
     try
-        nblk, nnz, npairs = Ref{Int64}(), Ref{Int64}(), Ref{Int64}()
-        check(ccall((:itg_bs_plan_sizes, libitg), Cint,
-                    (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}), plan[], nblk, nnz, npairs))
-        NR = length(labelsR)
-        blocksR = Vector{Int64}(undef, NR * nblk[])
-        offsR = Vector{Int64}(undef, nblk[])
-        check(ccall((:itg_bs_plan_blocks, libitg), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}),
-                    plan[], blocksR, offsR))
-        boffsR = BlockOffsets{NR}()
-        for k in 1:nblk[]
-            blk = Block(ntuple(i -> UInt(blocksR[(k - 1) * NR + i]), NR))
-            insert!(boffsR, blk, Int(offsR[k]))
-        end
         indsR = NDTensors.contract_inds(inds(T1), labels1, inds(T2), labels2, labelsR)
-        R = tensor(BlockSparse(ItgVector{elt}(undef, nnz[]), boffsR), indsR)
-        if npairs[] > 0
-            check(ccall((:itg_bs_execute, libitg), Cint,
-                        (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
-                         Ref{ComplexF64}, Ptr{Cvoid}, Cint, Cuint),
-                        handle(), plan[], eltcode(elt), data(R).ptr, data(T1).ptr, data(T2).ptr,
-                        ComplexF64(1), C_NULL, PRECISION[], 0))
+        R, npairs = plan_output(plan[], labelsR, indsR, elt)
+        if npairs > 0
+            st = ccall((:itg_bs_execute, libitg), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                        Ref{ComplexF64}, Ptr{Cvoid}, Cint, Cuint),
+                       handle(), plan[], eltcode(elt), data(R).ptr, data(T1).ptr, data(T2).ptr,
+                       ComplexF64(1), C_NULL, PRECISION[], Cuint(0))
+            st == ITG_ERR_UNSUPPORTED &&
+                return fallback_contract(T1, labels1, T2, labels2, labelsR)
+            check(st)
         end
         return R
     finally
@@ -226,41 +300,40 @@ function NDTensors.contract(
         T2::DiagBlockSparseTensor{<:Any, N2}, labels2,
         labelsR = contract_labels(labels1, labels2)
     ) where {N1, N2}
-    elt = eltype(T1)
-    eltype(T2) == elt || error("promote both operands to one element type first")
+    uniform = data(T2) isa Number
+    # a stored diagonal must be on the device (mi355x); it is promoted with T1
+    elt = promote_type(eltype(T1), eltype(T2))
+    if eltcode(elt) === nothing || !(uniform || data(T2) isa ItgVector)
+        return fallback_contract(T1, labels1, T2, labels2, labelsR)
+    end
+    T1 = promote_tensor(T1, elt)
+    D = uniform ? T2 : tensor(NDTensors.DiagBlockSparse(promote_data(data(T2), elt),
+                                                        NDTensors.diagblockoffsets(T2)), inds(T2))
     n1, b1, o1, ns1, sd1 = blockstructure(T1)
-    n2, b2, o2, ns2, sd2 = diagstructure(T2)
+    n2, b2, o2, ns2, sd2 = diagstructure(D)
     plan = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ccall((:itg_bs_diag_plan_create, libitg), Cint,
-                (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
-                 Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
-                 Cint, Ptr{Int64}, Ref{Ptr{Cvoid}}),
-                handle(), N1, n1, b1, o1, ns1, sd1, collect(Int64, labels1),
-                N2, n2, b2, o2, ns2, sd2, collect(Int64, labels2),
-                length(labelsR), collect(Int64, labelsR), plan))
+    st = ccall((:itg_bs_diag_plan_create, libitg), Cint,
+               (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                Cint, Ptr{Int64}, Ref{Ptr{Cvoid}}),
+               handle(), N1, n1, b1, o1, ns1, sd1, collect(Int64, labels1),
+               N2, n2, b2, o2, ns2, sd2, collect(Int64, labels2),
+               length(labelsR), collect(Int64, labelsR), plan)
+    st == ITG_ERR_UNSUPPORTED && return fallback_contract(T1, labels1, T2, labels2, labelsR)
+    check(st)
     try
-        nblk, nnz, npairs = Ref{Int64}(), Ref{Int64}(), Ref{Int64}()
-        check(ccall((:itg_bs_plan_sizes, libitg), Cint,
-                    (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}), plan[], nblk, nnz, npairs))
-        NR = length(labelsR)
-        blocksR = Vector{Int64}(undef, NR * nblk[])
-        offsR = Vector{Int64}(undef, nblk[])
-        check(ccall((:itg_bs_plan_blocks, libitg), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}),
-                    plan[], blocksR, offsR))
-        boffsR = BlockOffsets{NR}()
-        for k in 1:nblk[]
-            insert!(boffsR, Block(ntuple(i -> UInt(blocksR[(k - 1) * NR + i]), NR)), Int(offsR[k]))
-        end
-        indsR = NDTensors.contract_inds(inds(T1), labels1, inds(T2), labels2, labelsR)
-        R = tensor(BlockSparse(ItgVector{elt}(undef, nnz[]), boffsR), indsR)
-        if npairs[] > 0
-            uniform = data(T2) isa Number
-            check(ccall((:itg_bs_diag_execute, libitg), Cint,
-                        (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
-                         Ptr{ComplexF64}, Ref{ComplexF64}),
-                        handle(), plan[], eltcode(elt), data(R).ptr, data(T1).ptr,
-                        uniform ? C_NULL : data(T2).ptr,
-                        uniform ? Ref(ComplexF64(data(T2))) : C_NULL, ComplexF64(1)))
+        indsR = NDTensors.contract_inds(inds(T1), labels1, inds(D), labels2, labelsR)
+        R, npairs = plan_output(plan[], labelsR, indsR, elt)
+        if npairs > 0
+            st = ccall((:itg_bs_diag_execute, libitg), Cint,
+                       (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                        Ptr{ComplexF64}, Ref{ComplexF64}),
+                       handle(), plan[], eltcode(elt), data(R).ptr, data(T1).ptr,
+                       uniform ? C_NULL : data(D).ptr,
+                       uniform ? Ref(ComplexF64(data(D))) : C_NULL, ComplexF64(1))
+            st == ITG_ERR_UNSUPPORTED &&
+                return fallback_contract(T1, labels1, T2, labels2, labelsR)
+            check(st)
         end
         return R
     finally

@@ -47,6 +47,7 @@ def test_header_constants_match_binding():
     assert val("ITG_ERR_UNSUPPORTED") == _lib.ITG_ERR_UNSUPPORTED
     assert val("ITG_ERR_HIP") == _lib.ITG_ERR_HIP
     assert val("ITG_F64") == _lib.ITG_F64 and val("ITG_C128") == _lib.ITG_C128
+    assert val("ITG_F32") == _lib.ITG_F32 and val("ITG_C64") == _lib.ITG_C64
     assert val("ITG_PREC_FP64") == _lib.PREC["fp64"]
     assert val("ITG_PREC_SPLIT_BF16") == _lib.PREC["split_bf16"]
     assert val("ITG_PREC_SPLIT_INT8") == _lib.PREC["split_int8"]

@@ -226,3 +226,31 @@ def test_random_structures(env, seed):
     D.tensor.data.copy_(torch.from_numpy(rng.standard_normal(D.tensor.data.numel())))
     check(env, A, D)
     check(env, D, A)
+
+
+@pytest.mark.parametrize("cplx_x", [False, True])
+def test_scaled_diagonal_keeps_its_storage(env, cplx_x):
+    """x * diag_itensor and x * delta stay diagonal (scaled values, or a
+    scaled uniform value) and contract like the diagonal they are; the plan
+    checks every operand's data length against its blocks first."""
+    torch, itg, ND, O = env
+    x = (1.5 - 0.5j) if cplx_x else -2.5
+    a, b = qidx(itg, SP, "a"), qidx(itg, SP, "b", dir=itg.In)
+    U = itg.random_itensor(a, b, rng=np.random.default_rng(8))
+    s = np.random.default_rng(9).standard_normal(23)
+    bp = qidx(itg, SP, "bp", dir=itg.In)
+    S = itg.diag_itensor(s, b.dag(), bp)
+    for xS in (x * S, S * x):
+        assert xS.isdiag() and xS.tensor.uniform is None
+        assert xS.tensor.data.numel() == S.tensor.data.numel()
+        C = check(env, U, xS)
+        assert np.allclose(C.array(a, bp), x * U.array(a, b) * s[None, :], rtol=0, atol=1e-14)
+    d = itg.delta(b.dag(), bp)
+    for xd in (x * d, d * x):
+        assert xd.isdiag() and xd.tensor.uniform == (x if cplx_x else x)
+        C = check(env, U, xd)
+        assert np.allclose(C.array(a, bp), x * U.array(a, b), rtol=0, atol=1e-14)
+    # a BlockSparseTensor whose data is shorter than its blocks is refused
+    short = U.tensor.data[:-1]
+    with pytest.raises(ValueError):
+        ND.BlockSparseTensor(short, U.tensor.blockoffsets, U.tensor.blockdims)
