@@ -58,9 +58,9 @@ def parse():
 
 
 CALIB_MIB = 256
-# int8 slice products per FP64 product of the split form (S slices: the
+# int8 digit products per FP64 product of the split form (S = 6 digits: the
 # products v_i w_j with i + j < S)
-SLICE_PRODUCTS = 28
+SLICE_PRODUCTS = 21
 KERNELS = {"fp64": "gemm_f64_kernel", "split_int8": "split_gemm_kernel"}
 # dispatch-name prefixes of every tile variant of each mode's GEMM (the
 # library's timer reports them all under the KERNELS name)

@@ -64,11 +64,12 @@ typedef enum {
   ITG_PREC_FP64 = 0,       /* FP64 MFMA (v_mfma_f64_16x16x4_f64), FP64 accumulate */
   ITG_PREC_SPLIT_BF16 = 1, /* reserved for BF16 slices; returns                 */
                            /* ITG_ERR_UNSUPPORTED in this version                 */
-  ITG_PREC_SPLIT_INT8 = 2  /* FP64 operands split into 7 int8 slices of 7 bits    */
-                           /* each under a power-of-two scale per row (A) / per   */
-                           /* column (B); the 28 slice products of weight >=      */
-                           /* 2^-42 run on INT8 MFMA with exact INT32 sums per    */
-                           /* weight level, recombined and summed in FP64         */
+  ITG_PREC_SPLIT_INT8 = 2  /* FP64 operands split into 6 int8 digits of radix 254 */
+                           /* (|digit| <= 127) under a scale 2^E / 127 per row    */
+                           /* (A) / per column (B); the 21 digit products of      */
+                           /* weight >= 254^-5 run on INT8 MFMA with exact INT32  */
+                           /* sums per weight level, recombined and summed in     */
+                           /* FP64 (about 1e-13 relative Frobenius error)         */
 } itg_prec;
 
 /* Execution flags for itg_bs_execute / itg_contract_dense_ex. */

@@ -319,10 +319,18 @@ hipError_t launch_gemm_f64(hipStream_t s, int cfg, int cplx, int ta, int tb,
                            double beta_im);
 
 // ------------------------------------------------ split-precision GEMM
-// FP64 operands are split into kSliceCount int8 slices per element with one
-// power-of-two scale per row (M row of A, N column of B); see kernels_split.hip.
+// FP64 operands are split into kSliceCount int8 digits per element (radix
+// kSliceRadix, the first digit scaled by kSliceLead / 2^E with one E per row:
+// M row of A, N column of B); see kernels_split.hip.
 constexpr int kSliceModes = 4;  // row / K modes of a sliced operand after merging
-constexpr int kSliceCount = 7;  // int8 slices per FP64 value (7 bits each)
+constexpr int kSliceCount = 6;        // int8 digits per FP64 value
+constexpr double kSliceRadix = 254.0;  // digits in [-127, 127]
+constexpr double kSliceLead = 127.0;   // |first digit| <= 127
+// Longest contracted extent of one pair of the split GEMM: the level sums
+// (at most kSliceCount products of two digits per k) stay below 2^31 in
+// INT32 for 2*K (the complex form stacks two K halves): 6 * 127^2 * 16384 <
+// 2^31. Longer pairs are cut into K chunks summed in FP64.
+constexpr int kSplitKChunk = 8192;
 
 // One block of an operand to slice: rows x K read through mode strides.
 constexpr int kSliceRows = 64;    // rows per slicing workgroup

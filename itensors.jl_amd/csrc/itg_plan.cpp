@@ -1032,7 +1032,7 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
       if (!slice_modes(rows, T.d(b), T.N, d.rdim, d.rstr, &d.nrm, &R) ||
           !slice_modes(ks, T.d(b), T.N, d.kdim, d.kstr, &d.nkm, &K))
         return "too many modes for the split-precision operand reader";
-      if (K > 65536) return "contracted extent above 65536 per block pair";
+      if (K > INT32_MAX / 2) return "contracted extent too large for the split-precision reader";
       d.off = T.offs[b];
       d.R = (int32_t)R;
       d.K = (int32_t)K;
@@ -1061,23 +1061,42 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
     ep->split_why = err;
     return "";  // the FP64 path still works
   }
-  std::vector<SplitPair> sp(pairs.size());
+  // One split pair per K chunk of at most kSplitKChunk (the INT32 level sums
+  // of a chunk are exact; chunks of one output block are summed in FP64 like
+  // pairs). Chunk [k0, k0 + len) of a block's planes starts k0 * R bytes in
+  // (the planes are chunk-major). The groups get their own pair ranges.
+  std::vector<SplitPair> sp;
+  std::vector<int32_t> first(pairs.size() + 1, 0);
   for (size_t p = 0; p < pairs.size(); ++p) {
     const int64_t bp = swap ? pairs[p].blk2 : pairs[p].blk1;
     const int64_t bq = swap ? pairs[p].blk1 : pairs[p].blk2;
-    SplitPair& x = sp[p];
+    SplitPair x;
     std::memset(&x, 0, sizeof(x));
-    x.a_planes = dA[bp].plane_off;
-    x.b_planes = dB[bq].plane_off;
     x.a_pstride = (int64_t)dA[bp].R * dA[bp].Kpad;
     x.b_pstride = (int64_t)dB[bq].R * dB[bq].Kpad;
     x.a_exp = (int32_t)dA[bp].exp_off;
     x.b_exp = (int32_t)dB[bq].exp_off;
-    x.kpad = dA[bp].K == 0 ? 0 : dA[bp].Kpad;
     x.alpha = ep->h_pairs[p].alpha_re;
     x.alpha_im = ep->h_pairs[p].alpha_im;
     if (ep->h_pairs[p].alpha_im != 0.0) ep->cplx_pair_alpha = true;
     ep->split_kmax = std::max<int64_t>(ep->split_kmax, dA[bp].K);
+    const int64_t kpad = dA[bp].K == 0 ? 0 : dA[bp].Kpad;
+    first[p] = (int32_t)sp.size();
+    int64_t k0 = 0;
+    do {
+      const int64_t len = std::min<int64_t>(kpad - k0, kSplitKChunk);
+      x.a_planes = dA[bp].plane_off + k0 * dA[bp].R;
+      x.b_planes = dB[bq].plane_off + k0 * dB[bq].R;
+      x.kpad = (int32_t)len;
+      sp.push_back(x);
+      k0 += len;
+    } while (k0 < kpad);
+  }
+  first[pairs.size()] = (int32_t)sp.size();
+  std::vector<GemmGroup> sg(ep->h_groups);
+  for (GemmGroup& G : sg) {
+    G.p0 = first[G.p0];
+    G.p1 = first[G.p1];
   }
   // one slicing grid over both operands: A's blocks, then B's (side 1)
   std::vector<SliceBlock> dAB(dA);
@@ -1091,7 +1110,8 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
   hipError_t e;
   if ((e = ep->s_desc.upload(dAB.data(), dAB.size() * sizeof(SliceBlock), s)) != hipSuccess ||
       (e = ep->s_prefix.upload(pAB.data(), pAB.size() * sizeof(int64_t), s)) != hipSuccess ||
-      (e = ep->s_pairs.upload(sp.data(), sp.size() * sizeof(SplitPair), s)) != hipSuccess)
+      (e = ep->s_pairs.upload(sp.data(), sp.size() * sizeof(SplitPair), s)) != hipSuccess ||
+      (e = ep->s_groups.upload(sg.data(), sg.size() * sizeof(GemmGroup), s)) != hipSuccess)
     return std::string("device table upload failed: ") + hipGetErrorString(e);
   ep->s_blocks = (int64_t)dAB.size();
   ep->s_grid = pAB.back();
@@ -1282,10 +1302,6 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
     *err = "this split-precision form is not available";
     return hipErrorNotSupported;
   }
-  if (cplx && ep.split_kmax > 32768) {  // the two K halves share one INT32 sum
-    *err = "complex split precision needs a contracted extent <= 32768 per pair";
-    return hipErrorNotSupported;
-  }
   const int nsets = cplx ? 3 : 1;
   const void* A = ep.swap ? T2 : T1;
   const void* B = ep.swap ? T1 : T2;
@@ -1344,7 +1360,7 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
       for (int comp = 0; comp < 2 && e == hipSuccess; ++comp) {
         TimeScope ts(kt, "split_gemm_kernel", s);
         e = launch_split_gemm(s, prec, cplx, wmb, comp, split_sets(comp, conja, conjb),
-                              (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.g_groups.p,
+                              (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.s_groups.p,
                               (const int2*)gl.tiles.p, gl.grid, planesA, planesB, expA, expB,
                               part ? S2 : S1, 1.0, 0.0, part);
       }
@@ -1390,7 +1406,7 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
   for (int comp = 0; comp < (cplx ? 2 : 1) && e == hipSuccess; ++comp) {
     TimeScope ts(kt, "split_gemm_kernel", s);
     e = launch_split_gemm(s, prec, cplx, wmb, comp, split_sets(comp, conja, conjb),
-                          (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.g_groups.p,
+                          (const SplitPair*)ep.s_pairs.p, (const GemmGroup*)ep.s_groups.p,
                           (const int2*)gl.tiles.p, gl.grid, planesA, planesB, expA, expB, C, a_re,
                           beta_k);
   }

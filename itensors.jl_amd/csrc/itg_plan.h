@@ -92,6 +92,7 @@ struct DevBuf {
   hipError_t upload(const void* host, size_t n, hipStream_t s);
 };
 
+
 struct PermuteLaunch {
   int nm = 0;
   int32_t perm[kMaxModes];
@@ -155,7 +156,7 @@ struct ExecPlan {
   int64_t s_blocks = 0, s_grid = 0;  // slicing grid over both operands
   int64_t sA_plane_bytes = 0, sB_plane_bytes = 0, sA_rows = 0, sB_rows = 0;
   int64_t sA_pmax = 0, sB_pmax = 0;  // partial row exponents (ints)
-  DevBuf s_desc, s_prefix, s_pairs;
+  DevBuf s_desc, s_prefix, s_pairs, s_groups;  // s_groups: h_groups over the K-chunked pairs
 };
 
 // Build the execution plan of `pairs` (grouped: all pairs of one output block

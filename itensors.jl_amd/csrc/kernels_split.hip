@@ -4,23 +4,27 @@
 // 1. slice_kernel, once per operand: for every row r of every block (rows =
 //    the GEMM's M modes for A, N modes for B; the K modes are the columns),
 //    E_r = the smallest integer with max_k |x_rk| < 2^E_r, and
-//        x_rk = 2^(E_r-6) * sum_{i=1..S} v_i,rk * 2^(-7(i-1)) + residue,
-//    v_i in [-64, 64] (int8), |residue| <= 2^(E_r - 7S). The slices come
-//    from exact FP64 steps (y = x*2^(6-E); v = rint(y); y = (y-v)*128).
+//        x_rk = 2^E_r / 127 * sum_{i=0..S-1} v_i,rk * 254^(-i) + residue,
+//    v_i in [-127, 127] (int8), |residue| <= 2^E_r / 127 * 254^(1-S) / 2:
+//    y = 127 * (x / 2^E) (|y| < 127; one rounding, 2^-53 relative), then
+//    v = rint(y), y = (y - v) * 254 (|y| <= 127), S times. With S = 6 the
+//    residue is 3.7e-15 of the row maximum (the 7-bit digits of a power-of-two
+//    radix would need 7 digits, 28 products, for the same order).
 //    The operand is read in place through its mode strides (no permutation
 //    pass). Each slice plane holds K padded to a multiple of 32 (zeros) in
 //    16-byte chunks, chunk-major: chunk c of row r at byte (c*R + r)*16, so
 //    that 64 consecutive rows of one chunk are one contiguous 1 KB run for
 //    both the writer and the GEMM's loads.
 // 2. split_gemm_kernel: C = sum_{pairs} alpha_p * A_p B_p, where for every
-//    pair the products v^A_i v^B_j with i + j <= S + 1 run on
+//    pair the products v^A_i v^B_j with i + j <= S - 1 (21 of them) run on
 //    v_mfma_i32_32x32x32_i8, summed per level L = i + j in INT32 -- exact,
-//    since |sum| <= S * 64*64 * K < 2^31 for K <= 65536 (checked by the
-//    planner) -- and recombined in FP64 after the pair's K loop:
-//        C_mn += alpha * 2^(EA_m+EB_n-12) * sum_L 2^(-7(L-2)) S_L,mn.
-//    The error is that of representing A and B with 7S bits per row/column
-//    scale (about 2^-7S relative to the row maximum); no rounding occurs in
-//    the products or the INT32 sums.
+//    since |sum| <= S * 127^2 * K < 2^31 for K <= 2 * kSplitKChunk (the
+//    planner cuts longer pairs into K chunks) -- and recombined in FP64
+//    after the pair's K loop:
+//        C_mn += alpha * 2^(EA_m+EB_n) / 127^2 * sum_L 254^(-L) S_L,mn.
+//    The error is that of the two representations plus the dropped levels
+//    L >= S (same order), about 1e-13 relative Frobenius for N(0,1) data;
+//    no rounding occurs in the products or the INT32 sums.
 // Tiles: 256 threads = 4 wave64s (2x2), (64*WMB) x 64 per workgroup,
 // (32*WMB) x 32 per wave; K steps of 32 (two chunks) double-buffered in LDS.
 // Stages are fetched with global_load_lds_dwordx4 (HBM/L2 -> LDS without
@@ -40,6 +44,8 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
+constexpr double kInvRadix = 1.0 / kSliceRadix;               // rounded: 2^-53 relative
+constexpr double kInvLead2 = 1.0 / (kSliceLead * kSliceLead);  // rounded: 2^-53 relative
 constexpr int kNonFinite = 0x7fff;   // row exponent of a row holding Inf / NaN
 constexpr int kZeroRow = -0x40000000;  // partial exponent of an all-zero K range
 static_assert(kSliceRows == 64 && kSliceKPart % 64 == 0, "slice tiling");
@@ -325,11 +331,12 @@ __global__ void __launch_bounds__(kThreads)
         for (int w = 0; w < 4; ++w) packed[i][w] = 0u;
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        double y = ldexp(c == 0 ? xr[j] : xi[j], 6 - e);
+        // x / 2^E is exact (|.| < 1), the scale by 127 rounds once
+        double y = ldexp(c == 0 ? xr[j] : xi[j], -e) * kSliceLead;
 #pragma unroll
         for (int i = 0; i < S; ++i) {
           const double v = rint(y);
-          y = (y - v) * 128.0;
+          y = (y - v) * kSliceRadix;
           const int32_t dg = c == 2 ? -(int32_t)v : (int32_t)v;
           packed[i][j >> 2] |= ((uint32_t)(dg & 0xff)) << (8 * (j & 3));
         }
@@ -413,7 +420,7 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
                       double beta, int comp, int sets, int nsetsA, int nsetsB, int aim) {
   constexpr int BM = 32 * WM;
   constexpr int NW = 2 * WM;                    // waves
-  constexpr int NSTAGE = WM == 2 ? 2 : 3;
+  constexpr int NSTAGE = 3;
   constexpr int RB = BM / 64;                   // 64-row blocks of the A tile
   constexpr int NL = S;                         // levels 0..S-1
   constexpr int A_ST = S * 2 * BM;              // 16-byte LDS slots of one A stage
@@ -548,8 +555,8 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
       const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
       double h = (double)lev[NL - 1][r];
 #pragma unroll
-      for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[L][r]);
-      double v = ldexp(h, ea + eb - 12);
+      for (int L = NL - 2; L >= 0; --L) h = fma(h, kInvRadix, (double)lev[L][r]);
+      double v = ldexp(h * kInvLead2, ea + eb);
       if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
       acc[r] = fma(ga * (aim ? pr.alpha_im : pr.alpha), v, acc[r]);
     }
@@ -716,8 +723,8 @@ __global__ void __launch_bounds__(128, 1)
         const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
         double h = (double)lev[jb][NL - 1][r];
 #pragma unroll
-        for (int L = NL - 2; L >= 0; --L) h = fma(h, 0.0078125, (double)lev[jb][L][r]);
-        double v = ldexp(h, ea + eb - 12);
+        for (int L = NL - 2; L >= 0; --L) h = fma(h, kInvRadix, (double)lev[jb][L][r]);
+        double v = ldexp(h * kInvLead2, ea + eb);
         if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
         acc[jb][r] = fma(ga * pr.alpha, v, acc[jb][r]);
       }
