@@ -102,6 +102,21 @@ const char* itg_version(void);
  * tile shape; for tests and tuning. ITG_ERR_INVALID outside 0..3. */
 itg_status itg_set_split_tile(itg_handle* h, int tile);
 
+/* Kernel family of the contractions planned on this handle from now on (a
+ * block-sparse plan keeps the family it was created with; dense calls are
+ * planned per family). The counterpart of
+ * NDTensors' gemm_backend switch (NDTensors/src/abstractarray/tensoralgebra/
+ * contract.jl:15-24: :Auto, :BLAS, :Generic):
+ *   0 = Auto    -- by shape: grouped GEMM on MFMA (TTGT) when the
+ *                  flop-weighted mean contracted extent is >= 24, else the
+ *                  fiber or strided kernels;
+ *   1 = BLAS    -- the grouped GEMM whenever a mode is contracted;
+ *   2 = Generic -- the generic strided kernel for every contraction (each
+ *                  output element summed from strided reads: no
+ *                  permutation, no MFMA).
+ * Results agree to rounding across the three. ITG_ERR_INVALID otherwise. */
+itg_status itg_set_gemm_backend(itg_handle* h, int backend);
+
 /* Per-kernel timing: the device counterpart of NDTensors' global
  * `timer::TimerOutput` (NDTensors/src/NDTensors.jl:96-99) and its
  * `@timeit_debug` sections around BLAS.gemm! and the permutations

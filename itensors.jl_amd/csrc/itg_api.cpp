@@ -37,6 +37,7 @@ struct itg_handle {
   // autotuned tile cfg per (bucket, cplx, prec)
   std::map<std::string, int> tuned;
   int split_tile = 0;  // itg_set_split_tile (0 = per plan)
+  int backend = kBackendAuto;  // itg_set_gemm_backend: plans created from now on
   // itg_graph_begin / _end: the stream being captured, and the caller's
   // stream to restore (a handle on the null stream captures on its own)
   hipStream_t cap_stream = nullptr;  // created on first use, destroyed with the handle
@@ -57,6 +58,7 @@ struct itg_bs_plan {
   std::vector<int64_t> blocksR;  // NR x nblkR, 1-based
   std::vector<PairSpec> pairs;
   std::unique_ptr<ExecPlan> ep;  // pair_alpha = NULL
+  int backend = kBackendAuto;    // the handle's backend when the plan was created
   // one plan per set of per-pair factors used so far (the factors are baked
   // into the device tables). None is freed before the plan is destroyed: a
   // graph captured with one of them keeps reading its tables.
@@ -219,6 +221,49 @@ itg_status run_plan(itg_handle* h, ExecPlan& ep, int cplx, int prec, const void*
   return ITG_OK;
 }
 
+// The execution plan of block-sparse plan p for the per-pair factors
+// pair_alpha (NULL: the plan's own). The factors are baked into the device
+// tables, so each new set builds (and keeps) a plan of its own; with build =
+// false a set without tables is refused instead.
+itg_status plan_for_factors(itg_handle* h, itg_bs_plan* p, const double* pair_alpha, int cplx,
+                            bool build, ExecPlan** out) {
+  *out = p->ep.get();
+  if (!pair_alpha) return ITG_OK;
+  // pair_alpha is given in enumeration order; the plan stores pairs grouped
+  std::vector<size_t> idx(p->pairs.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    const PairSpec &x = p->pairs[a], &y = p->pairs[b];
+    return x.blk1 != y.blk1 ? x.blk1 < y.blk1 : x.blk2 < y.blk2;
+  });
+  std::vector<double> pa(2 * p->pairs.size());
+  for (size_t e = 0; e < idx.size(); ++e) {
+    pa[2 * idx[e]] = pair_alpha[2 * e];
+    pa[2 * idx[e] + 1] = cplx ? pair_alpha[2 * e + 1] : 0.0;
+  }
+  auto it = p->ep_alpha.find(pa);
+  if (it == p->ep_alpha.end()) {
+    if (!build)
+      return fail(h, ITG_ERR_INVALID,
+                  "these per-pair factors have no device tables yet: execute the plan once "
+                  "with them through itg_bs_execute first");
+    // building uploads tables on the stream, which a capture would record
+    // with host sources that do not outlive the call
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+      return fail(h, ITG_ERR_INVALID,
+                  "new per-pair factors during stream capture: execute once before capturing");
+    std::unique_ptr<ExecPlan> np(new ExecPlan);
+    std::string err =
+        build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, pa.data(), h->stream, np.get(),
+                        p->backend);
+    if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
+    it = p->ep_alpha.emplace(pa, std::move(np)).first;
+  }
+  *out = it->second.get();
+  return ITG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -358,6 +403,12 @@ itg_status itg_synchronize(itg_handle* h) {
 itg_status itg_set_split_tile(itg_handle* h, int tile) {
   if (!h || tile < 0 || tile > 3) return ITG_ERR_INVALID;
   h->split_tile = tile;
+  return ITG_OK;
+}
+
+itg_status itg_set_gemm_backend(itg_handle* h, int backend) {
+  if (!h || backend < kBackendAuto || backend > kBackendGeneric) return ITG_ERR_INVALID;
+  h->backend = backend;
   return ITG_OK;
 }
 
@@ -579,6 +630,7 @@ itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype, int NR, const i
   app(labels1, N1);
   app(dims2, N2);
   app(labels2, N2);
+  key.append((const char*)&h->backend, sizeof h->backend);
   ExecPlan* ep = nullptr;
   auto it = h->dense_map.find(key);
   if (it != h->dense_map.end()) {
@@ -606,7 +658,8 @@ itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype, int NR, const i
     if (n1 == 0 || n2 == 0) {
       // nothing to sum: R = beta*R, done by a K = 0 direct pass
     }
-    std::string err = build_exec_plan(st, b1, b2, bR, pairs, nullptr, h->stream, np.get());
+    std::string err =
+        build_exec_plan(st, b1, b2, bR, pairs, nullptr, h->stream, np.get(), h->backend);
     if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
     h->dense_lru.emplace_front(key, std::move(np));
     h->dense_map[key] = h->dense_lru.begin();
@@ -779,9 +832,10 @@ itg_status itg_bs_plan_create(itg_handle* h, int N1, int64_t nblk1, const int64_
   if (st != ITG_OK) return st;
   if (!p->pairs.empty()) {
     DeviceGuard g(h->device);
+    p->backend = h->backend;
     p->ep.reset(new ExecPlan);
-    std::string err =
-        build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, nullptr, h->stream, p->ep.get());
+    std::string err = build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, nullptr, h->stream,
+                                      p->ep.get(), p->backend);
     if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
   }
   *out = p.release();
@@ -971,38 +1025,9 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* 
   }
   const int cplx = dtype == ITG_C128;
   DeviceGuard g(h->device);
-  ExecPlan* ep = p->ep.get();
-  if (pair_alpha) {
-    // per-pair factors are baked into the device tables: rebuild when they change
-    // pair_alpha is given in enumeration order; the plan stores pairs grouped
-    std::vector<PairSpec> enum_order(p->pairs);
-    std::vector<size_t> idx(p->pairs.size());
-    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-      const PairSpec &x = p->pairs[a], &y = p->pairs[b];
-      return x.blk1 != y.blk1 ? x.blk1 < y.blk1 : x.blk2 < y.blk2;
-    });
-    std::vector<double> pa(2 * p->pairs.size());
-    for (size_t e = 0; e < idx.size(); ++e) {
-      pa[2 * idx[e]] = pair_alpha[2 * e];
-      pa[2 * idx[e] + 1] = cplx ? pair_alpha[2 * e + 1] : 0.0;
-    }
-    auto it = p->ep_alpha.find(pa);
-    if (it == p->ep_alpha.end()) {
-      // building uploads tables on the stream, which a capture would record
-      // with host sources that do not outlive the call
-      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-      if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
-        return fail(h, ITG_ERR_INVALID,
-                    "new per-pair factors during stream capture: execute once before capturing");
-      std::unique_ptr<ExecPlan> np(new ExecPlan);
-      std::string err =
-          build_exec_plan(p->st, p->b1, p->b2, p->bR, p->pairs, pa.data(), h->stream, np.get());
-      if (!err.empty()) return fail(h, ITG_ERR_HIP, err);
-      it = p->ep_alpha.emplace(pa, std::move(np)).first;
-    }
-    ep = it->second.get();
-  }
+  ExecPlan* ep = nullptr;
+  itg_status st = plan_for_factors(h, p, pair_alpha, cplx, true, &ep);
+  if (st != ITG_OK) return st;
   return run_plan(h, *ep, cplx, (int)prec, T1, T2, R, flags, alpha, nullptr);
 }
 
@@ -1342,7 +1367,7 @@ itg_status itg_batch_plan_create(itg_handle* h, itg_dtype dtype, int64_t nbatch,
   }
   DeviceGuard g(h->device);
   p->ep.reset(new ExecPlan);
-  m = build_exec_plan(st, L1, L2, LR, pairs, nullptr, h->stream, p->ep.get());
+  m = build_exec_plan(st, L1, L2, LR, pairs, nullptr, h->stream, p->ep.get(), h->backend);
   if (!m.empty()) return fail(h, ITG_ERR_HIP, m);
   *out = p.release();
   return ITG_OK;

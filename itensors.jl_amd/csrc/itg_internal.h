@@ -350,7 +350,8 @@ struct SliceBlock {
   int32_t kdim[kSliceModes], kstr[kSliceModes];
 };
 struct SplitPair {
-  int64_t a_planes, b_planes;    // byte offsets of slice plane 0
+  int64_t a_planes, b_planes;    // byte offsets of slice plane 0 of the block (per slice set)
+  int64_t a_koff, b_koff;        // byte offset of this K chunk inside each plane (k0 * rows)
   int64_t a_pstride, b_pstride;  // bytes between consecutive slice planes
   int32_t a_exp, b_exp;          // first row exponent of the A block / B block
   int32_t kpad, pad;

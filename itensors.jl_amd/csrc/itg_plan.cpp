@@ -657,7 +657,8 @@ hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const
 // ----------------------------------------------------------- exec plan
 std::string build_exec_plan(const Structure& st, const BlockList& b1, const BlockList& b2,
                             const BlockList& bR, const std::vector<PairSpec>& pairs,
-                            const double* pair_alpha, hipStream_t s, ExecPlan* ep) {
+                            const double* pair_alpha, hipStream_t s, ExecPlan* ep,
+                            int backend) {
   ep->st = st;
   const int NR = st.NR, nk = st.nk;
   // per-pair shape figures
@@ -682,7 +683,9 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
   ep->flops = flops;
   ep->max_dim = maxdim;
   const double kavg = mn > 0 ? mnk / mn : 0.0;
-  const bool use_gemm = nk > 0 && kavg >= 24.0;
+  const bool use_gemm = backend == kBackendGeneric ? false
+                       : backend == kBackendGemm  ? nk > 0
+                                                  : nk > 0 && kavg >= 24.0;
 
   // groups: consecutive pairs sharing blkR
   std::vector<std::pair<size_t, size_t>> granges;
@@ -704,7 +707,8 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
 
   if (!use_gemm) {
     std::string ferr;
-    if (build_fiber_tables(st, b1, b2, bR, pairs, granges, pair_alpha, s, ep, &ferr)) {
+    if (backend != kBackendGeneric &&
+        build_fiber_tables(st, b1, b2, bR, pairs, granges, pair_alpha, s, ep, &ferr)) {
       ep->strategy = ExecPlan::FIBER;
       return ferr;
     }
@@ -1085,8 +1089,10 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
     int64_t k0 = 0;
     do {
       const int64_t len = std::min<int64_t>(kpad - k0, kSplitKChunk);
-      x.a_planes = dA[bp].plane_off + k0 * dA[bp].R;
-      x.b_planes = dB[bq].plane_off + k0 * dB[bq].R;
+      x.a_planes = dA[bp].plane_off;
+      x.b_planes = dB[bq].plane_off;
+      x.a_koff = k0 * dA[bp].R;
+      x.b_koff = k0 * dB[bq].R;
       x.kpad = (int32_t)len;
       sp.push_back(x);
       k0 += len;

@@ -162,9 +162,16 @@ struct ExecPlan {
 // Build the execution plan of `pairs` (grouped: all pairs of one output block
 // must be contiguous, in plan order). alpha per pair is taken from
 // pair_alpha (2 doubles per pair) or 1.
+// backend: kBackendAuto lets the shape decide (GEMM for a flop-weighted
+// mean contracted extent >= 24, else FIBER or DIRECT); kBackendGemm takes
+// the GEMM (TTGT on MFMA) whenever a mode is contracted; kBackendGeneric
+// takes the generic strided kernel (DIRECT: no permutation, no MFMA) for
+// every contraction.
+constexpr int kBackendAuto = 0, kBackendGemm = 1, kBackendGeneric = 2;
 std::string build_exec_plan(const Structure& st, const BlockList& b1, const BlockList& b2,
                             const BlockList& bR, const std::vector<PairSpec>& pairs,
-                            const double* pair_alpha, hipStream_t s, ExecPlan* ep);
+                            const double* pair_alpha, hipStream_t s, ExecPlan* ep,
+                            int backend = kBackendAuto);
 
 // Slice and pair tables of the split-precision GEMM (sets split_ok / split_why).
 std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList& Q,

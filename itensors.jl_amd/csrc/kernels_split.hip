@@ -489,11 +489,11 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
     if (CPLX) {
       const int seta = h == 0 ? (sets & 3) : ((sets >> 4) & 3);
       const int setb = h == 0 ? ((sets >> 2) & 3) : ((sets >> 6) & 3);
-      pa = planesA + (int64_t)nsetsA * pr.a_planes + (int64_t)seta * S * pr.a_pstride;
-      pb = planesB + (int64_t)nsetsB * pr.b_planes + (int64_t)setb * S * pr.b_pstride;
+      pa = planesA + (int64_t)nsetsA * pr.a_planes + pr.a_koff + (int64_t)seta * S * pr.a_pstride;
+      pb = planesB + (int64_t)nsetsB * pr.b_planes + pr.b_koff + (int64_t)setb * S * pr.b_pstride;
     } else {
-      pa = planesA + pr.a_planes;
-      pb = planesB + pr.b_planes;
+      pa = planesA + pr.a_planes + pr.a_koff;
+      pb = planesB + pr.b_planes + pr.b_koff;
     }
     // Stage ring: step s computes from stage s % NSTAGE and fetches step
     // s + NSTAGE - 1 into the stage step s - 1 used (every wave finished
@@ -625,8 +625,8 @@ __global__ void __launch_bounds__(128, 1)
     const SplitPair pr = pairs[p];
     const int kpad = pr.kpad;
     if (kpad <= 0) continue;
-    const int8_t* pa = planesA + pr.a_planes;
-    const int8_t* pb = planesB + pr.b_planes;
+    const int8_t* pa = planesA + pr.a_planes + pr.a_koff;
+    const int8_t* pb = planesB + pr.b_planes + pr.b_koff;
     v16i lev[2][NL];
 #pragma unroll
     for (int jb = 0; jb < 2; ++jb)

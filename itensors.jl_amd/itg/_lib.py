@@ -44,7 +44,7 @@ EXPORTS = (
     "itg_bs_copy", "itg_bs_chain_create", "itg_bs_chain_execute", "itg_bs_chain_destroy",
     "itg_bs_diag_plan_create", "itg_bs_diag_execute",
     "itg_graph_begin", "itg_graph_end", "itg_graph_launch", "itg_graph_destroy",
-    "itg_bs_permute_ex", "itg_convert",
+    "itg_bs_permute_ex", "itg_convert", "itg_set_gemm_backend",
 )
 
 
@@ -128,6 +128,7 @@ def lib():
         L.itg_graph_destroy.argtypes = [P]
         L.itg_timer_enable.argtypes = [P, I]
         L.itg_set_split_tile.argtypes = [P, I]
+        L.itg_set_gemm_backend.argtypes = [P, I]
         L.itg_timer_count.argtypes = [P]
         L.itg_timer_read.argtypes = [P, I64, ctypes.POINTER(ctypes.c_char_p), I64P, DP]
         for name in EXPORTS:
@@ -196,6 +197,10 @@ class Handle:
 _handles = threading.local()
 
 
+# itg_set_gemm_backend code applied to every handle (ndtensors.backend_*)
+backend = [0]
+
+
 def handle_for(device_index):
     d = getattr(_handles, "d", None)
     if d is None:
@@ -203,4 +208,11 @@ def handle_for(device_index):
     h = d.get(device_index)
     if h is None:
         h = d[device_index] = Handle(device_index)
+        if backend[0]:
+            h.check(lib().itg_set_gemm_backend(h.h, backend[0]))
     return h
+
+
+def open_handles():
+    """The handles this thread has opened (one per device)."""
+    return list(getattr(_handles, "d", {}).values())

@@ -55,11 +55,16 @@ def precision() -> str:
     return _precision[0]
 
 
-# The reference's algorithm switches, kept for API parity. On the device a
-# block-sparse plan is one grouped launch whatever the threading switch says,
-# there is no TBLIS, and the GEMM backend is libitg's own kernels, so these
-# record the setting and change nothing.
+# The reference's algorithm switches. gemm_backend selects the kernel family
+# of contractions planned afterwards (itg_set_gemm_backend on every device
+# handle): "Auto" by shape, "BLAS" the grouped MFMA GEMM whenever a mode is
+# contracted, "Generic" the generic strided kernel. The threaded-block-sparse
+# and TBLIS switches are accepted and change nothing: on the device every
+# block-sparse plan runs all its output blocks in one grouped launch (the
+# parallelism the threaded switch turns on in the reference), and there is
+# no TBLIS.
 _flags = {"threaded_blocksparse": False, "tblis": False, "gemm_backend": "Auto"}
+_BACKEND_CODE = {"Auto": 0, "BLAS": 1, "Generic": 2}
 
 
 def using_threaded_blocksparse() -> bool:
@@ -98,16 +103,28 @@ def gemm_backend() -> str:
     return _flags["gemm_backend"]
 
 
+def _set_backend(name):
+    _flags["gemm_backend"] = name
+    _lib.backend[0] = _BACKEND_CODE[name]
+    for h in _lib.open_handles():
+        h.check(_lib.lib().itg_set_gemm_backend(h.h, _BACKEND_CODE[name]))
+
+
 def backend_auto():
-    _flags["gemm_backend"] = "Auto"
+    """Plans made from now on pick their kernels by shape."""
+    _set_backend("Auto")
 
 
 def backend_blas():
-    _flags["gemm_backend"] = "BLAS"
+    """Plans made from now on use the grouped MFMA GEMM (TTGT) whenever a mode
+    is contracted (the reference's BLAS.gemm! path)."""
+    _set_backend("BLAS")
 
 
 def backend_generic():
-    _flags["gemm_backend"] = "Generic"
+    """Plans made from now on use the generic strided kernel (no permutation,
+    no MFMA; the reference's generic matmul path)."""
+    _set_backend("Generic")
 
 
 # ------------------------------------------------------------------ types
@@ -705,7 +722,7 @@ def bs_plan(T1, labels1, T2, labels2, labelsR, diag=False):
     structure repeats every time a network is applied). diag: T2 is a
     DiagBlockSparseTensor."""
     key = (_structure_key(T1), tuple(labels1), _structure_key(T2), tuple(labels2), tuple(labelsR),
-           T1.data.device.index, diag)
+           T1.data.device.index, diag, _flags["gemm_backend"])
     p = _plan_cache.get(key)
     if p is not None:
         _plan_cache.move_to_end(key)

@@ -66,6 +66,16 @@ end
 
 using_mi355x_split_precision(on::Bool = true) = (PRECISION[] = on ? Cint(2) : Cint(0))
 
+# NDTensors.gemm_backend[] (:Auto / :BLAS / :Generic, NDTensors/src/abstractarray/
+# tensoralgebra/contract.jl:15-24) selects the device kernel family of the plans
+# made from here on: by shape, the grouped MFMA GEMM, or the strided kernel.
+const BACKEND_CODE = Dict(:Auto => Cint(0), :BLAS => Cint(1), :Generic => Cint(2))
+function sync_backend()
+    code = get(BACKEND_CODE, NDTensors.gemm_backend[], Cint(0))
+    check(ccall((:itg_set_gemm_backend, libitg), Cint, (Ptr{Cvoid}, Cint), handle(), code))
+    return nothing
+end
+
 eltcode(::Type{Float32}) = ITG_F32
 eltcode(::Type{Float64}) = ITG_F64
 eltcode(::Type{ComplexF32}) = ITG_C64
@@ -176,6 +186,7 @@ function NDTensors.contract!(
     if code !== nothing
         T1 = promote_tensor(T1, elt)
         T2 = promote_tensor(T2, elt)
+        sync_backend()
         st = ccall((:itg_contract_dense, libitg), Cint,
                    (Ptr{Cvoid}, Cint,
                     Cint, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid},
@@ -249,6 +260,7 @@ function NDTensors.contract(
     n1, b1, o1, ns1, sd1 = blockstructure(T1)
     n2, b2, o2, ns2, sd2 = blockstructure(T2)
     plan = Ref{Ptr{Cvoid}}(C_NULL)
+    sync_backend()
     st = ccall((:itg_bs_plan_create, libitg), Cint,
                (Ptr{Cvoid}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
                 Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},

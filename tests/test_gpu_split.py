@@ -163,6 +163,31 @@ def test_split_large_dense_tiles(env, cplx):
         assert err < SPLIT_RTOL, err
 
 
+@pytest.mark.parametrize("cplx", [False, True], ids=["f64", "c128"])
+@pytest.mark.parametrize("K", [8192, 8200, 20480])
+def test_split_long_k_chunks(env, cplx, K):
+    """Contracted extents above one K chunk (8192: the INT32 level sums of a
+    chunk are exact, chunks are summed in FP64): one chunk exactly, a chunk
+    plus a partial one, and the 20480 of the dense chi = 4096 network's last
+    step. Complex operands keep three slice sets per block, so a chunk's
+    offset inside a set's planes must not be scaled by the set count. Also
+    the conjugated complex product. FP64 numpy products are the reference."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(K + cplx)
+    M, N = 96, 72
+    A = rand(rng, (M, K), cplx)
+    B = rand(rng, (K, N), cplx)
+    got = run(env, A, (1, -1), B, (-1, 2), (1, 2))
+    err = relerr(got, A @ B)
+    assert err < SPLIT_RTOL, err
+    if cplx:
+        TA, TB = ND.DenseTensor.from_numpy(A), ND.DenseTensor.from_numpy(B)
+        T = ND.contract(TA, (1, -1), TB, (-1, 2), (1, 2), prec=PREC, conj1=True)
+        torch.cuda.synchronize()
+        err = relerr(T.to_numpy(), np.conj(A) @ B)
+        assert err < SPLIT_RTOL, err
+
+
 @pytest.mark.parametrize("shape", [(200, 150, 300), (70, 64, 20), (64, 130, 2000)])
 def test_split_tile_shapes_agree(env, shape):
     """The split tile shapes (64 x 64, 128 x 64, 64 x 64 with A in registers)
