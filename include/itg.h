@@ -267,6 +267,31 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype,
 
 itg_status itg_bs_plan_destroy(itg_bs_plan* p);
 
+/* Size of the library's fixed device scratch buffer (64 MiB per GPU). */
+#define ITG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* itg_bs_execute without any device allocation, for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc. No counterpart in the reference (NDTensors
+ * allocates its outputs and temporaries as it goes). Same arguments and
+ * results as itg_bs_execute, with three limits:
+ *  - the workspace comes from a fixed device array of
+ *    ITG_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ *    the process ends (its address is resolved per device by itg_init); a
+ *    contraction needing more returns ITG_ERR_INVALID with the size;
+ *  - the plan's device tables for this element type, precision and set of
+ *    per-pair factors must exist: execute the plan once through
+ *    itg_bs_execute first (ITG_ERR_INVALID otherwise);
+ *  - Float64 / ComplexF64 only (ITG_ERR_UNSUPPORTED for Float32 /
+ *    ComplexF32, whose widened copies would need memory).
+ * Every call on one GPU shares that GPU's scratch array and it keeps the
+ * last call's contents: calls that overlap on one GPU (several streams or
+ * host threads) race, so serialise them. */
+itg_status itg_bs_execute_noalloc(itg_handle* h, itg_bs_plan* p, itg_dtype dtype,
+                                  void* R, const void* T1, const void* T2,
+                                  const double* alpha, const double* pair_alpha,
+                                  itg_prec prec, unsigned flags);
+
 /* Block-sparse tensor T1 times a block-diagonal tensor D (DiagBlockSparse:
  * every stored block has equal block coordinates (k, ..., k) and holds only
  * its diagonal, min(block dims) elements at offsD[b]). Same arguments as

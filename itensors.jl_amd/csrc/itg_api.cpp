@@ -266,6 +266,57 @@ itg_status plan_for_factors(itg_handle* h, itg_bs_plan* p, const double* pair_al
 
 }  // namespace
 
+namespace itg {
+
+itg_status bs_execute_on(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* R,
+                         const void* T1, const void* T2, const double* alpha,
+                         const double* pair_alpha, itg_prec prec, unsigned flags, void* ws,
+                         size_t ws_bytes) {
+  if (!h || !p) return ITG_ERR_INVALID;
+  if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
+    return fail(h, ITG_ERR_INVALID, "unknown precision mode");
+  if (dtype32(dtype))
+    return fail(h, ITG_ERR_UNSUPPORTED,
+                "Float32 / ComplexF32 operands need widened copies: use itg_bs_execute");
+  if (!dtype_ok(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
+  if (p->diag) return fail(h, ITG_ERR_INVALID, "diagonal plan (use itg_bs_diag_execute)");
+  if (p->pairs.empty()) return ITG_OK;
+  if (!R || !T1 || !T2) return fail(h, ITG_ERR_INVALID, "null data pointer");
+  if (p->device != h->device) return fail(h, ITG_ERR_INVALID, "plan built on another device");
+  if (!ws) return fail(h, ITG_ERR_INVALID, "no workspace");
+  const int cplx = dtype == ITG_C128;
+  DeviceGuard g(h->device);
+  ExecPlan* ep = nullptr;
+  itg_status st = plan_for_factors(h, p, pair_alpha, cplx, false, &ep);
+  if (st != ITG_OK) return st;
+  const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
+  const size_t need = exec_workspace_bytes(*ep, cplx, (int)prec, a_im != 0.0);
+  if (need > ws_bytes) {
+    char msg[200];
+    std::snprintf(msg, sizeof msg,
+                  "this contraction needs %zu bytes of workspace, above the %zu bytes of the "
+                  "library's fixed scratch buffer: use itg_bs_execute",
+                  need, ws_bytes);
+    return fail(h, ITG_ERR_INVALID, msg);
+  }
+  std::string err;
+  const int cfg = pick_cfg(h, *ep, cplx, (int)prec);
+  hipError_t e = exec_run(*ep, cplx, (int)prec, cfg, h->stream, ws, T1, T2, R, flags, a_re, a_im,
+                          0.0, 0.0, &err, nullptr, false);
+  if (e == hipErrorNotSupported) return fail(h, ITG_ERR_UNSUPPORTED, err);
+  if (e == hipErrorNotReady) return fail(h, ITG_ERR_INVALID, err);
+  if (e != hipSuccess) return fail(h, ITG_ERR_HIP, err);
+  return ITG_OK;
+}
+
+int handle_device(const itg_handle* h) { return h->device; }
+
+itg_status handle_fail(itg_handle* h, itg_status st, const char* msg) {
+  return fail(h, st, msg);
+}
+
+}  // namespace itg
+
 extern "C" {
 
 const char* itg_version(void) { return "0.1.0"; }
@@ -277,6 +328,9 @@ itg_status itg_init(int device, itg_handle** out) {
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || n == 0) return ITG_ERR_HIP;
   if (device < 0 || device >= n) return ITG_ERR_INVALID;
+  // the address of this device's copy of the fixed scratch buffer
+  // (itg_bs_execute_noalloc), resolved once per device
+  if (noalloc_setup(device) != hipSuccess) return ITG_ERR_HIP;
   itg_handle* h = new itg_handle;
   h->device = device;
   *out = h;

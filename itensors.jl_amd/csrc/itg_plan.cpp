@@ -10,6 +10,13 @@
 
 namespace itg {
 
+namespace {
+// Error of a run that may not build tables (exec_run with may_build = false).
+constexpr const char* kTablesMissing =
+    "the launch tables of this tile shape are not built yet: execute the plan once with "
+    "itg_bs_execute in the same element type and precision first";
+}  // namespace
+
 DevBuf::~DevBuf() {
   if (p) (void)hipFree(p);
 }
@@ -1154,11 +1161,12 @@ int gemm_launch_key(int cfg, int cplx) {
   return gemm_cfg_ok(cfg, cplx) ? cfg : 0;
 }
 
-hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s) {
+hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s, bool may_build) {
   if (cfg != kCfgMixed && !gemm_cfg_ok(cfg, cplx)) cfg = 0;
   const int key = gemm_launch_key(cfg, cplx);
   auto it = ep.launches.find(key);
   if (it != ep.launches.end()) return hipSuccess;
+  if (!may_build) return hipErrorNotReady;
   std::unique_ptr<GemmLaunch> gl(new GemmLaunch);
   gl->cfg = cfg;
   hipError_t e;
@@ -1299,7 +1307,7 @@ struct TimeScope {
 hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                       const void* T1, const void* T2, void* R, unsigned flags, double a_re,
                       double a_im, double b_re, double b_im, std::string* err,
-                      KernelTimer* kt) {
+                      KernelTimer* kt, bool may_build) {
   if (!ep.split_ok) {
     *err = "split precision is unavailable for this contraction: " + ep.split_why;
     return hipErrorNotSupported;
@@ -1349,8 +1357,9 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
   // with A in registers; otherwise 64 x 64
   const int wmb = cfg == 2 ? 2 : (cfg == 3 && !cplx ? 3 : 1);
   const int tcfg = wmb == 2 ? 2 : 0;  // the FP64 tile table of the same shape
-  if ((e = ensure_gemm_launch(ep, tcfg, 0, s)) != hipSuccess) {
-    *err = std::string("gemm table upload: ") + hipGetErrorString(e);
+  if ((e = ensure_gemm_launch(ep, tcfg, 0, s, may_build)) != hipSuccess) {
+    *err = e == hipErrorNotReady ? std::string(kTablesMissing)
+                                 : std::string("gemm table upload: ") + hipGetErrorString(e);
     return e;
   }
   GemmLaunch& gl = *ep.launches[tcfg];
@@ -1434,7 +1443,8 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
 
 hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                     const void* T1, const void* T2, void* R, unsigned flags, double a_re,
-                    double a_im, double b_re, double b_im, std::string* err, KernelTimer* kt) {
+                    double a_im, double b_re, double b_im, std::string* err, KernelTimer* kt,
+                    bool may_build) {
   hipError_t e;
   if (ep.strategy == ExecPlan::DIRECT) {
     {
@@ -1470,7 +1480,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
   }
   if (prec != 0)
     return exec_split(ep, cplx, prec, cfg, s, ws, T1, T2, R, flags, a_re, a_im, b_re, b_im, err,
-                      kt);
+                      kt, may_build);
   const size_t es = cplx ? 16 : 8;
   char* w = (char*)ws;
   const void* A = ep.swap ? T2 : T1;
@@ -1511,8 +1521,9 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     gbr = 0.0;
     gbi = 0.0;
   }
-  if ((e = ensure_gemm_launch(ep, cfg, cplx, s)) != hipSuccess) {
-    *err = std::string("gemm table upload: ") + hipGetErrorString(e);
+  if ((e = ensure_gemm_launch(ep, cfg, cplx, s, may_build)) != hipSuccess) {
+    *err = e == hipErrorNotReady ? std::string(kTablesMissing)
+                                 : std::string("gemm table upload: ") + hipGetErrorString(e);
     return e;
   }
   GemmLaunch& gl = *ep.launches[gemm_launch_key(cfg, cplx)];

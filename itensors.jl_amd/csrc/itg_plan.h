@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/itg.h"
 #include "itg_internal.h"
 
 namespace itg {
@@ -190,17 +191,21 @@ hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const
 // Launch the plan. ws: workspace of at least exec_workspace_bytes().
 // alpha_cplx: the call's alpha has a nonzero imaginary part.
 size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec, bool alpha_cplx = false);
-// kt (optional) brackets each launch with timer events.
+// kt (optional) brackets each launch with timer events. may_build = false:
+// the launch tables of the tile shape must already exist (built by an
+// earlier run with the same element type and precision); the call then
+// allocates no device memory and fails with hipErrorNotReady otherwise.
 hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, void* ws,
                     const void* T1, const void* T2, void* R, unsigned flags, double a_re,
                     double a_im, double b_re, double b_im, std::string* err,
-                    KernelTimer* kt = nullptr);
+                    KernelTimer* kt = nullptr, bool may_build = true);
 // Tile cfg picked by the built-in heuristic (the autotuner may override).
 int default_gemm_cfg(const ExecPlan& ep, int cplx);
 // Split-precision tile: 2 (128 x 64, three LDS stages, one workgroup per CU)
 // when that shape still gives two workgroups per CU, else 0 (64 x 64).
 int default_split_cfg(const ExecPlan& ep);
-hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s);
+hipError_t ensure_gemm_launch(ExecPlan& ep, int cfg, int cplx, hipStream_t s,
+                              bool may_build = true);
 // XCD-balanced workgroup -> {group, tile} table for tile shape tc.
 // filter: 0 = every group, 1 = only groups with M, N >= the tile, 2 = the rest.
 std::vector<int2> xcd_tile_table(const ExecPlan& ep, GemmTileCfg tc, int filter = 0);
@@ -220,5 +225,25 @@ std::string build_fiber_chain(const ExecPlan& e1, const ExecPlan& e2, hipStream_
 // Z = (X * B1) * B2 through a ChainPlan (unit scale, beta 0, no conjugation).
 hipError_t exec_chain(const ChainPlan& cp, int cplx, hipStream_t s, const void* X, const void* B1,
                       const void* B2, void* Z, std::string* err, KernelTimer* kt = nullptr);
+
+// ------------------------------------------------ execution without allocation
+// itg_bs_execute on a caller-provided workspace of ws_bytes bytes: no
+// device allocation (the plan's tables for these factors, element type and
+// precision must exist already; ITG_ERR_INVALID otherwise, and when the
+// contraction needs more workspace than ws_bytes). Float32 / ComplexF32 are
+// refused (their widened copies would need memory). itg_api.cpp.
+itg_status bs_execute_on(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* R,
+                         const void* T1, const void* T2, const double* alpha,
+                         const double* pair_alpha, itg_prec prec, unsigned flags, void* ws,
+                         size_t ws_bytes);
+int handle_device(const itg_handle* h);
+// Records msg as the handle's last error and returns st.
+itg_status handle_fail(itg_handle* h, itg_status st, const char* msg);
+// Per-device address of the library's fixed scratch buffer
+// (itg_noalloc.hip): noalloc_setup looks it up once per device (with that
+// device current; called by itg_init), noalloc_scratch returns it (nullptr
+// before setup).
+hipError_t noalloc_setup(int device);
+void* noalloc_scratch(int device);
 
 }  // namespace itg

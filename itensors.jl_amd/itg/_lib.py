@@ -37,7 +37,7 @@ EXPORTS = (
     "itg_init", "itg_destroy", "itg_last_error", "itg_set_stream", "itg_synchronize",
     "itg_version", "itg_permute", "itg_contract_dense", "itg_contract_dense_ex",
     "itg_bs_plan_create", "itg_bs_plan_sizes", "itg_bs_plan_blocks", "itg_bs_plan_pairs",
-    "itg_bs_execute", "itg_bs_plan_destroy", "itg_bs_permute", "itg_bs_autotune",
+    "itg_bs_execute", "itg_bs_execute_noalloc", "itg_bs_plan_destroy", "itg_bs_permute", "itg_bs_autotune",
     "itg_batch_plan_create", "itg_batch_execute", "itg_batch_plan_destroy",
     "itg_malloc", "itg_free", "itg_memcpy_htod", "itg_memcpy_dtoh", "itg_memcpy_dtod",
     "itg_timer_enable", "itg_timer_count", "itg_timer_read", "itg_set_split_tile",
@@ -100,6 +100,7 @@ def lib():
         L.itg_bs_plan_blocks.argtypes = [P, I64P, I64P]
         L.itg_bs_plan_pairs.argtypes = [P, I64P]
         L.itg_bs_execute.argtypes = [P, P, I, P, P, P, DP, DP, I, ctypes.c_uint]
+        L.itg_bs_execute_noalloc.argtypes = L.itg_bs_execute.argtypes
         L.itg_bs_plan_destroy.argtypes = [P]
         L.itg_bs_permute.argtypes = [P, I, I, I64, I64P, I64P, I64P, I32P, P, P]
         L.itg_bs_permute_ex.argtypes = [P, I, I, I64, I64P, I64P, I64P, P, I32P, P, P, DP, DP,
