@@ -407,10 +407,14 @@ __device__ inline v4i lds_read16_untracked(uint32_t addr) {
 }
 
 // WM waves along M (the tile is (32*WM) x 64, 2*WM waves, one 32x32 block
-// each). WM = 2: 64 x 64 tiles, two LDS stages, two workgroups per CU.
-// WM = 4: 128 x 64 tiles, three LDS stages (126 KB) so that every stage is
-// fetched two K steps ahead; one workgroup (eight waves) per CU. Either way a
-// SIMD runs two waves.
+// each). WM = 2: 64 x 64 tiles, three LDS stages of 24 KB (each fetched two
+// K steps ahead), two workgroups per CU. WM = 4: 128 x 64 tiles, four LDS
+// stages of 36 KB (144 KB, each fetched three K steps ahead), one workgroup
+// (eight waves) per CU. Either way a SIMD runs two waves. The fills bound
+// this kernel more than the matrix cores do: with its MFMAs taken out it
+// still takes three quarters of its time, without its fills a little over
+// half; a stage waits for its slowest load, which an L2 miss makes long, so
+// the ring is as deep as the LDS allows.
 template <int S, int CPLX, int WM>
 __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
     split_gemm_kernel(const SplitPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
@@ -420,7 +424,7 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
                       double beta, int comp, int sets, int nsetsA, int nsetsB, int aim) {
   constexpr int BM = 32 * WM;
   constexpr int NW = 2 * WM;                    // waves
-  constexpr int NSTAGE = 3;
+  constexpr int NSTAGE = WM == 2 ? 3 : 4;
   constexpr int RB = BM / 64;                   // 64-row blocks of the A tile
   constexpr int NL = S;                         // levels 0..S-1
   constexpr int A_ST = S * 2 * BM;              // 16-byte LDS slots of one A stage
@@ -428,7 +432,18 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
   constexpr int NLD_A = S * 2 * RB;             // wave loads (64 rows x 16 bytes) per A stage
   constexpr int NLD = NLD_A + S * 2;
   constexpr int NLD_W = (NLD + NW - 1) / NW;    // most loads a wave issues per stage
-  constexpr int kInFlight = NSTAGE == 2 ? 0 : NLD / NW;  // newest-stage loads left in flight
+  constexpr int PER = NLD / NW;                 // fewest loads a wave issues per stage
+  // wait until the loads of every stage but the newest `n` have landed (a
+  // wave issuing more than PER per stage waits a little longer than needed)
+  auto wait_newer = [](int n) {
+    if (n >= 2)
+      wait_vmcnt<2 * PER>();
+    else if (n == 1)
+      wait_vmcnt<PER>();
+    else
+      wait_vmcnt<0>();
+  };
+  static_assert(NSTAGE - 2 <= 2, "wait_newer covers at most two newer stages");
   __shared__ v4i lds[NSTAGE * (A_ST + B_ST)];
 
   const int2 tt = tiles[blockIdx.x];
@@ -483,6 +498,15 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
       }
     };
 
+    // this lane's row and column exponents, fetched before the K loop so
+    // that their latency hides under it (indices clamped into the block: a
+    // row or column past the block only feeds outputs that are not stored)
+    const int ea = expA[pr.a_exp + min(m0 + wm0 + l32, grp.M - 1)];
+    int eb[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      eb[r] = expB[pr.b_exp + min(n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh, grp.N - 1)];
+
     for (int h = 0; h < (CPLX ? 2 : 1); ++h) {
     // slice sets (0 = re, 1 = im, 2 = -im) of this half, chosen by the host
     // for the output component and the conj flags
@@ -501,12 +525,9 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
     // past the previous half's / pair's last barrier, so all stages are free.
     const int nsteps = kpad / BKB;
     issue(0, 0);
-    if (NSTAGE == 3 && nsteps > 1) {
-      issue(BKB, 1);
-      wait_vmcnt<kInFlight>();
-    } else {
-      wait_vmcnt<0>();
-    }
+    int pre = 1;  // stages issued ahead of the first step
+    for (; pre < NSTAGE - 1 && pre < nsteps; ++pre) issue(pre * BKB, pre);
+    wait_newer(pre - 1);  // stage 0 landed
     stage_barrier();
     int st = 0;
     for (int s = 0; s < nsteps; ++s) {
@@ -531,33 +552,32 @@ __global__ void __launch_bounds__(128 * WM, WM == 2 ? 2 : 1)
 #pragma unroll
         for (int j = 0; j < S - i; ++j)  // slice i of A with slice j of B, level i+j
           lev[i + j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bv[j], av[i], lev[i + j], 0, 0, 0);
+      // All 2S fragment reads are issued before the first MFMA, each into its
+      // own registers, and waited for once. Left to itself the scheduler
+      // reuses one register set for the A slices and waits for every read
+      // with lgkmcnt(0) between MFMA groups, draining the matrix pipe.
+      __builtin_amdgcn_sched_group_barrier(0x100, 2 * S, 0);       // DS reads
+      __builtin_amdgcn_sched_group_barrier(0x008, S * (S + 1) / 2, 0);  // MFMAs
       // The MFMAs only touch registers, so without this fence the scheduler
       // sinks them below the wait for the next stage, which then serialises
       // the fetch with the math.
       __builtin_amdgcn_sched_barrier(0);
-      // stage s + 1 must have landed; the loads issued just now (stage
-      // s + NSTAGE - 1) may stay in flight
-      if (NSTAGE == 3 && nxt < nsteps)
-        wait_vmcnt<kInFlight>();
-      else
-        wait_vmcnt<0>();
+      // stage s + 1 must have landed; the stages issued after it may stay in
+      // flight
+      wait_newer(min(nsteps - 1, s + NSTAGE - 1) - (s + 1));
       stage_barrier();
       if (++st == NSTAGE) st = 0;
     }
     }  // halves
 
     // FP64 recombination of the levels, scaled per row/column
-    const int gm = m0 + wm0 + l32;
-    const int ea = (gm < grp.M) ? expA[pr.a_exp + gm] : 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int gn = n0 + wn0 + (r & 3) + 8 * (r >> 2) + 4 * lh;
-      const int eb = (gn < grp.N) ? expB[pr.b_exp + gn] : 0;
       double h = (double)lev[NL - 1][r];
 #pragma unroll
       for (int L = NL - 2; L >= 0; --L) h = fma(h, kInvRadix, (double)lev[L][r]);
-      double v = ldexp(h * kInvLead2, ea + eb);
-      if (ea == kNonFinite || eb == kNonFinite) v = __builtin_nan("");
+      double v = ldexp(h * kInvLead2, ea + eb[r]);
+      if (ea == kNonFinite || eb[r] == kNonFinite) v = __builtin_nan("");
       acc[r] = fma(ga * (aim ? pr.alpha_im : pr.alpha), v, acc[r]);
     }
   }
