@@ -169,36 +169,47 @@ __global__ void __launch_bounds__(kThreads)
     for (int k0 = 0; k0 < K; k0 += BK) {
       const bool more = k0 + BK < K;
       if (more) load_regs(k0 + BK);
+      // every fragment of the slice is read before the first MFMA (each kk
+      // into its own registers), so the reads of later kk land while the
+      // MFMAs of earlier ones run; left to itself the scheduler reuses one
+      // register set and waits for each kk's reads with lgkmcnt(0)
+      constexpr int NKK = BK / 4;
+      double af[NKK][NPLANE][FM], bf[NKK][NPLANE][FN];
 #pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-        double af[NPLANE][FM], bf[NPLANE][FN];
+      for (int q = 0; q < NKK; ++q)
 #pragma unroll
         for (int c = 0; c < NPLANE; ++c) {
 #pragma unroll
-          for (int i = 0; i < FM; ++i) af[c][i] = As[buf][c][kk + lk][wm0 + i * 16 + l16];
+          for (int i = 0; i < FM; ++i)
+            af[q][c][i] = As[buf][c][4 * q + lk][wm0 + i * 16 + l16];
 #pragma unroll
-          for (int j = 0; j < FN; ++j) bf[c][j] = Bs[buf][c][kk + lk][wn0 + j * 16 + l16];
+          for (int j = 0; j < FN; ++j)
+            bf[q][c][j] = Bs[buf][c][4 * q + lk][wn0 + j * 16 + l16];
         }
+#pragma unroll
+      for (int q = 0; q < NKK; ++q)
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
             if (CPLX) {
               // re += Bre*Are - Bim*Aim ; im += Bre*Aim + Bim*Are
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[0][j], af[0][i],
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[q][0][j], af[q][0][i],
                                                                   acc[0][i][j], 0, 0, 0);
               acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                  -bf[NPLANE - 1][j], af[NPLANE - 1][i], acc[0][i][j], 0, 0, 0);
+                  -bf[q][NPLANE - 1][j], af[q][NPLANE - 1][i], acc[0][i][j], 0, 0, 0);
               acc[NPLANE - 1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                  bf[0][j], af[NPLANE - 1][i], acc[NPLANE - 1][i][j], 0, 0, 0);
+                  bf[q][0][j], af[q][NPLANE - 1][i], acc[NPLANE - 1][i][j], 0, 0, 0);
               acc[NPLANE - 1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                  bf[NPLANE - 1][j], af[0][i], acc[NPLANE - 1][i][j], 0, 0, 0);
+                  bf[q][NPLANE - 1][j], af[q][0][i], acc[NPLANE - 1][i][j], 0, 0, 0);
             } else {
-              acc[0][i][j] =
-                  __builtin_amdgcn_mfma_f64_16x16x4f64(bf[0][j], af[0][i], acc[0][i][j], 0, 0, 0);
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[q][0][j], af[q][0][i],
+                                                                  acc[0][i][j], 0, 0, 0);
             }
           }
-      }
+      __builtin_amdgcn_sched_group_barrier(0x100, NKK * NPLANE * (FM + FN), 0);  // DS reads
+      __builtin_amdgcn_sched_group_barrier(0x008, NKK * FM * FN * (CPLX ? 4 : 1), 0);  // MFMAs
+      __builtin_amdgcn_sched_barrier(0);
       // the other buffer was last read in the previous slice, before the
       // barrier that ended it
       if (more) store_lds(buf ^ 1);
