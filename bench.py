@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the dominant "
                          "kernel's HBM traffic")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the achievable-peak measurements (device copy, library FP64 GEMM, "
+                         "library INT8 GEMM)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the other configs of the metric (dense Float64 chi=1024, "
                          "block-sparse ComplexF64 chi=4096, dense ComplexF64 chi=4096)")
@@ -491,6 +494,41 @@ def diag_config(args, torch, dev):
     return r
 
 
+def achievable_peaks(torch, dev):
+    """What this box reaches on the three units the apply uses, measured with
+    library code (not this package's kernels): a 1 GiB device-to-device copy
+    (HBM), an FP64 8192^3 matmul (rocBLAS / hipBLASLt DGEMM on the FP64 matrix
+    cores) and an INT8 8192^3 matmul with INT32 output (torch._int_mm). The
+    roofline fractions use the datasheet peaks; these put them in context."""
+    out = {}
+    n = (1 << 30) // 8
+    src = torch.randn(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    ms = time_calls(torch, lambda: dst.copy_(src), 10)
+    out["hbm_copy_GBps"] = 2 * n * 8 / (ms * 1e-3) / 1e9
+    del src, dst
+    m = 8192
+    a = torch.randn(m, m, dtype=torch.float64, device=dev)
+    b = torch.randn(m, m, dtype=torch.float64, device=dev)
+    ms = time_calls(torch, lambda: torch.mm(a, b), 5)
+    out["fp64_gemm_TFLOPs"] = 2 * m ** 3 / (ms * 1e-3) / 1e12
+    del a, b
+    try:
+        a8 = torch.randint(-127, 128, (m, m), dtype=torch.int8, device=dev)
+        b8 = torch.randint(-127, 128, (m, m), dtype=torch.int8, device=dev)
+        ms = time_calls(torch, lambda: torch._int_mm(a8, b8), 5)
+        out["int8_gemm_TOPs"] = 2 * m ** 3 / (ms * 1e-3) / 1e12
+        del a8, b8
+    except (RuntimeError, AttributeError) as e:  # not every build has an int8 GEMM
+        out["int8_gemm_TOPs"] = None
+        out["int8_gemm_note"] = str(e)[:200]
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    out["note"] = ("measured on this box with library code: 1 GiB device copy (read + write bytes), "
+                   "torch.mm FP64 8192^3, torch._int_mm INT8 8192^3 (INT32 out)")
+    return out
+
+
 def _calib_copy(torch, dev, mib):
     """Five device copies of a mib-MiB buffer: the known-size reference the
     FETCH_SIZE / WRITE_SIZE readings are scaled by."""
@@ -663,6 +701,10 @@ def main():
         torch.cuda.empty_cache()
         out["secondary_configs"] = secondary_configs(args, torch, dev)
         out["diag_blocksparse"] = diag_config(args, torch, dev)
+    if not args.no_peaks:
+        log(args, "[bench] achievable peaks (library copy / FP64 GEMM / INT8 GEMM)")
+        out["achievable_peaks"] = achievable_peaks(torch, dev)
+        log(args, f"[bench]   {json.dumps(out['achievable_peaks'])}")
     if args.calib_copy_mib > 0:
         _calib_copy(torch, dev, args.calib_copy_mib)
     print(json.dumps(out))
