@@ -75,7 +75,22 @@ typedef enum {
 /* Execution flags for itg_bs_execute / itg_contract_dense_ex. */
 enum {
   ITG_FLAG_CONJ_A = 1u, /* use conj(T1) (dag of complex data, src/itensor.jl:1806-1816) */
-  ITG_FLAG_CONJ_B = 2u  /* use conj(T2)                                                  */
+  ITG_FLAG_CONJ_B = 2u, /* use conj(T2)                                                  */
+  /* Fixed operands in split precision (itg_bs_execute / _noalloc only;
+   * ignored in FP64 and by the other entry points). No NDTensors
+   * counterpart: an eigensolver applies one effective Hamiltonian many times
+   * with the same environment tensors, whose digit planes need not be
+   * recomputed each time.
+   *   KEEP_SLICES_1 / _2: slice T1 / T2 into storage owned by the plan (and
+   *     its per-pair factor set) and keep it, remembering the data pointer;
+   *   REUSE_SLICES_1 / _2: do not slice T1 / T2; use the planes a KEEP call
+   *     of this plan left for the same pointer and element type
+   *     (ITG_ERR_INVALID if there are none). The caller guarantees the data
+   *     has not changed since that KEEP call. */
+  ITG_FLAG_KEEP_SLICES_1 = 4u,
+  ITG_FLAG_KEEP_SLICES_2 = 8u,
+  ITG_FLAG_REUSE_SLICES_1 = 16u,
+  ITG_FLAG_REUSE_SLICES_2 = 32u
 };
 
 typedef struct itg_handle itg_handle;

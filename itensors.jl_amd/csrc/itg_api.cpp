@@ -19,6 +19,11 @@
 
 using namespace itg;
 
+static_assert(ITG_FLAG_KEEP_SLICES_1 == kFlagKeep1 && ITG_FLAG_KEEP_SLICES_2 == kFlagKeep2 &&
+                  ITG_FLAG_REUSE_SLICES_1 == kFlagReuse1 && ITG_FLAG_REUSE_SLICES_2 == kFlagReuse2 &&
+                  (ITG_FLAG_CONJ_A | ITG_FLAG_CONJ_B) == kFlagConj,
+              "execution flag values");
+
 struct itg_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -221,6 +226,26 @@ itg_status run_plan(itg_handle* h, ExecPlan& ep, int cplx, int prec, const void*
   return ITG_OK;
 }
 
+// ITG_FLAG_REUSE_SLICES_*: the plan must hold digit planes kept (by a KEEP
+// call) from the same operand pointer with the same number of slice sets.
+// Nothing to check outside split-precision GEMM plans, where the flags do
+// nothing.
+itg_status check_reuse(itg_handle* h, const ExecPlan& ep, int cplx, int prec, const void* T1,
+                       const void* T2, unsigned flags) {
+  if (prec == ITG_PREC_FP64 || ep.strategy != ExecPlan::GEMM || !ep.split_ok) return ITG_OK;
+  for (int op = 0; op < 2; ++op) {
+    const unsigned keep = op == 0 ? kFlagKeep1 : kFlagKeep2;
+    const unsigned reuse = op == 0 ? kFlagReuse1 : kFlagReuse2;
+    if (!(flags & reuse) || (flags & keep)) continue;
+    const int side = ep.swap ? 1 - op : op;
+    if (ep.kept_sets[side] != (cplx ? 3 : 1) || ep.kept_src[side] != (op == 0 ? T1 : T2))
+      return fail(h, ITG_ERR_INVALID,
+                  op == 0 ? "ITG_FLAG_REUSE_SLICES_1: this plan kept no slices of this T1"
+                          : "ITG_FLAG_REUSE_SLICES_2: this plan kept no slices of this T2");
+  }
+  return ITG_OK;
+}
+
 // The execution plan of block-sparse plan p for the per-pair factors
 // pair_alpha (NULL: the plan's own). The factors are baked into the device
 // tables, so each new set builds (and keeps) a plan of its own; with build =
@@ -286,9 +311,13 @@ itg_status bs_execute_on(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* R
   if (!ws) return fail(h, ITG_ERR_INVALID, "no workspace");
   const int cplx = dtype == ITG_C128;
   DeviceGuard g(h->device);
+  if (flags & (kFlagKeep1 | kFlagKeep2))
+    return fail(h, ITG_ERR_INVALID,
+                "ITG_FLAG_KEEP_SLICES_* allocates the plan's slice storage: use itg_bs_execute");
   ExecPlan* ep = nullptr;
   itg_status st = plan_for_factors(h, p, pair_alpha, cplx, false, &ep);
   if (st != ITG_OK) return st;
+  if ((st = check_reuse(h, *ep, cplx, (int)prec, T1, T2, flags)) != ITG_OK) return st;
   const double a_re = alpha ? alpha[0] : 1.0, a_im = (alpha && cplx) ? alpha[1] : 0.0;
   const size_t need = exec_workspace_bytes(*ep, cplx, (int)prec, a_im != 0.0);
   if (need > ws_bytes) {
@@ -619,6 +648,7 @@ itg_status itg_contract_dense_ex(itg_handle* h, itg_dtype dtype, int NR, const i
                                  const double* alpha, const double* beta, itg_prec prec,
                                  unsigned flags) {
   if (!h) return ITG_ERR_INVALID;
+  flags &= kFlagConj;  // the slice-keeping flags are for block-sparse plans only
   if (!dtype_any(dtype)) return fail(h, ITG_ERR_UNSUPPORTED, "unsupported dtype");
   if (prec != ITG_PREC_FP64 && prec != ITG_PREC_SPLIT_BF16 && prec != ITG_PREC_SPLIT_INT8)
     return fail(h, ITG_ERR_INVALID, "unknown precision mode");
@@ -1071,7 +1101,7 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* 
                                    extent_of(p->b2, false), T2, p->bR.nnz, &w);
     if (st != ITG_OK) return st;
     st = itg_bs_execute(h, p, wide_of(dtype), w.r, w.t1, w.t2, alpha, pair_alpha,
-                        ITG_PREC_FP64, flags);
+                        ITG_PREC_FP64, flags & kFlagConj);
     if (st != ITG_OK) return st;
     hipError_t e = launch_narrow(h->stream, dtype == ITG_C64, p->bR.nnz, w.r, R, 0.0, 0.0);
     if (e != hipSuccess) return hip_fail(h, e, "narrow launch");
@@ -1082,6 +1112,7 @@ itg_status itg_bs_execute(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void* 
   ExecPlan* ep = nullptr;
   itg_status st = plan_for_factors(h, p, pair_alpha, cplx, true, &ep);
   if (st != ITG_OK) return st;
+  if ((st = check_reuse(h, *ep, cplx, (int)prec, T1, T2, flags)) != ITG_OK) return st;
   return run_plan(h, *ep, cplx, (int)prec, T1, T2, R, flags, alpha, nullptr);
 }
 

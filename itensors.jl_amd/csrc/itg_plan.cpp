@@ -21,6 +21,20 @@ DevBuf::~DevBuf() {
   if (p) (void)hipFree(p);
 }
 
+hipError_t DevBuf::reserve(size_t n) {
+  if (n <= bytes) return hipSuccess;
+  if (p) {
+    hipError_t e = hipFree(p);
+    if (e != hipSuccess) return e;
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) return e;
+  bytes = n;
+  return hipSuccess;
+}
+
 hipError_t DevBuf::upload(const void* host, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (n > bytes) {
@@ -1121,8 +1135,12 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
     pAB.push_back(pA.back() + pB[b + 1]);
   }
   hipError_t e;
+  ep->sA_blocks = (int64_t)dA.size();
+  ep->sA_grid = pA.back();
+  ep->sB_grid = pB.back();
   if ((e = ep->s_desc.upload(dAB.data(), dAB.size() * sizeof(SliceBlock), s)) != hipSuccess ||
       (e = ep->s_prefix.upload(pAB.data(), pAB.size() * sizeof(int64_t), s)) != hipSuccess ||
+      (e = ep->s_prefixB.upload(pB.data(), pB.size() * sizeof(int64_t), s)) != hipSuccess ||
       (e = ep->s_pairs.upload(sp.data(), sp.size() * sizeof(SplitPair), s)) != hipSuccess ||
       (e = ep->s_groups.upload(sg.data(), sg.size() * sizeof(GemmGroup), s)) != hipSuccess)
     return std::string("device table upload failed: ") + hipGetErrorString(e);
@@ -1335,7 +1353,33 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
   void* pmaxB = w;
   w += align256((size_t)ep.sB_pmax * 4);
   hipError_t e;
-  {
+  // fixed operands: side 0 is A (T1 unless swapped), side 1 is B. A kept
+  // operand's planes and exponents live in the plan's own buffer; a reused
+  // one (checked by the caller: kept from the same source with the same set
+  // count) is not sliced again.
+  const unsigned fk1 = flags & kFlagKeep1, fk2 = flags & kFlagKeep2;
+  const unsigned fr1 = flags & kFlagReuse1, fr2 = flags & kFlagReuse2;
+  const bool keep[2] = {(ep.swap ? fk2 : fk1) != 0, (ep.swap ? fk1 : fk2) != 0};
+  const bool reuse[2] = {!keep[0] && (ep.swap ? fr2 : fr1) != 0,
+                         !keep[1] && (ep.swap ? fr1 : fr2) != 0};
+  void** planes_of[2] = {&planesA, &planesB};
+  void** exps_of[2] = {&expA, &expB};
+  const int64_t pbytes[2] = {ep.sA_plane_bytes, ep.sB_plane_bytes};
+  const int64_t rows[2] = {ep.sA_rows, ep.sB_rows};
+  for (int side = 0; side < 2; ++side) {
+    if (!keep[side] && !reuse[side]) continue;
+    const size_t pb = align256((size_t)pbytes[side] * nsets);
+    if (keep[side]) {
+      if ((e = ep.kept[side].reserve(pb + align256((size_t)rows[side] * 4))) != hipSuccess) {
+        *err = std::string("kept slice storage: ") + hipGetErrorString(e);
+        return e;
+      }
+      ep.kept_sets[side] = 0;  // valid again once sliced below
+    }
+    *planes_of[side] = ep.kept[side].p;
+    *exps_of[side] = (char*)ep.kept[side].p + pb;
+  }
+  if (!reuse[0] || !reuse[1]) {
     SliceOperands ops;
     ops.src[0] = (const double*)A;
     ops.src[1] = (const double*)B;
@@ -1345,14 +1389,28 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
     ops.exps[1] = (int32_t*)expB;
     ops.pmax[0] = (int32_t*)pmaxA;
     ops.pmax[1] = (int32_t*)pmaxB;
+    const SliceBlock* desc = (const SliceBlock*)ep.s_desc.p;
     TimeScope ts(kt, "slice_max_kernel+slice_kernel", s);
-    e = launch_slice(s, prec, cplx, nsets, (const SliceBlock*)ep.s_desc.p,
-                     (const int64_t*)ep.s_prefix.p, ep.s_blocks, ep.s_grid, ops);
+    if (!reuse[0] && !reuse[1])
+      e = launch_slice(s, prec, cplx, nsets, desc, (const int64_t*)ep.s_prefix.p, ep.s_blocks,
+                       ep.s_grid, ops);
+    else if (!reuse[0])
+      e = launch_slice(s, prec, cplx, nsets, desc, (const int64_t*)ep.s_prefix.p, ep.sA_blocks,
+                       ep.sA_grid, ops);
+    else
+      e = launch_slice(s, prec, cplx, nsets, desc + ep.sA_blocks,
+                       (const int64_t*)ep.s_prefixB.p, ep.s_blocks - ep.sA_blocks, ep.sB_grid,
+                       ops);
+    if (e != hipSuccess) {
+      *err = std::string("slice launch: ") + hipGetErrorString(e);
+      return e;
+    }
   }
-  if (e != hipSuccess) {
-    *err = std::string("slice launch: ") + hipGetErrorString(e);
-    return e;
-  }
+  for (int side = 0; side < 2; ++side)
+    if (keep[side]) {
+      ep.kept_src[side] = side == 0 ? A : B;
+      ep.kept_sets[side] = nsets;
+    }
   // cfg 2: 128 x 64 split tiles (three-stage); cfg 3 (real data): 64 x 64
   // with A in registers; otherwise 64 x 64
   const int wmb = cfg == 2 ? 2 : (cfg == 3 && !cplx ? 3 : 1);
@@ -1451,7 +1509,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
       TimeScope ts(kt, "direct_kernel", s);
       e = launch_direct(s, cplx, ep.st.NR, (const DirectPair*)ep.d_pairs.p,
                         (const DirectGroup*)ep.d_groups.p, (const int32_t*)ep.d_koffs.p,
-                        (const int32_t*)ep.d_wgg.p, ep.d_tiles, T1, T2, R, flags, a_re, a_im,
+                        (const int32_t*)ep.d_wgg.p, ep.d_tiles, T1, T2, R, flags & kFlagConj, a_re, a_im,
                         b_re, b_im);
     }
     if (e != hipSuccess) *err = std::string("direct kernel launch: ") + hipGetErrorString(e);
@@ -1460,7 +1518,8 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
   if (ep.strategy == ExecPlan::FIBER) {
     const void* A = ep.f_a_is_2 ? T2 : T1;
     const void* B = ep.f_a_is_2 ? T1 : T2;
-    const unsigned fl = ep.f_a_is_2 ? (((flags & 1u) << 1) | ((flags & 2u) >> 1)) : flags;
+    const unsigned fl =
+        ep.f_a_is_2 ? (((flags & 1u) << 1) | ((flags & 2u) >> 1)) : (flags & kFlagConj);
     if (ep.f_mv) {
       TimeScope ts(kt, "fiber_mv_kernel", s);
       e = launch_fiber_mv(s, cplx, ep.f_nf, ep.f_smax, (const FiberMVGroup*)ep.fm_groups.p,
@@ -1485,7 +1544,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
   char* w = (char*)ws;
   const void* A = ep.swap ? T2 : T1;
   const void* B = ep.swap ? T1 : T2;
-  unsigned gflags = ep.swap ? (((flags & 1u) << 1) | ((flags & 2u) >> 1)) : flags;
+  unsigned gflags = ep.swap ? (((flags & 1u) << 1) | ((flags & 2u) >> 1)) : (flags & kFlagConj);
   if (ep.permA) {
     {
       TimeScope ts(kt, "permute_kernel", s);

@@ -91,7 +91,16 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
   hipError_t upload(const void* host, size_t n, hipStream_t s);
+  // at least n bytes, contents undefined (frees and reallocates when short)
+  hipError_t reserve(size_t n);
 };
+
+// Execution flags beyond the conjugation bits (values of include/itg.h's
+// ITG_FLAG_KEEP_SLICES_1/2 and ITG_FLAG_REUSE_SLICES_1/2): split precision
+// slices a fixed operand into storage owned by the plan and keeps it
+// (KEEP), or uses what an earlier KEEP call left (REUSE) instead of slicing.
+constexpr unsigned kFlagKeep1 = 4u, kFlagKeep2 = 8u, kFlagReuse1 = 16u, kFlagReuse2 = 32u;
+constexpr unsigned kFlagConj = 3u;
 
 
 struct PermuteLaunch {
@@ -158,6 +167,17 @@ struct ExecPlan {
   int64_t sA_plane_bytes = 0, sB_plane_bytes = 0, sA_rows = 0, sB_rows = 0;
   int64_t sA_pmax = 0, sB_pmax = 0;  // partial row exponents (ints)
   DevBuf s_desc, s_prefix, s_pairs, s_groups;  // s_groups: h_groups over the K-chunked pairs
+  // per-operand slicing grids: A's blocks are s_desc[0, sA_blocks) with the
+  // first sA_blocks + 1 entries of s_prefix (sA_grid workgroups); B's are
+  // s_desc[sA_blocks, s_blocks) with s_prefixB (sB_grid workgroups)
+  int64_t sA_blocks = 0, sA_grid = 0, sB_grid = 0;
+  DevBuf s_prefixB;
+  // digit planes and row exponents kept for a fixed operand (side 0 = A,
+  // 1 = B) by ITG_FLAG_KEEP_SLICES_*: the source pointer they were made from
+  // and the number of slice sets (1 real, 3 complex); 0 = none kept
+  DevBuf kept[2];
+  const void* kept_src[2] = {nullptr, nullptr};
+  int kept_sets[2] = {0, 0};
 };
 
 // Build the execution plan of `pairs` (grouped: all pairs of one output block

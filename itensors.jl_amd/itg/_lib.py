@@ -31,6 +31,10 @@ PREC = {"fp64": 0, "split_bf16": 1, "split_int8": 2}
 
 ITG_FLAG_CONJ_A = 1
 ITG_FLAG_CONJ_B = 2
+ITG_FLAG_KEEP_SLICES_1 = 4
+ITG_FLAG_KEEP_SLICES_2 = 8
+ITG_FLAG_REUSE_SLICES_1 = 16
+ITG_FLAG_REUSE_SLICES_2 = 32
 
 # every symbol include/itg.h declares
 EXPORTS = (

@@ -22,15 +22,30 @@ class ContractionSequence:
     """fuse: run consecutive block-sparse steps that admit it (ND.BSChain:
     two fiber-path contractions whose intermediate is only read by the
     next) as one kernel that never stores the intermediate. The result is
-    bit-identical either way; the skipped intermediates are left unwritten."""
+    bit-identical either way; the skipped intermediates are left unwritten.
 
-    def __init__(self, tensors, *, prec=None, fuse=True):
+    fixed: positions in `tensors` whose data stays the same from one run to
+    the next (the environments L and R of an effective Hamiltonian, (0, 4) for
+    the two-site network). In split precision their digit planes are made
+    by the first run and kept by the plans (ITG_FLAG_KEEP_SLICES_*), and later
+    runs reuse them (ITG_FLAG_REUSE_SLICES_*) instead of slicing again; call
+    invalidate_fixed() after changing such a tensor's data. FP64 runs are
+    unaffected."""
+
+    def __init__(self, tensors, *, prec=None, fuse=True, fixed=()):
         import torch
 
         self.tensors = list(tensors)
         self.prec = prec
         self.steps = []
         self.chains = {}  # first step index -> BSChain over it and the next
+        # step -> slice-keeping flag bits for its fixed operands: position 0
+        # is step 0's T1, position j >= 1 is step j-1's T2
+        self._fixed_bits = {}
+        for j in fixed:
+            step, op = (0, 1) if j == 0 else (j - 1, 2)
+            self._fixed_bits[step] = self._fixed_bits.get(step, 0) | op
+        self._kept = set()  # precision modes whose fixed slices are kept
         cur = self.tensors[0]
         for B in self.tensors[1:]:
             lA, lB = compute_contraction_labels(cur.inds, B.inds)
@@ -69,12 +84,30 @@ class ContractionSequence:
         _k, _p, _Y2, B2, Z, _l = self.steps[i + 1]
         self.chains[i].execute(Z.data, X.data, B1.data, B2.data)
 
+    def _flags(self, i, prec):
+        from . import _lib
+
+        ops = self._fixed_bits.get(i, 0)
+        if not ops:
+            return 0
+        reuse = (prec or ND.precision()) in self._kept
+        f = 0
+        if ops & 1:
+            f |= _lib.ITG_FLAG_REUSE_SLICES_1 if reuse else _lib.ITG_FLAG_KEEP_SLICES_1
+        if ops & 2:
+            f |= _lib.ITG_FLAG_REUSE_SLICES_2 if reuse else _lib.ITG_FLAG_KEEP_SLICES_2
+        return f
+
+    def invalidate_fixed(self):
+        """The fixed tensors' data changed: the next run slices them again."""
+        self._kept.clear()
+
     def run_step(self, i, prec=None):
         kind, p, A, B, R, (lA, lB, lR) = self.steps[i]
         prec = prec or self.prec
         if kind == "bs":
             if p.npairs:
-                p.execute(R.data, A.data, B.data, prec=prec)
+                p.execute(R.data, A.data, B.data, prec=prec, flags=self._flags(i, prec))
         else:
             ND.contract_(R, lR, A, lA, B, lB, 1.0, 0.0, prec=prec)
 
@@ -87,6 +120,8 @@ class ContractionSequence:
             else:
                 self.run_step(i, prec)
                 i += 1
+        if self._fixed_bits:
+            self._kept.add(prec or self.prec or ND.precision())
         return self.result
 
     def autotune(self, prec=None):
