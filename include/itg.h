@@ -442,9 +442,9 @@ itg_status itg_batch_plan_destroy(itg_batch_plan* p);
 
 /* ------------------------------------------------------------- autotuner */
 
-/* Time the GEMM tile shapes on this plan's pairs and keep the fastest per
- * (shape bucket, dtype, prec); later executes of any plan use the cached
- * choice. Synchronises the stream. */
+/* Time the GEMM tile shapes on this plan's pairs and keep the fastest for
+ * this plan and (dtype, prec); plans not tuned themselves use the choice
+ * cached for their (shape bucket, dtype, prec). Synchronises the stream. */
 itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype,
                            void* R, const void* T1, const void* T2, itg_prec prec);
 

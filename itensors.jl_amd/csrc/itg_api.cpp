@@ -205,6 +205,7 @@ KernelTimer* active_timer(itg_handle* h) {
 
 int pick_cfg(itg_handle* h, const ExecPlan& ep, int cplx, int prec) {
   if (prec != 0 && h->split_tile > 0) return h->split_tile == 1 ? 0 : h->split_tile;
+  if (ep.tuned[cplx][prec] >= 0) return ep.tuned[cplx][prec];
   auto it = h->tuned.find(tune_key(ep, cplx, prec));
   if (it != h->tuned.end()) return it->second;
   return prec != 0 ? default_split_cfg(ep) : default_gemm_cfg(ep, cplx);
@@ -1368,7 +1369,10 @@ itg_status itg_bs_autotune(itg_handle* h, itg_bs_plan* p, itg_dtype dtype, void*
   (void)hipEventDestroy(t1);
   if (e == hipErrorNotSupported) return fail(h, ITG_ERR_UNSUPPORTED, err);
   if (e != hipSuccess) return fail(h, ITG_ERR_HIP, err);
-  if (best >= 0) h->tuned[tune_key(ep, cplx, (int)prec)] = best;
+  if (best >= 0) {
+    h->tuned[tune_key(ep, cplx, (int)prec)] = best;  // for plans not tuned themselves
+    ep.tuned[cplx][(int)prec] = best;
+  }
   return ITG_OK;
 }
 

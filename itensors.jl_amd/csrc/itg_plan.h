@@ -155,6 +155,7 @@ struct ExecPlan {
   std::vector<GemmGroup> h_groups;
   std::vector<double> group_cost;  // flops per group (for LPT ordering)
   std::map<int, std::unique_ptr<GemmLaunch>> launches;  // per tile cfg
+  int tuned[2][3] = {{-1, -1, -1}, {-1, -1, -1}};      // autotuned cfg per (complex, prec)
   double flops = 0.0;                                   // real flops (2*M*N*K summed)
   int64_t max_dim = 0;
   // split precision (real data): slice tables of the A-side operand P and
