@@ -266,6 +266,12 @@ itg_status itg_bs_plan_sizes(const itg_bs_plan* p, int64_t* nblkR, int64_t* nnzR
 itg_status itg_bs_plan_blocks(const itg_bs_plan* p, int64_t* blocksR, int64_t* offsR);
 /* pairs: 3 x npairs 0-based (block index in T1, in T2, in R), plan order. */
 itg_status itg_bs_plan_pairs(const itg_bs_plan* p, int64_t* pairs);
+/* How the plan runs (no reference counterpart; for callers choosing the
+ * mode order of intermediates): info[0] = strategy (0 strided kernel, 1
+ * fiber kernel, 2 grouped GEMM, -1 empty or diagonal plan); info[1..3] =
+ * 1 when the GEMM permutes T1, T2, or its output before / after the GEMM
+ * (FP64; split precision reads the operands through their strides). */
+itg_status itg_bs_plan_info(const itg_bs_plan* p, int32_t* info);
 
 /* R = alpha * (T1 * T2) on every output block of the plan (each output
  * block is overwritten by its first pair and accumulated by the rest, as

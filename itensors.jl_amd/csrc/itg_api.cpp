@@ -1060,6 +1060,24 @@ itg_status itg_bs_plan_sizes(const itg_bs_plan* p, int64_t* nblkR, int64_t* nnzR
   return ITG_OK;
 }
 
+itg_status itg_bs_plan_info(const itg_bs_plan* p, int32_t* info) {
+  if (!p || !info) return ITG_ERR_INVALID;
+  info[0] = info[1] = info[2] = info[3] = 0;
+  if (!p->ep || p->diag) {
+    info[0] = -1;
+    return ITG_OK;
+  }
+  const ExecPlan& ep = *p->ep;
+  info[0] = ep.strategy == ExecPlan::GEMM ? 2 : ep.strategy == ExecPlan::FIBER ? 1 : 0;
+  if (ep.strategy == ExecPlan::GEMM) {
+    // permA / permB name the GEMM's operands, which are T2 / T1 when swapped
+    info[1] = (ep.swap ? ep.permB : ep.permA) ? 1 : 0;
+    info[2] = (ep.swap ? ep.permA : ep.permB) ? 1 : 0;
+    info[3] = ep.permC ? 1 : 0;
+  }
+  return ITG_OK;
+}
+
 itg_status itg_bs_plan_blocks(const itg_bs_plan* p, int64_t* blocksR, int64_t* offsR) {
   if (!p) return ITG_ERR_INVALID;
   if (blocksR && !p->blocksR.empty())

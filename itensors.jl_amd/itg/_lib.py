@@ -48,7 +48,7 @@ EXPORTS = (
     "itg_bs_copy", "itg_bs_chain_create", "itg_bs_chain_execute", "itg_bs_chain_destroy",
     "itg_bs_diag_plan_create", "itg_bs_diag_execute",
     "itg_graph_begin", "itg_graph_end", "itg_graph_launch", "itg_graph_destroy",
-    "itg_bs_permute_ex", "itg_convert", "itg_set_gemm_backend",
+    "itg_bs_permute_ex", "itg_convert", "itg_set_gemm_backend", "itg_bs_plan_info",
 )
 
 
@@ -103,6 +103,7 @@ def lib():
         L.itg_bs_plan_sizes.argtypes = [P, I64P, I64P, I64P]
         L.itg_bs_plan_blocks.argtypes = [P, I64P, I64P]
         L.itg_bs_plan_pairs.argtypes = [P, I64P]
+        L.itg_bs_plan_info.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
         L.itg_bs_execute.argtypes = [P, P, I, P, P, P, DP, DP, I, ctypes.c_uint]
         L.itg_bs_execute_noalloc.argtypes = L.itg_bs_execute.argtypes
         L.itg_bs_plan_destroy.argtypes = [P]

@@ -613,6 +613,16 @@ class BSPlan:
         self.blocksR = [tuple(int(x) for x in blocks[i * NR:(i + 1) * NR]) for i in range(self.nblkR)]
         self.offsR = [int(x) for x in offs[:self.nblkR]]
 
+    def info(self):
+        """{"strategy": "direct" | "fiber" | "gemm" | None, "perm1", "perm2",
+        "permR"}: how the plan runs (itg_bs_plan_info); the perm flags say
+        whether the FP64 GEMM permutes T1, T2 or its output."""
+        arr = (ctypes.c_int32 * 4)()
+        _lib.lib().itg_bs_plan_info(self.plan, arr)
+        strat = {0: "direct", 1: "fiber", 2: "gemm"}.get(arr[0])
+        return {"strategy": strat, "perm1": bool(arr[1]), "perm2": bool(arr[2]),
+                "permR": bool(arr[3])}
+
     def pairs(self):
         L = _lib.lib()
         arr = np.zeros(max(3 * self.npairs, 1), dtype=np.int64)

@@ -30,11 +30,16 @@ class ContractionSequence:
     by the first run and kept by the plans (ITG_FLAG_KEEP_SLICES_*), and later
     runs reuse them (ITG_FLAG_REUSE_SLICES_*) instead of slicing again; call
     invalidate_fixed() after changing such a tensor's data. FP64 runs are
-    unaffected."""
+    unaffected.
 
-    def __init__(self, tensors, *, prec=None, fuse=True, fixed=()):
-        import torch
+    reorder: let an intermediate produced by a fiber or strided step hold the
+    indices its consumer contracts last (in the consumer's order) when the
+    consumer is a grouped GEMM that would otherwise permute it first (the Z
+    of the two-site network before its product with R). Only the order of
+    intermediates changes: the result's index order and values are the ones
+    of the left-to-right fold of tensor_algebra.jl:121-130."""
 
+    def __init__(self, tensors, *, prec=None, fuse=True, fixed=(), reorder=True):
         self.tensors = list(tensors)
         self.prec = prec
         self.steps = []
@@ -47,24 +52,28 @@ class ContractionSequence:
             self._fixed_bits[step] = self._fixed_bits.get(step, 0) | op
         self._kept = set()  # precision modes whose fixed slices are kept
         cur = self.tensors[0]
-        for B in self.tensors[1:]:
-            lA, lB = compute_contraction_labels(cur.inds, B.inds)
-            lR = ND.contract_labels(lA, lB)
-            indsR = tuple(cur.inds[lA.index(x)] if x in lA else B.inds[lB.index(x)] for x in lR)
-            if cur.hasqns():
-                p = ND.bs_plan(cur.tensor, lA, B.tensor, lB, lR)
-                bd = []
-                for r in lR:
-                    bd.append(cur.tensor.blockdims[lA.index(r)] if r in lA else B.tensor.blockdims[lB.index(r)])
-                data = torch.empty(p.nnzR, dtype=cur.eltype, device=cur.tensor.data.device)
-                R = ND.BlockSparseTensor(data, dict(zip(p.blocksR, p.offsR)), bd)
-                self.steps.append(("bs", p, cur.tensor, B.tensor, R, (lA, lB, lR)))
-            else:
-                dims = tuple(i.dim for i in indsR)
-                R = ND.DenseTensor(torch.empty(math.prod(dims), dtype=cur.eltype,
-                                               device=cur.tensor.data.device), dims)
-                self.steps.append(("dense", None, cur.tensor, B.tensor, R, (lA, lB, lR)))
-            cur = ITensor(R, indsR)
+        nxt = self.tensors[2:] + [None]
+        for k, B in enumerate(self.tensors[1:]):
+            a_it = cur
+            step, cur = self._build_step(a_it, B)
+            # An intermediate's mode order is internal (only the result's is
+            # the reference's): when the next step is a grouped GEMM that
+            # would permute this operand, and this step is not a GEMM (whose
+            # output order could then need a permutation of its own), put the
+            # modes the next step contracts last, in the order the next
+            # operand holds them. The modes that stay free keep their order,
+            # so every later output, the result included, is unchanged.
+            C = nxt[k]
+            if reorder and C is not None and step[0] == "bs" and step[1].npairs and \
+                    step[1].info()["strategy"] != "gemm":
+                probe = self._probe(cur, C)
+                if probe is not None and probe["strategy"] == "gemm" and probe["perm1"]:
+                    order = self._order_for_next(cur.inds, C.inds)
+                    step2, cur2 = self._build_step(a_it, B, out_inds=order)
+                    probe2 = self._probe(cur2, C)
+                    if probe2 is not None and not probe2["perm1"]:
+                        step, cur = step2, cur2
+            self.steps.append(step)
         self.result = cur
         i = 0
         while fuse and i + 1 < len(self.steps):
@@ -77,6 +86,48 @@ class ContractionSequence:
                 except ItgUnsupported:
                     pass
             i += 1
+
+    @staticmethod
+    def _build_step(A, B, out_inds=None):
+        """The step A * B: (step tuple, output ITensor). out_inds: the output's
+        index order (default: the reference's, A's free indices then B's)."""
+        import torch
+
+        lA, lB = compute_contraction_labels(A.inds, B.inds)
+        lR = ND.contract_labels(lA, lB)
+        if out_inds is not None:
+            pos = {}
+            for x in lR:
+                pos[A.inds[lA.index(x)] if x in lA else B.inds[lB.index(x)]] = x
+            lR = tuple(pos[i] for i in out_inds)
+        indsR = tuple(A.inds[lA.index(x)] if x in lA else B.inds[lB.index(x)] for x in lR)
+        if A.hasqns():
+            p = ND.bs_plan(A.tensor, lA, B.tensor, lB, lR)
+            bd = [A.tensor.blockdims[lA.index(r)] if r in lA else B.tensor.blockdims[lB.index(r)]
+                  for r in lR]
+            data = torch.empty(p.nnzR, dtype=A.eltype, device=A.tensor.data.device)
+            R = ND.BlockSparseTensor(data, dict(zip(p.blocksR, p.offsR)), bd)
+            return ("bs", p, A.tensor, B.tensor, R, (lA, lB, lR)), ITensor(R, indsR)
+        dims = tuple(i.dim for i in indsR)
+        R = ND.DenseTensor(torch.empty(math.prod(dims), dtype=A.eltype,
+                                       device=A.tensor.data.device), dims)
+        return ("dense", None, A.tensor, B.tensor, R, (lA, lB, lR)), ITensor(R, indsR)
+
+    @staticmethod
+    def _probe(A, B):
+        """How A * B would run (the plan's info), or None without QNs."""
+        if not A.hasqns():
+            return None
+        lA, lB = compute_contraction_labels(A.inds, B.inds)
+        p = ND.bs_plan(A.tensor, lA, B.tensor, lB, ND.contract_labels(lA, lB))
+        return p.info() if p.npairs else None
+
+    @staticmethod
+    def _order_for_next(inds, next_inds):
+        """inds with the indices `next_inds` shares moved last, in next_inds'
+        order; the others keep their order."""
+        shared = [i for i in next_inds if i in inds]
+        return tuple(i for i in inds if i not in shared) + tuple(shared)
 
     def run_chain(self, i):
         """Steps i and i + 1 as one fused launch."""
