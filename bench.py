@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--diag-only", action="store_true",
                     help="run only the DiagBlockSparse x BlockSparse measurement (chi=4096 psi "
                          "times a diagonal over one bond) and print its JSON")
+    ap.add_argument("--permute-only", action="store_true",
+                    help="run only the block-sparse permutedims measurement (the chi=4096 "
+                         "intermediate Z into the order the last contraction reads) and print "
+                         "its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="(internal) run step 1 in every mode a few times plus a 256 MiB copy "
                          "and exit; the workload of the counter passes")
@@ -494,6 +498,63 @@ def diag_config(args, torch, dev):
     return r
 
 
+def permute_config(args, torch, dev):
+    """Block-sparse permutedims at chi = 4096: the intermediate Z(a', b, s1',
+    s2', w'') of the plain left-to-right fold permuted into the order the
+    last contraction reads it (the contracted b, w'' last; the apply itself
+    writes Z in that order and needs no permutation). One itg_bs_permute
+    call. HBM roofline: algorithmic bytes = nnz read + nnz written, 8 B each.
+    CPU beside it: the oracle's blocksparse_permutedims_ (numpy transpose of
+    every block)."""
+    import numpy as np
+    import itg
+    from collections import OrderedDict
+    from itg import ndtensors as ND
+    from itg.configs import heff_network
+    from itg.network import ContractionSequence
+    from oracle import ndtensors_oracle as O
+
+    L, psi, W1, W2, R, ix = heff_network(4096, qns=True, seed=0, device=dev)
+    Z = ((L * psi) * W1) * W2
+    del L, psi, W1, W2
+    order = ContractionSequence._order_for_next(Z.inds, R.inds)
+    perm = [Z.inds.index(i) + 1 for i in order]
+    torch.cuda.synchronize()
+    kt = kernel_times(torch, lambda: ND.permutedims(Z.tensor, perm), 20)
+    launches, k_ms = kt["permute_kernel"]
+    nbytes = 2 * Z.tensor.nnz() * 8
+    gbs = nbytes / (k_ms * 1e-3) / 1e9
+    r = {"workload": "Z(a', b, s1', s2', w'') -> (a', s1', s2', w'', b), chi=4096 U(1)",
+         "nnz": Z.tensor.nnz(), "perm": perm, "kernel_ms": k_ms, "launches": launches,
+         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes}}
+    if not args.no_cpu:
+        def oi(inds):
+            return tuple(O.Index([(O.QN(*[(n, v, m) for n, (v, m) in q.vals.items()]), d)
+                                  for q, d in i.space], tags=i.tags, plev=i.plev, dir=i.dir,
+                                 id=i.id) for i in inds)
+        zi = oi(Z.inds)
+        offs = OrderedDict((tuple(c - 1 for c in bb), o) for bb, o in Z.tensor.blockoffsets.items())
+        OZ = O.BlockSparse(Z.tensor.data.cpu().numpy(), offs, zi)
+        p0 = [x - 1 for x in perm]
+        empty = O.BlockSparse(np.zeros(0), OrderedDict(), tuple(zi[i] for i in p0))
+        t0 = time.perf_counter()
+        ref = O.blocksparse_permutedims_(empty, OZ, p0)
+        dt = time.perf_counter() - t0
+        r["cpu_baseline"] = {"value": nbytes / dt / 1e9, "unit": "GB/s", "seconds": dt,
+                             "kind": "port", "cores": 1,
+                             "sample": "the same permutation by the numpy restatement"}
+        got = ND.permutedims(Z.tensor, perm)
+        # the device result holds the blocks in Z's order; compare block by block
+        v = got.data.cpu().numpy()
+        same = all(np.array_equal(v[o:o + ref.block(b).size],
+                                  ref.block(b).reshape(-1, order="F"))
+                   for b, o in ((tuple(c - 1 for c in bb), o) for bb, o in got.blockoffsets.items()))
+        r["bit_exact_vs_oracle"] = bool(same)
+    log(args, f"[bench] permute: {json.dumps(r)}")
+    return r
+
+
 def achievable_peaks(torch, dev):
     """What this box reaches on the three units the apply uses, measured with
     library code (not this package's kernels): a 1 GiB device-to-device copy
@@ -513,6 +574,14 @@ def achievable_peaks(torch, dev):
     ms = time_calls(torch, lambda: torch.mm(a, b), 5)
     out["fp64_gemm_TFLOPs"] = 2 * m ** 3 / (ms * 1e-3) / 1e12
     del a, b
+    # the same library on the shape of the apply's central blocks: a batch of
+    # 64 independent 410 x 410 x 410 FP64 products (torch.bmm)
+    bb, s = 64, 410
+    a = torch.randn(bb, s, s, dtype=torch.float64, device=dev)
+    b = torch.randn(bb, s, s, dtype=torch.float64, device=dev)
+    ms = time_calls(torch, lambda: torch.bmm(a, b), 10)
+    out["fp64_bmm_410_TFLOPs"] = 2 * bb * s ** 3 / (ms * 1e-3) / 1e12
+    del a, b
     try:
         a8 = torch.randint(-127, 128, (m, m), dtype=torch.int8, device=dev)
         b8 = torch.randint(-127, 128, (m, m), dtype=torch.int8, device=dev)
@@ -525,7 +594,8 @@ def achievable_peaks(torch, dev):
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     out["note"] = ("measured on this box with library code: 1 GiB device copy (read + write bytes), "
-                   "torch.mm FP64 8192^3, torch._int_mm INT8 8192^3 (INT32 out)")
+                   "torch.mm FP64 8192^3, torch.bmm FP64 64 x 410^3 (the central block shape of "
+                   "the apply), torch._int_mm INT8 8192^3 (INT32 out)")
     return out
 
 
@@ -549,11 +619,11 @@ def main():
     if args.traffic_probe:
         traffic_probe(args)
         return
-    if args.diag_only:
+    if args.diag_only or args.permute_only:
         import torch
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
-        r = diag_config(args, torch, dev)
+        r = (diag_config if args.diag_only else permute_config)(args, torch, dev)
         if args.calib_copy_mib > 0:
             _calib_copy(torch, dev, args.calib_copy_mib)
         print(json.dumps(r))
@@ -726,6 +796,7 @@ def main():
         torch.cuda.empty_cache()
         out["secondary_configs"] = secondary_configs(args, torch, dev)
         out["diag_blocksparse"] = diag_config(args, torch, dev)
+        out["permute_blocksparse"] = permute_config(args, torch, dev)
     if not args.no_peaks:
         log(args, "[bench] achievable peaks (library copy / FP64 GEMM / INT8 GEMM)")
         out["achievable_peaks"] = achievable_peaks(torch, dev)
