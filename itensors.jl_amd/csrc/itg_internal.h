@@ -372,9 +372,10 @@ struct SliceOperands {
 // partial row maxima (as exponents, into pmax), then the slices (planes) and
 // the row exponents (exps). Complex operands: nsets = 3 slice sets (re, im,
 // -im); plane buffers hold nsets x the per-set bytes.
+// Workgroup w slices tile wgmap[w] = (block, tile index within the block:
+// row tile * kparts + K part).
 hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
-                        const int64_t* prefix, int64_t nblk, int64_t grid,
-                        const SliceOperands& ops);
+                        const int2* wgmap, int64_t grid, const SliceOperands& ops);
 // Tile shape of the split GEMM: 64 x 64 (wmb = 1, 256 threads, tile table of
 // gemm cfg 0) or 128 x 64 (wmb = 2, 512 threads, tile table of gemm cfg 2).
 // Complex: one launch per output component comp (0 = re, 1 = im); `sets`

@@ -1134,13 +1134,24 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
     dAB.push_back(d);
     pAB.push_back(pA.back() + pB[b + 1]);
   }
+  // workgroup -> (block, tile within the block): one table read per
+  // workgroup instead of a search over the blocks' workgroup prefix
+  auto wgmap = [](const std::vector<int64_t>& prefix) {
+    std::vector<int2> m((size_t)std::max<int64_t>(prefix.back(), 1), int2{0, 0});
+    for (size_t b = 0; b + 1 < prefix.size(); ++b)
+      for (int64_t w = prefix[b]; w < prefix[b + 1]; ++w)
+        m[(size_t)w] = int2{(int)b, (int)(w - prefix[b])};
+    return m;
+  };
+  const std::vector<int2> mAB = wgmap(pAB), mB = wgmap(pB);
   hipError_t e;
   ep->sA_blocks = (int64_t)dA.size();
   ep->sA_grid = pA.back();
   ep->sB_grid = pB.back();
+  if (pAB.back() > INT32_MAX) return "slicing grid too large";
   if ((e = ep->s_desc.upload(dAB.data(), dAB.size() * sizeof(SliceBlock), s)) != hipSuccess ||
-      (e = ep->s_prefix.upload(pAB.data(), pAB.size() * sizeof(int64_t), s)) != hipSuccess ||
-      (e = ep->s_prefixB.upload(pB.data(), pB.size() * sizeof(int64_t), s)) != hipSuccess ||
+      (e = ep->s_wgmap.upload(mAB.data(), mAB.size() * sizeof(int2), s)) != hipSuccess ||
+      (e = ep->s_wgmapB.upload(mB.data(), mB.size() * sizeof(int2), s)) != hipSuccess ||
       (e = ep->s_pairs.upload(sp.data(), sp.size() * sizeof(SplitPair), s)) != hipSuccess ||
       (e = ep->s_groups.upload(sg.data(), sg.size() * sizeof(GemmGroup), s)) != hipSuccess)
     return std::string("device table upload failed: ") + hipGetErrorString(e);
@@ -1392,15 +1403,12 @@ hipError_t exec_split(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, 
     const SliceBlock* desc = (const SliceBlock*)ep.s_desc.p;
     TimeScope ts(kt, "slice_max_kernel+slice_kernel", s);
     if (!reuse[0] && !reuse[1])
-      e = launch_slice(s, prec, cplx, nsets, desc, (const int64_t*)ep.s_prefix.p, ep.s_blocks,
-                       ep.s_grid, ops);
+      e = launch_slice(s, prec, cplx, nsets, desc, (const int2*)ep.s_wgmap.p, ep.s_grid, ops);
     else if (!reuse[0])
-      e = launch_slice(s, prec, cplx, nsets, desc, (const int64_t*)ep.s_prefix.p, ep.sA_blocks,
-                       ep.sA_grid, ops);
+      e = launch_slice(s, prec, cplx, nsets, desc, (const int2*)ep.s_wgmap.p, ep.sA_grid, ops);
     else
-      e = launch_slice(s, prec, cplx, nsets, desc + ep.sA_blocks,
-                       (const int64_t*)ep.s_prefixB.p, ep.s_blocks - ep.sA_blocks, ep.sB_grid,
-                       ops);
+      e = launch_slice(s, prec, cplx, nsets, desc + ep.sA_blocks, (const int2*)ep.s_wgmapB.p,
+                       ep.sB_grid, ops);
     if (e != hipSuccess) {
       *err = std::string("slice launch: ") + hipGetErrorString(e);
       return e;

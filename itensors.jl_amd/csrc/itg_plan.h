@@ -167,12 +167,14 @@ struct ExecPlan {
   int64_t s_blocks = 0, s_grid = 0;  // slicing grid over both operands
   int64_t sA_plane_bytes = 0, sB_plane_bytes = 0, sA_rows = 0, sB_rows = 0;
   int64_t sA_pmax = 0, sB_pmax = 0;  // partial row exponents (ints)
-  DevBuf s_desc, s_prefix, s_pairs, s_groups;  // s_groups: h_groups over the K-chunked pairs
-  // per-operand slicing grids: A's blocks are s_desc[0, sA_blocks) with the
-  // first sA_blocks + 1 entries of s_prefix (sA_grid workgroups); B's are
-  // s_desc[sA_blocks, s_blocks) with s_prefixB (sB_grid workgroups)
+  DevBuf s_desc, s_pairs, s_groups;  // s_groups: h_groups over the K-chunked pairs
+  // slicing grids: workgroup w of the launch over both operands slices tile
+  // s_wgmap[w] = (block index into s_desc, tile index within the block); A's
+  // blocks are s_desc[0, sA_blocks) and its tiles the first sA_grid entries;
+  // a launch over B alone uses s_desc + sA_blocks with s_wgmapB (sB_grid
+  // workgroups, block indices counted from B's first block)
   int64_t sA_blocks = 0, sA_grid = 0, sB_grid = 0;
-  DevBuf s_prefixB;
+  DevBuf s_wgmap, s_wgmapB;
   // digit planes and row exponents kept for a fixed operand (side 0 = A,
   // 1 = B) by ITG_FLAG_KEEP_SLICES_*: the source pointer they were made from
   // and the number of slice sets (1 real, 3 complex); 0 = none kept

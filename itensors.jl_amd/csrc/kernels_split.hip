@@ -50,18 +50,6 @@ constexpr int kNonFinite = 0x7fff;   // row exponent of a row holding Inf / NaN
 constexpr int kZeroRow = -0x40000000;  // partial exponent of an all-zero K range
 static_assert(kSliceRows == 64 && kSliceKPart % 64 == 0, "slice tiling");
 
-__device__ inline int64_t find_slot(const int64_t* prefix, int64_t n, int64_t wg) {
-  int64_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int64_t mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= wg)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
 __device__ inline int64_t mode_offset(uint32_t idx, int n, const int32_t* dim,
                                       const int32_t* str) {
   int64_t o = 0;
@@ -136,14 +124,13 @@ struct SliceTile {
   int64_t b;
   int r0, kp;
 };
-__device__ inline SliceTile slice_tile(const SliceBlock* blocks, const int64_t* prefix,
-                                       int64_t nblk, int64_t wg) {
+__device__ inline SliceTile slice_tile(const SliceBlock* blocks, const int2* wgmap, int64_t wg) {
   SliceTile t;
-  t.b = find_slot(prefix, nblk, wg);
-  const int local = (int)(wg - prefix[t.b]);
+  const int2 m = wgmap[wg];
+  t.b = m.x;
   const int kparts = blocks[t.b].kparts;
-  t.kp = local % kparts;
-  t.r0 = (local / kparts) * kSliceRows;
+  t.kp = m.y % kparts;
+  t.r0 = (m.y / kparts) * kSliceRows;
   return t;
 }
 
@@ -161,11 +148,11 @@ __device__ inline void load_row_offsets(const SliceBlock& d, int r0, int tid, in
 // K; threads run along the contiguous direction so that loads coalesce.
 template <int CPLX>
 __global__ void __launch_bounds__(kThreads)
-    slice_max_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
-                     int64_t nblk, SliceOperands ops) {
+    slice_max_kernel(const SliceBlock* __restrict__ blocks, const int2* __restrict__ wgmap,
+                     SliceOperands ops) {
   __shared__ double red[4][kSliceRows];
   __shared__ int64_t srow[kSliceRows];
-  const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
+  const SliceTile t = slice_tile(blocks, wgmap, blockIdx.x);
   const SliceBlock& d = blocks[t.b];
   const int tid = threadIdx.x;
   const double* x = ops.src[d.side] + (CPLX ? 2 : 1) * d.off;
@@ -185,7 +172,18 @@ __global__ void __launch_bounds__(kThreads)
     const int row = tid % kSliceRows, part = tid / kSliceRows;
     double m = 0.0;
     const int ka = k_lo + part * kQ, kb = min(k_hi, ka + kQ);
-    if (t.r0 + row < d.R && ka < kb) {
+    if (t.r0 + row < d.R && ka < kb && d.nkm == 1) {
+      // one K mode (the usual case once adjacent modes are merged): the K
+      // offset of element k is k * kstr for the whole wave, so all kQ loads
+      // go out before the first is used (k past the range re-reads the
+      // range's last element, which leaves the maximum unchanged)
+      const int64_t ro = srow[row], ks = d.kstr[0];
+      double v[kQ];
+#pragma unroll
+      for (int u = 0; u < kQ; ++u) v[u] = elem_mag<CPLX>(x, ro + (int64_t)min(ka + u, kb - 1) * ks);
+#pragma unroll
+      for (int u = 0; u < kQ; ++u) m = fmax(m, v[u]);
+    } else if (t.r0 + row < d.R && ka < kb) {
       const int64_t ro = srow[row];
       KWalk w;
       w.init((uint32_t)ka, d);
@@ -252,11 +250,11 @@ __global__ void __launch_bounds__(kThreads)
 // complex); set s of block b starts at planes + nsets*plane_off + s*S*R*Kpad.
 template <int S, int CPLX>
 __global__ void __launch_bounds__(kThreads)
-    slice_kernel(const SliceBlock* __restrict__ blocks, const int64_t* __restrict__ prefix,
-                 int64_t nblk, SliceOperands ops, int nsets) {
+    slice_kernel(const SliceBlock* __restrict__ blocks, const int2* __restrict__ wgmap,
+                 SliceOperands ops, int nsets) {
   __shared__ int32_t sexp[kSliceRows];
   __shared__ int64_t srow[kSliceRows];
-  const SliceTile t = slice_tile(blocks, prefix, nblk, blockIdx.x);
+  const SliceTile t = slice_tile(blocks, wgmap, blockIdx.x);
   const SliceBlock& d = blocks[t.b];
   const int r0 = t.r0;
   const int tid = threadIdx.x;
@@ -283,68 +281,90 @@ __global__ void __launch_bounds__(kThreads)
   constexpr int kChunks = kSliceKPart / 16;
   const int ch_end = min(d.Kpad / 16, (t.kp + 1) * kChunks);
   const int64_t pstride = (int64_t)d.R * d.Kpad;
-  for (int it = tid; it < kSliceRows * kChunks; it += kThreads) {
-    const int row = it % kSliceRows;
-    const int ch = t.kp * kChunks + it / kSliceRows;
-    const int gr = r0 + row;
-    if (gr >= d.R || ch >= ch_end) continue;
-    const int e = sexp[row];
-    const int64_t ro = srow[row];
-    KWalk kw;
-    kw.init((uint32_t)(ch * 16), d);
-    constexpr int NC = CPLX ? 3 : 1;  // re, im, -im
-    // the 16 offsets first (padding k >= K clamped onto the row's k = 0),
-    // then the 16 loads back to back and unconditional, then the digits
-    int64_t off[16];
+  // A thread has kItems items; for real data the loads of all of them go
+  // out before the first item's digits are formed (complex data: one item
+  // at a time, for registers). Items past the block (rows >= R, chunks past
+  // the K range) load from clamped, valid addresses and store nothing.
+  constexpr int kItems = kSliceRows * kChunks / kThreads;
+  static_assert(kSliceRows * kChunks % kThreads == 0, "items per thread");
+  constexpr int NB = CPLX ? 1 : kItems;
+  constexpr int NC = CPLX ? 3 : 1;  // re, im, -im
+  for (int i0 = 0; i0 < kItems; i0 += NB) {
+    double xr[NB][16], xi[NB][CPLX ? 16 : 1];
+    int eq[NB], grq[NB], chq[NB];
+    bool lq[NB];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      off[j] = ro + (ch * 16 + j < d.K ? kw.off : 0);
-      kw.step(d);
-    }
-    const bool ok = e != kNonFinite;
-    double xr[16], xi[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const bool in = ok && ch * 16 + j < d.K;
-      if (CPLX) {
-        const double2 v = reinterpret_cast<const double2*>(x)[off[j]];
-        xr[j] = in ? v.x : 0.0;
-        xi[j] = in ? v.y : 0.0;
-      } else {
-        const double v = x[off[j]];
-        xr[j] = in ? v : 0.0;
-        xi[j] = 0.0;
-      }
-    }
-    // one slice set at a time (re, then im and -im for complex data: the
-    // digits of -im are those of im negated), so only one set's packed
-    // digits are live: with all three at once the complex kernel needed a
-    // full register file and ran one wave per SIMD
-    int8_t* dst = planes + (int64_t)nsets * d.plane_off + ((int64_t)ch * d.R + gr) * 16;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if (c >= nsets) break;
-      uint32_t packed[S][4];
-#pragma unroll
-      for (int i = 0; i < S; ++i)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) packed[i][w] = 0u;
+    for (int q = 0; q < NB; ++q) {
+      const int it = tid + (i0 + q) * kThreads;
+      const int row = it % kSliceRows;
+      const int ch0 = t.kp * kChunks + it / kSliceRows;
+      const int gr = r0 + row;
+      lq[q] = gr < d.R && ch0 < ch_end;
+      const int ch = min(ch0, ch_end - 1);
+      const int e = sexp[row];
+      eq[q] = e;
+      grq[q] = gr;
+      chq[q] = ch;
+      const int64_t ro = srow[row];
+      KWalk kw;
+      kw.init((uint32_t)(ch * 16), d);
+      // the 16 offsets first (padding k >= K clamped onto the row's k = 0),
+      // then the 16 loads back to back and unconditional
+      int64_t off[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        // x / 2^E is exact (|.| < 1), the scale by 127 rounds once
-        double y = ldexp(c == 0 ? xr[j] : xi[j], -e) * kSliceLead;
+        off[j] = ro + (ch * 16 + j < d.K ? kw.off : 0);
+        kw.step(d);
+      }
+      const bool ok = e != kNonFinite;
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-          const double v = rint(y);
-          y = (y - v) * kSliceRadix;
-          const int32_t dg = c == 2 ? -(int32_t)v : (int32_t)v;
-          packed[i][j >> 2] |= ((uint32_t)(dg & 0xff)) << (8 * (j & 3));
+      for (int j = 0; j < 16; ++j) {
+        const bool in = ok && ch * 16 + j < d.K;
+        if (CPLX) {
+          const double2 v = reinterpret_cast<const double2*>(x)[off[j]];
+          xr[q][j] = in ? v.x : 0.0;
+          xi[q][CPLX ? j : 0] = in ? v.y : 0.0;
+        } else {
+          const double v = x[off[j]];
+          xr[q][j] = in ? v : 0.0;
         }
       }
+    }
 #pragma unroll
-      for (int i = 0; i < S; ++i)
-        *reinterpret_cast<v4i*>(dst + (c * S + i) * pstride) =
-            v4i{(int)packed[i][0], (int)packed[i][1], (int)packed[i][2], (int)packed[i][3]};
+    for (int q = 0; q < NB; ++q) {
+      if (!lq[q]) continue;
+      const int e = eq[q];
+      // one slice set at a time (re, then im and -im for complex data: the
+      // digits of -im are those of im negated), so only one set's packed
+      // digits are live: with all three at once the complex kernel needed a
+      // full register file and ran one wave per SIMD
+      int8_t* dst =
+          planes + (int64_t)nsets * d.plane_off + ((int64_t)chq[q] * d.R + grq[q]) * 16;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        if (c >= nsets) break;
+        uint32_t packed[S][4];
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+#pragma unroll
+          for (int w = 0; w < 4; ++w) packed[i][w] = 0u;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          // x / 2^E is exact (|.| < 1), the scale by 127 rounds once
+          double y = ldexp(c == 0 ? xr[q][j] : xi[q][CPLX ? j : 0], -e) * kSliceLead;
+#pragma unroll
+          for (int i = 0; i < S; ++i) {
+            const double v = rint(y);
+            y = (y - v) * kSliceRadix;
+            const int32_t dg = c == 2 ? -(int32_t)v : (int32_t)v;
+            packed[i][j >> 2] |= ((uint32_t)(dg & 0xff)) << (8 * (j & 3));
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+          *reinterpret_cast<v4i*>(dst + (c * S + i) * pstride) =
+              v4i{(int)packed[i][0], (int)packed[i][1], (int)packed[i][2], (int)packed[i][3]};
+      }
     }
   }
 }
@@ -770,21 +790,20 @@ __global__ void __launch_bounds__(128, 1)
 int split_slices(int prec) { return prec == 2 ? kSliceCount : 0; }
 
 hipError_t launch_slice(hipStream_t s, int prec, int cplx, int nsets, const SliceBlock* blocks,
-                        const int64_t* prefix, int64_t nblk, int64_t grid,
-                        const SliceOperands& ops) {
+                        const int2* wgmap, int64_t grid, const SliceOperands& ops) {
   if (grid == 0) return hipSuccess;
   if (prec != 2) return hipErrorNotSupported;
   const dim3 g((unsigned)grid), blk(kThreads);
   if (cplx) {
-    ITG_LAUNCH((slice_max_kernel<1>), g, blk, 0, s, blocks, prefix, nblk, ops);
+    ITG_LAUNCH((slice_max_kernel<1>), g, blk, 0, s, blocks, wgmap, ops);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    ITG_LAUNCH((slice_kernel<kSliceCount, 1>), g, blk, 0, s, blocks, prefix, nblk, ops, nsets);
+    ITG_LAUNCH((slice_kernel<kSliceCount, 1>), g, blk, 0, s, blocks, wgmap, ops, nsets);
   } else {
-    ITG_LAUNCH((slice_max_kernel<0>), g, blk, 0, s, blocks, prefix, nblk, ops);
+    ITG_LAUNCH((slice_max_kernel<0>), g, blk, 0, s, blocks, wgmap, ops);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    ITG_LAUNCH((slice_kernel<kSliceCount, 0>), g, blk, 0, s, blocks, prefix, nblk, ops, 1);
+    ITG_LAUNCH((slice_kernel<kSliceCount, 0>), g, blk, 0, s, blocks, wgmap, ops, 1);
   }
   return hipGetLastError();
 }
