@@ -71,7 +71,9 @@ class ContractionSequence:
                     order = self._order_for_next(cur.inds, C.inds)
                     step2, cur2 = self._build_step(a_it, B, out_inds=order)
                     probe2 = self._probe(cur2, C)
-                    if probe2 is not None and not probe2["perm1"]:
+                    # kept only if this step still runs the same way
+                    if probe2 is not None and not probe2["perm1"] and \
+                            step2[1].info()["strategy"] == step[1].info()["strategy"]:
                         step, cur = step2, cur2
             self.steps.append(step)
         self.result = cur
