@@ -20,6 +20,8 @@
 // NDTensors/src/blocksparse/contract_generic.jl:78-129.
 #include "itg_internal.h"
 
+#include <type_traits>
+
 namespace itg {
 
 namespace {
@@ -484,60 +486,94 @@ __global__ void __launch_bounds__(kThreads)
     double xnr[XM], xni[CPLX ? XM : 1];
     coords(f + kThreads, cn);
     load(cn, xnr, xni);
-    // y = W1 x (masked, in slot order)
-    double yre[YM], yim[CPLX ? YM : 1];
+    // y = W1 x, z = W2 y, each in slot order. A (slot, output) pair without
+    // a coefficient holds 0 in LDS, so for finite values the unmasked sum
+    // equals the masked one bit for bit: the accumulator starts at +0 and an
+    // exact zero sum rounds to +0, so it is never -0, and adding x * 0 = +-0
+    // leaves it unchanged. Inf or NaN times a missing coefficient would
+    // not, so a wave holding any non-finite x (or y) takes the masked form.
+    bool fin = true;
 #pragma unroll
-    for (int v = 0; v < YM; ++v) {
-      const uint32_t m = y_mask[v];
-      double acc_re = 0.0, acc_im = 0.0;
-#pragma unroll
-      for (int u = 0; u < XM; ++u) {
-        const bool lk = (m >> u) & 1u;
-        const double wr = W1re[v * kChainX + u];
-        if (CPLX) {
-          const double wi = W1im[v * kChainX + u];
-          const double tr = fma(xre[u], wr, fma(-xim[CPLX ? u : 0], wi, acc_re));
-          const double ti = fma(xre[u], wi, fma(xim[CPLX ? u : 0], wr, acc_im));
-          acc_re = lk ? tr : acc_re;
-          acc_im = lk ? ti : acc_im;
-        } else {
-          const double t = fma(xre[u], wr, acc_re);
-          acc_re = lk ? t : acc_re;
-        }
-      }
-      yre[v] = acc_re;
-      if (CPLX) yim[CPLX ? v : 0] = acc_im;
+    for (int u = 0; u < XM; ++u) {
+      fin = fin && isfinite(xre[u]);
+      if (CPLX) fin = fin && isfinite(xim[CPLX ? u : 0]);
     }
-    // z = W2 y (masked, in slot order)
-    const bool live = f < cg.nfib;
-    for (int q = 0; q < nz; ++q) {
-      const uint32_t m = z_mask[q];
-      double acc_re = 0.0, acc_im = 0.0;
+    const bool fast_y = __all(fin);
+    double yre[YM], yim[CPLX ? YM : 1];
+    // MASKED = false: every term is added (the compile-time mask drops the
+    // selects); true: only the pairs of the mask, as fiber_mv_kernel
+    auto form_y = [&](auto masked) {
+      constexpr bool MASKED = decltype(masked)::value;
 #pragma unroll
       for (int v = 0; v < YM; ++v) {
-        const bool lk = (m >> v) & 1u;
-        const double wr = W2re[q * kChainY + v];
-        if (CPLX) {
-          const double wi = W2im[q * kChainY + v];
-          const double tr = fma(yre[v], wr, fma(-yim[CPLX ? v : 0], wi, acc_re));
-          const double ti = fma(yre[v], wi, fma(yim[CPLX ? v : 0], wr, acc_im));
-          acc_re = lk ? tr : acc_re;
-          acc_im = lk ? ti : acc_im;
-        } else {
-          const double t = fma(yre[v], wr, acc_re);
-          acc_re = lk ? t : acc_re;
+        const uint32_t m = MASKED ? y_mask[v] : ~0u;
+        double acc_re = 0.0, acc_im = 0.0;
+#pragma unroll
+        for (int u = 0; u < XM; ++u) {
+          const bool lk = !MASKED || ((m >> u) & 1u);
+          const double wr = W1re[v * kChainX + u];
+          if (CPLX) {
+            const double wi = W1im[v * kChainX + u];
+            const double tr = fma(xre[u], wr, fma(-xim[CPLX ? u : 0], wi, acc_re));
+            const double ti = fma(xre[u], wi, fma(xim[CPLX ? u : 0], wr, acc_im));
+            acc_re = lk ? tr : acc_re;
+            acc_im = lk ? ti : acc_im;
+          } else {
+            const double t = fma(xre[u], wr, acc_re);
+            acc_re = lk ? t : acc_re;
+          }
+        }
+        yre[v] = acc_re;
+        if (CPLX) yim[CPLX ? v : 0] = acc_im;
+      }
+    };
+    if (fast_y)
+      form_y(std::false_type{});
+    else
+      form_y(std::true_type{});
+    bool yfin = true;
+#pragma unroll
+    for (int v = 0; v < YM; ++v) {
+      yfin = yfin && isfinite(yre[v]);
+      if (CPLX) yfin = yfin && isfinite(yim[CPLX ? v : 0]);
+    }
+    const bool fast_z = fast_y && __all(yfin);
+    const bool live = f < cg.nfib;
+    auto form_z = [&](auto masked) {
+      constexpr bool MASKED = decltype(masked)::value;
+      for (int q = 0; q < nz; ++q) {
+        const uint32_t m = MASKED ? z_mask[q] : ~0u;
+        double acc_re = 0.0, acc_im = 0.0;
+#pragma unroll
+        for (int v = 0; v < YM; ++v) {
+          const bool lk = !MASKED || ((m >> v) & 1u);
+          const double wr = W2re[q * kChainY + v];
+          if (CPLX) {
+            const double wi = W2im[q * kChainY + v];
+            const double tr = fma(yre[v], wr, fma(-yim[CPLX ? v : 0], wi, acc_re));
+            const double ti = fma(yre[v], wi, fma(yim[CPLX ? v : 0], wr, acc_im));
+            acc_re = lk ? tr : acc_re;
+            acc_im = lk ? ti : acc_im;
+          } else {
+            const double t = fma(yre[v], wr, acc_re);
+            acc_re = lk ? t : acc_re;
+          }
+        }
+        if (live) {
+          int64_t co = z_off[q];
+#pragma unroll
+          for (int i = 0; i < NF; ++i) co += (int64_t)c[i] * z_str[q][i];
+          if (CPLX)
+            reinterpret_cast<double2*>(Z)[co] = make_double2(acc_re, acc_im);
+          else
+            Z[co] = acc_re;
         }
       }
-      if (live) {
-        int64_t co = z_off[q];
-#pragma unroll
-        for (int i = 0; i < NF; ++i) co += (int64_t)c[i] * z_str[q][i];
-        if (CPLX)
-          reinterpret_cast<double2*>(Z)[co] = make_double2(acc_re, acc_im);
-        else
-          Z[co] = acc_re;
-      }
-    }
+    };
+    if (fast_z)
+      form_z(std::false_type{});
+    else
+      form_z(std::true_type{});
 #pragma unroll
     for (int i = 0; i < NF; ++i) c[i] = cn[i];
 #pragma unroll

@@ -86,6 +86,43 @@ def test_fused_mpo_steps_bit_identical_and_match_oracle(env, chi, cplx):
         assert relerr(fused.result.tensor.data.cpu().numpy(), ref.data) < FP64_RTOL
 
 
+@pytest.mark.parametrize("cplx", [False, True], ids=["f64", "c128"])
+def test_fused_pair_non_finite_values(env, cplx):
+    """A wave whose x (or y) values are not all finite forms its sums over
+    the coefficient masks only (Inf or NaN times a missing coefficient must
+    not reach an output); every other wave adds all terms, which is the same
+    sum for finite values. Both must equal the separate steps bit for bit,
+    NaN positions included."""
+    torch, itg, ND, O = env
+    from itg.configs import heff_network
+    from itg.network import ContractionSequence
+    net = heff_network(128, qns=True, seed=7, dtype=np.complex128 if cplx else np.float64)[:5]
+    plain = ContractionSequence(net, fuse=False)
+    plain.run("fp64")
+    _k, p1, X, B1, Y, _l = plain.steps[1]
+    _k, p2, _Y, B2, Z, _l = plain.steps[2]
+    xs = X.data.clone()
+    flat = X.data.view(torch.float64) if not cplx else torch.view_as_real(X.data).reshape(-1)
+    n = flat.numel()
+    flat[n // 7] = float("inf")
+    flat[n // 3] = float("-inf")
+    flat[n // 2] = float("nan")
+    p1.execute(Y.data, X.data, B1.data)
+    p2.execute(Z.data, Y.data, B2.data)
+    zsep = Z.data.clone()
+    Z.data.fill_(0.0)
+    ND.BSChain(p1, p2).execute(Z.data, X.data, B1.data, B2.data)
+    torch.cuda.synchronize()
+    a = torch.view_as_real(Z.data) if cplx else Z.data
+    b = torch.view_as_real(zsep) if cplx else zsep
+    assert torch.equal(torch.isnan(a), torch.isnan(b))
+    assert int(torch.isnan(b).sum()) > 0  # the NaN reached some outputs
+    fin = ~torch.isnan(b)
+    assert torch.equal(a[fin], b[fin])
+    assert int(torch.isinf(b).sum()) + int(torch.isnan(b).sum()) < b.numel() // 4  # not everywhere
+    X.data.copy_(xs)
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 def test_fused_pair_with_wide_link_sectors(env, cplx):
     """A 3-wide MPO link sector gives more than eight x / y values per
