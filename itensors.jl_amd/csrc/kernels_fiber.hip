@@ -270,14 +270,27 @@ __global__ void __launch_bounds__(kThreads)
     coords(f + kThreads, cn);
     load(cn, anr, ani);
     const bool live = f < fg.nfib;
+    // An unlinked (output, slot) pair holds 0 in LDS, so for finite slot
+    // values the sum over all slots equals the sum over the linked ones bit
+    // for bit (the accumulator starts at +0 and is never -0, and x * 0 =
+    // +-0 leaves it unchanged); a wave holding any non-finite value masks,
+    // so that an Inf or NaN reaches only the outputs it is linked to.
+    bool fin = true;
+#pragma unroll
+    for (int sl = 0; sl < SM; ++sl) {
+      fin = fin && isfinite(are[sl]);
+      if (CPLX) fin = fin && isfinite(aim[CPLX ? sl : 0]);
+    }
+    auto outputs = [&](auto masked) {
+    constexpr bool MASKED = decltype(masked)::value;
     for (int o = 0; o < nout; ++o) {
-      const uint32_t m = o_mask[o];
+      const uint32_t m = MASKED ? o_mask[o] : ~0u;
       double acc_re = 0.0, acc_im = 0.0;
       // branch-free over all slots: an unlinked slot's product is formed
       // and discarded by a select (never added), so its value cannot leak
 #pragma unroll
       for (int sl = 0; sl < SM; ++sl) {
-        const bool lk = (m >> sl) & 1u;
+        const bool lk = !MASKED || ((m >> sl) & 1u);
         const double wr = Wre[o * kFiberSlots + sl];
         if (CPLX) {
           const double wi = Wim[o * kFiberSlots + sl];
@@ -309,6 +322,11 @@ __global__ void __launch_bounds__(kThreads)
         }
       }
     }
+    };
+    if (__all(fin))
+      outputs(std::false_type{});
+    else
+      outputs(std::true_type{});
 #pragma unroll
     for (int i = 0; i < NF; ++i) c[i] = cn[i];
 #pragma unroll
