@@ -282,46 +282,46 @@ __global__ void __launch_bounds__(kThreads)
       if (CPLX) fin = fin && isfinite(aim[CPLX ? sl : 0]);
     }
     auto outputs = [&](auto masked) {
-    constexpr bool MASKED = decltype(masked)::value;
-    for (int o = 0; o < nout; ++o) {
-      const uint32_t m = MASKED ? o_mask[o] : ~0u;
-      double acc_re = 0.0, acc_im = 0.0;
-      // branch-free over all slots: an unlinked slot's product is formed
-      // and discarded by a select (never added), so its value cannot leak
+      constexpr bool MASKED = decltype(masked)::value;
+      for (int o = 0; o < nout; ++o) {
+        const uint32_t m = MASKED ? o_mask[o] : ~0u;
+        double acc_re = 0.0, acc_im = 0.0;
+        // branch-free over all slots: an unlinked slot's product is formed
+        // and discarded by a select (never added), so its value cannot leak
 #pragma unroll
-      for (int sl = 0; sl < SM; ++sl) {
-        const bool lk = !MASKED || ((m >> sl) & 1u);
-        const double wr = Wre[o * kFiberSlots + sl];
-        if (CPLX) {
-          const double wi = Wim[o * kFiberSlots + sl];
-          const double tr = fma(are[sl], wr, fma(-aim[CPLX ? sl : 0], wi, acc_re));
-          const double ti = fma(are[sl], wi, fma(aim[CPLX ? sl : 0], wr, acc_im));
-          acc_re = lk ? tr : acc_re;
-          acc_im = lk ? ti : acc_im;
-        } else {
-          const double t = fma(are[sl], wr, acc_re);
-          acc_re = lk ? t : acc_re;
-        }
-      }
-      if (live) {
-        int64_t co = o_off[o];
-#pragma unroll
-        for (int i = 0; i < NF; ++i) co += (int64_t)c[i] * o_str[o][i];
-        if (CPLX) {
-          double2 out = make_double2(acc_re, acc_im);
-          if (use_beta) {
-            const double2 cv = reinterpret_cast<double2*>(C)[co];
-            out.x += beta_re * cv.x - beta_im * cv.y;
-            out.y += beta_re * cv.y + beta_im * cv.x;
+        for (int sl = 0; sl < SM; ++sl) {
+          const bool lk = !MASKED || ((m >> sl) & 1u);
+          const double wr = Wre[o * kFiberSlots + sl];
+          if (CPLX) {
+            const double wi = Wim[o * kFiberSlots + sl];
+            const double tr = fma(are[sl], wr, fma(-aim[CPLX ? sl : 0], wi, acc_re));
+            const double ti = fma(are[sl], wi, fma(aim[CPLX ? sl : 0], wr, acc_im));
+            acc_re = lk ? tr : acc_re;
+            acc_im = lk ? ti : acc_im;
+          } else {
+            const double t = fma(are[sl], wr, acc_re);
+            acc_re = lk ? t : acc_re;
           }
-          reinterpret_cast<double2*>(C)[co] = out;
-        } else {
-          double out = acc_re;
-          if (use_beta) out = fma(beta_re, C[co], out);
-          C[co] = out;
+        }
+        if (live) {
+          int64_t co = o_off[o];
+#pragma unroll
+          for (int i = 0; i < NF; ++i) co += (int64_t)c[i] * o_str[o][i];
+          if (CPLX) {
+            double2 out = make_double2(acc_re, acc_im);
+            if (use_beta) {
+              const double2 cv = reinterpret_cast<double2*>(C)[co];
+              out.x += beta_re * cv.x - beta_im * cv.y;
+              out.y += beta_re * cv.y + beta_im * cv.x;
+            }
+            reinterpret_cast<double2*>(C)[co] = out;
+          } else {
+            double out = acc_re;
+            if (use_beta) out = fma(beta_re, C[co], out);
+            C[co] = out;
+          }
         }
       }
-    }
     };
     if (__all(fin))
       outputs(std::false_type{});
