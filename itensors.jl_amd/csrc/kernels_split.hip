@@ -180,7 +180,8 @@ __global__ void __launch_bounds__(kThreads)
       const int64_t ro = srow[row], ks = d.kstr[0];
       double v[kQ];
 #pragma unroll
-      for (int u = 0; u < kQ; ++u) v[u] = elem_mag<CPLX>(x, ro + (int64_t)min(ka + u, kb - 1) * ks);
+      for (int u = 0; u < kQ; ++u)
+        v[u] = elem_mag<CPLX>(x, ro + (int64_t)min(ka + u, kb - 1) * ks);
 #pragma unroll
       for (int u = 0; u < kQ; ++u) m = fmax(m, v[u]);
     } else if (t.r0 + row < d.R && ka < kb) {
