@@ -389,8 +389,12 @@ hipError_t launch_mv_sm(hipStream_t s, int nf, int smax, dim3 grid, const FiberM
 // stored; Y never reaches memory. W1 and W2 of the super-group sit in LDS.
 // The arithmetic per y (per z) value is fiber_mv_kernel's, operation for
 // operation, with unit scale, zero beta and no conjugation.
+// The kernel is bound by memory latency. Real data with up to 8 x and y slots
+// runs five waves per SIMD (96 VGPRs, a few spilled): 5 % faster than the
+// four that its 106 VGPRs allow. Loading two rounds ahead instead (four
+// waves, 126 VGPRs) is slower; the other forms spill too much at five.
 template <int CPLX, int NF, int XM, int YM>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, (!CPLX && XM <= 8 && YM <= 8) ? 5 : 1)
     fiber_chain_kernel(const ChainGroup* __restrict__ groups, const FiberSlot* __restrict__ slots,
                        const FiberOutElem* __restrict__ zouts,
                        const uint32_t* __restrict__ ymask, const FiberCoef* __restrict__ coefs1,
