@@ -746,6 +746,7 @@ def main():
         roof = {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                 "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic_bytes,
                 "fp64_equivalent_frac": achieved / FP64_PEAK_TFLOPS,
+                "bf16_peak_frac": achieved / BF16_PEAK_TFLOPS,
                 "slice_products_per_flop": nprod,
                 "slice_product_frac": achieved * nprod / INT8_PEAK_TOPS,
                 "kernel": f"split_gemm_kernel, steps 1 and 4 of the apply ({launches:g} launches, "
