@@ -132,6 +132,29 @@ itg_status itg_set_split_tile(itg_handle* h, int tile);
  * Results agree to rounding across the three. ITG_ERR_INVALID otherwise. */
 itg_status itg_set_gemm_backend(itg_handle* h, int backend);
 
+/* Block-pair order of the block-sparse plans created on this handle from now
+ * on: the counterpart of NDTensors' threaded block-sparse switch, which picks
+ * the algorithm of contract_blockoffsets (NDTensors/src/blocksparse/
+ * contract.jl:42-55).
+ *   ITG_BS_ORDER_SEQUENTIAL -- T1's blocks outer, T2's inner
+ *                              (contract_sequential.jl:26-39);
+ *   ITG_BS_ORDER_THREADED   -- contract_blocks(::Algorithm"threaded_threads")
+ *                              (contract_threaded.jl:2-73): the same when T1
+ *                              has more blocks than T2, else T2's blocks
+ *                              outer and T1's inner.
+ * The order fixes the output's BlockOffsets (blocks placed in order of first
+ * appearance, contract_generic.jl:23-30) and the order of the sums into each
+ * output block. The reference takes the threaded algorithm when
+ * using_threaded_blocksparse() && Threads.nthreads() > 1; a Julia caller
+ * passes that condition. DiagBlockSparse plans have one order (the
+ * reference's diagonal contraction has no threaded form). ITG_ERR_INVALID
+ * for another value. */
+typedef enum {
+  ITG_BS_ORDER_SEQUENTIAL = 0,
+  ITG_BS_ORDER_THREADED = 1
+} itg_bs_order;
+itg_status itg_set_blocksparse_order(itg_handle* h, int order);
+
 /* Per-kernel timing: the device counterpart of NDTensors' global
  * `timer::TimerOutput` (NDTensors/src/NDTensors.jl:96-99) and its
  * `@timeit_debug` sections around BLAS.gemm! and the permutations

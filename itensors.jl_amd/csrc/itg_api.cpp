@@ -43,6 +43,7 @@ struct itg_handle {
   std::map<std::string, int> tuned;
   int split_tile = 0;  // itg_set_split_tile (0 = per plan)
   int backend = kBackendAuto;  // itg_set_gemm_backend: plans created from now on
+  int bs_order = ITG_BS_ORDER_SEQUENTIAL;  // itg_set_blocksparse_order: likewise
   // itg_graph_begin / _end: the stream being captured, and the caller's
   // stream to restore (a handle on the null stream captures on its own)
   hipStream_t cap_stream = nullptr;  // created on first use, destroyed with the handle
@@ -62,6 +63,7 @@ struct itg_bs_plan {
   BlockList b1, b2, bR;
   std::vector<int64_t> blocksR;  // NR x nblkR, 1-based
   std::vector<PairSpec> pairs;
+  bool t2_outer = false;  // enumerated with T2's blocks outer (threaded order)
   std::unique_ptr<ExecPlan> ep;  // pair_alpha = NULL
   int backend = kBackendAuto;    // the handle's backend when the plan was created
   // one plan per set of per-pair factors used so far (the factors are baked
@@ -247,6 +249,13 @@ itg_status check_reuse(itg_handle* h, const ExecPlan& ep, int cplx, int prec, co
   return ITG_OK;
 }
 
+// Enumeration order of a plan's pairs (they are stored grouped by output
+// block): the order of the outer and inner loops over the block lists.
+bool enum_before(const itg_bs_plan* p, const PairSpec& x, const PairSpec& y) {
+  if (p->t2_outer) return x.blk2 != y.blk2 ? x.blk2 < y.blk2 : x.blk1 < y.blk1;
+  return x.blk1 != y.blk1 ? x.blk1 < y.blk1 : x.blk2 < y.blk2;
+}
+
 // The execution plan of block-sparse plan p for the per-pair factors
 // pair_alpha (NULL: the plan's own). The factors are baked into the device
 // tables, so each new set builds (and keeps) a plan of its own; with build =
@@ -259,8 +268,7 @@ itg_status plan_for_factors(itg_handle* h, itg_bs_plan* p, const double* pair_al
   std::vector<size_t> idx(p->pairs.size());
   for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
   std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-    const PairSpec &x = p->pairs[a], &y = p->pairs[b];
-    return x.blk1 != y.blk1 ? x.blk1 < y.blk1 : x.blk2 < y.blk2;
+    return enum_before(p, p->pairs[a], p->pairs[b]);
   });
   std::vector<double> pa(2 * p->pairs.size());
   for (size_t e = 0; e < idx.size(); ++e) {
@@ -493,6 +501,13 @@ itg_status itg_set_split_tile(itg_handle* h, int tile) {
 itg_status itg_set_gemm_backend(itg_handle* h, int backend) {
   if (!h || backend < kBackendAuto || backend > kBackendGeneric) return ITG_ERR_INVALID;
   h->backend = backend;
+  return ITG_OK;
+}
+
+itg_status itg_set_blocksparse_order(itg_handle* h, int order) {
+  if (!h || (order != ITG_BS_ORDER_SEQUENTIAL && order != ITG_BS_ORDER_THREADED))
+    return ITG_ERR_INVALID;
+  h->bs_order = order;
   return ITG_OK;
 }
 
@@ -772,18 +787,21 @@ itg_status itg_contract_dense(itg_handle* h, itg_dtype dtype, int NR, const int6
 }  // extern "C"
 
 namespace {
-// Block lists, output structure and the sequential pair list of T1 x T2
+// Block lists, output structure and the pair list of T1 x T2
 // (contract_blockoffsets, NDTensors/src/blocksparse/contract_sequential.jl:1-41);
 // shared by the plain and the diagonal plans. `diag2`: T2 is block diagonal
 // (every block (k, ..., k)) and each block stores only its diagonal, min of
-// its dims elements.
+// its dims elements. `t2_outer`: enumerate T2's blocks in the outer loop and
+// T1's in the inner one (the threaded order when T1 has no more blocks than
+// T2, contract_threaded.jl:45-72); otherwise T1's outer (contract_sequential.jl:26-39,
+// and contract_threaded.jl:14-44 for T1 with more blocks).
 itg_status build_bs_structure(itg_handle* h, int N1, int64_t nblk1, const int64_t* blocks1,
                               const int64_t* offs1, const int64_t* nsec1,
                               const int64_t* secdims1, const int64_t* labels1, int N2,
                               int64_t nblk2, const int64_t* blocks2, const int64_t* offs2,
                               const int64_t* nsec2, const int64_t* secdims2,
                               const int64_t* labels2, int NR, const int64_t* labelsR,
-                              bool diag2, std::unique_ptr<itg_bs_plan>& p) {
+                              bool diag2, bool t2_outer, std::unique_ptr<itg_bs_plan>& p) {
   if (nblk1 < 0 || nblk2 < 0) return fail(h, ITG_ERR_INVALID, "negative block count");
   if ((nblk1 > 0 && (!blocks1 || !offs1)) || (nblk2 > 0 && (!blocks2 || !offs2)))
     return fail(h, ITG_ERR_INVALID, "null block arrays");
@@ -844,25 +862,32 @@ itg_status build_bs_structure(itg_handle* h, int N1, int64_t nblk1, const int64_
   if (err.empty()) err = fill(p->b2, N2, nblk2, blocks2, offs2, sec2, diag2);
   if (!err.empty()) return fail(h, ITG_ERR_INVALID, err);
 
-  // sequential pair enumeration (contract_sequential.jl:26-39), with T2
-  // blocks bucketed by their contracted coordinates so the scan is
-  // O(nblk1 + nblk2 + npairs) while keeping the (block1 outer, block2 inner)
-  // order.
+  // pair enumeration: the inner operand's blocks are bucketed by their
+  // contracted coordinates (in block order), so the scan is
+  // O(nblk1 + nblk2 + npairs) while keeping the (outer, inner) order of the
+  // nested loops over the two block lists
   std::unordered_map<std::string, std::vector<int64_t>> by_key;
   auto kkey = [&](const int64_t* blk, const int* pos) {
     std::string k;
     for (int i = 0; i < st.nk; ++i) k.append((const char*)&blk[pos[i]], sizeof(int64_t));
     return k;
   };
-  for (int64_t b = 0; b < nblk2; ++b) by_key[kkey(blocks2 + b * N2, st.k2)].push_back(b);
+  if (t2_outer) {
+    for (int64_t b = 0; b < nblk1; ++b) by_key[kkey(blocks1 + b * N1, st.k1)].push_back(b);
+  } else {
+    for (int64_t b = 0; b < nblk2; ++b) by_key[kkey(blocks2 + b * N2, st.k2)].push_back(b);
+  }
   std::map<std::vector<int64_t>, int64_t> rindex;  // block coords -> R block index
   BlockList& bR = p->bR;
   bR.N = NR;
   int64_t nnzR = 0;
-  for (int64_t b1 = 0; b1 < nblk1; ++b1) {
-    auto f = by_key.find(kkey(blocks1 + b1 * N1, st.k1));
+  const int64_t nouter = t2_outer ? nblk2 : nblk1;
+  for (int64_t bo = 0; bo < nouter; ++bo) {
+    auto f = t2_outer ? by_key.find(kkey(blocks2 + bo * N2, st.k2))
+                      : by_key.find(kkey(blocks1 + bo * N1, st.k1));
     if (f == by_key.end()) continue;
-    for (int64_t b2 : f->second) {
+    for (int64_t bi : f->second) {
+      const int64_t b1 = t2_outer ? bi : bo, b2 = t2_outer ? bo : bi;
       std::vector<int64_t> cR(NR);
       for (int r = 0; r < NR; ++r)
         cR[r] = st.r_src[r] == 1 ? blocks1[b1 * N1 + st.r_pos[r]] : blocks2[b2 * N2 + st.r_pos[r]];
@@ -911,10 +936,14 @@ itg_status itg_bs_plan_create(itg_handle* h, int N1, int64_t nblk1, const int64_
   if (!h || !out) return ITG_ERR_INVALID;
   *out = nullptr;
   std::unique_ptr<itg_bs_plan> p;
+  // the threaded algorithm takes T2's blocks outer unless T1 has more
+  // blocks (contract_threaded.jl:14, 45)
+  const bool t2_outer = h->bs_order == ITG_BS_ORDER_THREADED && !(nblk1 > nblk2);
   itg_status st = build_bs_structure(h, N1, nblk1, blocks1, offs1, nsec1, secdims1, labels1, N2,
                                      nblk2, blocks2, offs2, nsec2, secdims2, labels2, NR,
-                                     labelsR, false, p);
+                                     labelsR, false, t2_outer, p);
   if (st != ITG_OK) return st;
+  p->t2_outer = t2_outer;
   if (!p->pairs.empty()) {
     DeviceGuard g(h->device);
     p->backend = h->backend;
@@ -940,7 +969,7 @@ itg_status itg_bs_diag_plan_create(itg_handle* h, int N1, int64_t nblk1,
   std::unique_ptr<itg_bs_plan> p;
   itg_status st = build_bs_structure(h, N1, nblk1, blocks1, offs1, nsec1, secdims1, labels1, ND,
                                      nblkD, blocksD, offsD, nsecD, secdimsD, labelsD, NR,
-                                     labelsR, true, p);
+                                     labelsR, true, false, p);
   if (st != ITG_OK) return st;
   p->diag = true;
   p->d_nr = std::max(NR, 1);
@@ -1089,10 +1118,10 @@ itg_status itg_bs_plan_blocks(const itg_bs_plan* p, int64_t* blocksR, int64_t* o
 
 itg_status itg_bs_plan_pairs(const itg_bs_plan* p, int64_t* pairs) {
   if (!p) return ITG_ERR_INVALID;
-  // report in enumeration order (block1 outer, block2 inner)
+  // report in enumeration order
   std::vector<PairSpec> v(p->pairs);
-  std::stable_sort(v.begin(), v.end(), [](const PairSpec& a, const PairSpec& b) {
-    return a.blk1 != b.blk1 ? a.blk1 < b.blk1 : a.blk2 < b.blk2;
+  std::stable_sort(v.begin(), v.end(), [p](const PairSpec& a, const PairSpec& b) {
+    return enum_before(p, a, b);
   });
   for (size_t i = 0; i < v.size(); ++i) {
     pairs[3 * i] = v[i].blk1;

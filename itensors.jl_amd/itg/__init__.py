@@ -21,6 +21,8 @@ from .itensors import (  # noqa: F401
     using_contraction_sequence_optimization,
 )
 from .ndtensors import precision, set_precision  # noqa: F401
+from .ndtensors import (  # noqa: F401
+    disable_threaded_blocksparse, enable_threaded_blocksparse, using_threaded_blocksparse)
 from .itensors import tagstring  # noqa: F401
 from . import hdf5  # noqa: F401
 

@@ -35,6 +35,8 @@ ITG_FLAG_KEEP_SLICES_1 = 4
 ITG_FLAG_KEEP_SLICES_2 = 8
 ITG_FLAG_REUSE_SLICES_1 = 16
 ITG_FLAG_REUSE_SLICES_2 = 32
+ITG_BS_ORDER_SEQUENTIAL = 0
+ITG_BS_ORDER_THREADED = 1
 
 # every symbol include/itg.h declares
 EXPORTS = (
@@ -49,6 +51,7 @@ EXPORTS = (
     "itg_bs_diag_plan_create", "itg_bs_diag_execute",
     "itg_graph_begin", "itg_graph_end", "itg_graph_launch", "itg_graph_destroy",
     "itg_bs_permute_ex", "itg_convert", "itg_set_gemm_backend", "itg_bs_plan_info",
+    "itg_set_blocksparse_order",
 )
 
 
@@ -135,6 +138,7 @@ def lib():
         L.itg_timer_enable.argtypes = [P, I]
         L.itg_set_split_tile.argtypes = [P, I]
         L.itg_set_gemm_backend.argtypes = [P, I]
+        L.itg_set_blocksparse_order.argtypes = [P, I]
         L.itg_timer_count.argtypes = [P]
         L.itg_timer_read.argtypes = [P, I64, ctypes.POINTER(ctypes.c_char_p), I64P, DP]
         for name in EXPORTS:
@@ -205,6 +209,9 @@ _handles = threading.local()
 
 # itg_set_gemm_backend code applied to every handle (ndtensors.backend_*)
 backend = [0]
+# itg_set_blocksparse_order code applied to every handle
+# (ndtensors.enable_threaded_blocksparse)
+bs_order = [ITG_BS_ORDER_SEQUENTIAL]
 
 
 def handle_for(device_index):
@@ -216,6 +223,8 @@ def handle_for(device_index):
         h = d[device_index] = Handle(device_index)
         if backend[0]:
             h.check(lib().itg_set_gemm_backend(h.h, backend[0]))
+        if bs_order[0]:
+            h.check(lib().itg_set_blocksparse_order(h.h, bs_order[0]))
     return h
 
 

@@ -59,10 +59,14 @@ def precision() -> str:
 # of contractions planned afterwards (itg_set_gemm_backend on every device
 # handle): "Auto" by shape, "BLAS" the grouped MFMA GEMM whenever a mode is
 # contracted, "Generic" the generic strided kernel. The threaded-block-sparse
-# and TBLIS switches are accepted and change nothing: on the device every
-# block-sparse plan runs all its output blocks in one grouped launch (the
-# parallelism the threaded switch turns on in the reference), and there is
-# no TBLIS.
+# switch selects the block-pair order of block-sparse plans made afterwards
+# (itg_set_blocksparse_order): the order of contract_blocks(
+# ::Algorithm"threaded_threads") (NDTensors/src/blocksparse/
+# contract_threaded.jl:2-73), which the reference uses when the switch is on
+# and Julia runs more than one thread; it fixes the output's block order and
+# the order of the sums into each output block. On the device every plan
+# runs all its output blocks in one grouped launch either way. The TBLIS
+# switch is accepted and changes nothing: there is no TBLIS.
 _flags = {"threaded_blocksparse": False, "tblis": False, "gemm_backend": "Auto"}
 _BACKEND_CODE = {"Auto": 0, "BLAS": 1, "Generic": 2}
 
@@ -72,17 +76,31 @@ def using_threaded_blocksparse() -> bool:
     return _flags["threaded_blocksparse"]
 
 
-def enable_threaded_blocksparse() -> bool:
-    """Returns the previous state (NDTensors.jl:109-130)."""
+def _set_threaded(on):
     old = _flags["threaded_blocksparse"]
-    _flags["threaded_blocksparse"] = True
+    _flags["threaded_blocksparse"] = on
+    code = _lib.ITG_BS_ORDER_THREADED if on else _lib.ITG_BS_ORDER_SEQUENTIAL
+    _lib.bs_order[0] = code
+    for h in _lib.open_handles():
+        h.check(_lib.lib().itg_set_blocksparse_order(h.h, code))
     return old
+
+
+def enable_threaded_blocksparse(enable=None) -> bool:
+    """Block-sparse contractions planned from now on enumerate their block
+    pairs in the threaded order (T2's blocks outer when T1 has no more blocks
+    than T2; NDTensors/src/blocksparse/contract_threaded.jl:2-73), as the
+    reference does with the switch on in a Julia session of more than one
+    thread. enable=False disables (ITensors' enable_threaded_blocksparse(::Bool),
+    src/global_variables.jl:156-162). Returns the previous state
+    (NDTensors.jl:140-161)."""
+    return _set_threaded(enable is None or bool(enable))
 
 
 def disable_threaded_blocksparse() -> bool:
-    old = _flags["threaded_blocksparse"]
-    _flags["threaded_blocksparse"] = False
-    return old
+    """Back to the sequential pair order (contract_sequential.jl:26-39).
+    Returns the previous state (NDTensors.jl:163-169)."""
+    return _set_threaded(False)
 
 
 def using_tblis() -> bool:
@@ -732,7 +750,7 @@ def bs_plan(T1, labels1, T2, labels2, labelsR, diag=False):
     structure repeats every time a network is applied). diag: T2 is a
     DiagBlockSparseTensor."""
     key = (_structure_key(T1), tuple(labels1), _structure_key(T2), tuple(labels2), tuple(labelsR),
-           T1.data.device.index, diag, _flags["gemm_backend"])
+           T1.data.device.index, diag, _flags["gemm_backend"], _flags["threaded_blocksparse"])
     p = _plan_cache.get(key)
     if p is not None:
         _plan_cache.move_to_end(key)
@@ -746,8 +764,10 @@ def bs_plan(T1, labels1, T2, labels2, labelsR, diag=False):
 
 def contract_blockoffsets(T1, labels1, T2, labels2, labelsR):
     """(blockoffsetsR, nnzR, plan) with plan = [(block1, block2, blockR)] as
-    1-based block tuples in the reference's order
-    (blocksparse/contract_sequential.jl:1-41)."""
+    1-based block tuples in the reference's order: sequential
+    (blocksparse/contract_sequential.jl:1-41), or threaded
+    (contract_threaded.jl:2-73, contract_generic.jl:3-32) while
+    enable_threaded_blocksparse() is in effect."""
     p = bs_plan(T1, labels1, T2, labels2, labelsR)
     offs = OrderedDict(zip(p.blocksR, p.offsR))
     b1 = list(T1.blockoffsets)

@@ -70,9 +70,18 @@ using_mi355x_split_precision(on::Bool = true) = (PRECISION[] = on ? Cint(2) : Ci
 # tensoralgebra/contract.jl:15-24) selects the device kernel family of the plans
 # made from here on: by shape, the grouped MFMA GEMM, or the strided kernel.
 const BACKEND_CODE = Dict(:Auto => Cint(0), :BLAS => Cint(1), :Generic => Cint(2))
+# The threaded block-sparse switch selects the block-pair order of the plans
+# (contract_blockoffsets, NDTensors/src/blocksparse/contract.jl:42-55): the
+# reference takes its threaded algorithm when the switch is on and Julia runs
+# more than one thread, which fixes the output's block order.
+const ITG_BS_ORDER_SEQUENTIAL = Cint(0)
+const ITG_BS_ORDER_THREADED = Cint(1)
 function sync_backend()
     code = get(BACKEND_CODE, NDTensors.gemm_backend[], Cint(0))
     check(ccall((:itg_set_gemm_backend, libitg), Cint, (Ptr{Cvoid}, Cint), handle(), code))
+    order = NDTensors.using_threaded_blocksparse() && Threads.nthreads() > 1 ?
+            ITG_BS_ORDER_THREADED : ITG_BS_ORDER_SEQUENTIAL
+    check(ccall((:itg_set_blocksparse_order, libitg), Cint, (Ptr{Cvoid}, Cint), handle(), order))
     return nothing
 end
 

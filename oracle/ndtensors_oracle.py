@@ -20,6 +20,15 @@ Pinning status (what checks each function; see tests/test_oracle.py):
                                         invariant flux(C)=flux(A)+flux(B), the
                                         zero-block case (test_threading.jl:60-78),
                                         densify-and-compare
+  contract_blockoffsets_threaded     -- the same pairs as the sequential plan;
+                                        equal to it when T1 has more blocks;
+                                        otherwise the sequential plan of the
+                                        swapped product T2*T1 with each pair
+                                        swapped back; offsets by first
+                                        appearance; contraction values equal
+                                        the sequential ones and the
+                                        reference's threading KATs
+                                        (test_threading.jl:30-78)
   dense_contract                     -- textbook reductions: matrix product vs
                                         explicit loops, outer product, scalar
                                         results, alpha/beta + NaN overwrite
@@ -405,18 +414,66 @@ def contract_blockoffsets_sequential(T1, l1, T2, l2, lR):
     return offsR, nnzR, plan, indsR
 
 
+def contract_blockoffsets_threaded(T1, l1, T2, l2, lR):
+    """contract_blockoffsets with Algorithm"threaded_threads"
+    (NDTensors/src/blocksparse/contract_generic.jl:3-32 over contract_blocks,
+    contract_threaded.jl:2-73), the algorithm the reference picks when
+    using_threaded_blocksparse() and Julia runs more than one thread
+    (contract.jl:42-55).
+
+    contract_blocks partitions T1's blocks over the tasks when T1 has more
+    blocks than T2 (T1 outer, T2 inner, as the sequential loop), else T2's
+    blocks (T2 outer, T1 inner); each task keeps its loop order and the task
+    results are concatenated in partition order, so the plan is the plain
+    nested loop in that outer/inner order. Offsets are then assigned by
+    first appearance over the plan. Same return value as the sequential
+    version."""
+    indsR = contract_inds(T1.inds, l1, T2.inds, l2, lR)
+    m12 = _find_matching_positions(l1, l2)
+    m1R = _find_matching_positions(l1, lR)
+    m2R = _find_matching_positions(l2, lR)
+    blocks1, blocks2 = list(T1.offsets), list(T2.offsets)
+    if len(blocks1) > len(blocks2):
+        order = [(b1, b2) for b1 in blocks1 for b2 in blocks2]
+    else:
+        order = [(b1, b2) for b2 in blocks2 for b1 in blocks1]
+    plan = []
+    for b1, b2 in order:  # maybe_contract_blocks (contract_utilities.jl:17-26)
+        if not all(p == 0 or b1[i] == b2[p - 1] for i, p in enumerate(m12)):
+            continue
+        bR = [0] * len(lR)
+        for i, p in enumerate(m1R):
+            if p > 0:
+                bR[p - 1] = b1[i]
+        for i, p in enumerate(m2R):
+            if p > 0:
+                bR[p - 1] = b2[i]
+        plan.append((b1, b2, tuple(bR)))
+    offsR = OrderedDict()
+    nnzR = 0
+    for _, _, bR in plan:
+        if bR not in offsR:
+            offsR[bR] = nnzR
+            nnzR += int(np.prod(blockdims(indsR, bR)))
+    return offsR, nnzR, plan, indsR
+
+
 def blocksparse_contract(T1, l1, T2, l2, lR=None, alpha=1.0, exact=False, conj1=False, conj2=False,
-                         pair_alpha=None):
+                         pair_alpha=None, threaded=False):
     """NDTensors.contract(::BlockSparseTensor, ...) (blocksparse/contract.jl:3-76,
     contract_generic.jl:37-129): plan, allocate R, then for each output block
     the pairs in plan order with beta = 0 for the first and 1 afterwards.
     pair_alpha: None, or one factor per pair of the plan (enumeration order)
     multiplying alpha -- the value compute_alpha returns for that pair
     (contract_utilities.jl:2-15, called per pair at contract_generic.jl:91).
+    threaded: the plan of contract_blockoffsets_threaded (the reference with
+    threaded block sparse on); the sums into each output block then follow
+    that plan's order.
     Returns (R, plan)."""
     if lR is None:
         lR = contract_labels(l1, l2)
-    offsR, nnzR, plan, indsR = contract_blockoffsets_sequential(T1, l1, T2, l2, lR)
+    plan_of = contract_blockoffsets_threaded if threaded else contract_blockoffsets_sequential
+    offsR, nnzR, plan, indsR = plan_of(T1, l1, T2, l2, lR)
     if pair_alpha is not None and len(pair_alpha) != len(plan):
         raise ValueError(f"pair_alpha has {len(pair_alpha)} factors for {len(plan)} pairs")
     cplx = (np.iscomplexobj(T1.data) or np.iscomplexobj(T2.data) or np.iscomplexobj(alpha)

@@ -143,6 +143,75 @@ def test_plan_brute_force_and_flux_invariant():
         done += 1
 
 
+def test_threaded_plan_against_sequential():
+    """contract_blockoffsets_threaded (contract_threaded.jl:2-73) holds the
+    sequential plan's pairs; it is the sequential plan itself when T1 has
+    more blocks than T2, and otherwise the sequential plan of the swapped
+    product T2 * T1 with every pair swapped back (T2's blocks outer). Output
+    offsets follow first appearance over its own plan (contract_generic.jl:
+    23-30), and the contraction's values do not depend on the order."""
+    rng = np.random.default_rng(21)
+    kinds = set()
+    done = 0
+    while done < 30:
+        try:
+            A, B = _random_case(rng, 3, int(rng.integers(2, 4)), int(rng.integers(1, 3)))
+        except ValueError:
+            continue
+        la, lb = O.compute_contraction_labels(A.inds, B.inds)
+        lR = O.contract_labels(la, lb)
+        offs_s, nnz_s, plan_s, _ = O.contract_blockoffsets_sequential(A, la, B, lb, lR)
+        offs_t, nnz_t, plan_t, _ = O.contract_blockoffsets_threaded(A, la, B, lb, lR)
+        assert sorted(plan_t) == sorted(plan_s) and nnz_t == nnz_s
+        if len(A.offsets) > len(B.offsets):
+            assert plan_t == plan_s and offs_t == offs_s
+        else:
+            _, _, plan_sw, _ = O.contract_blockoffsets_sequential(B, lb, A, la, lR)
+            assert plan_t == [(b1, b2, bR) for b2, b1, bR in plan_sw]
+        first = []
+        for _, _, bR in plan_t:
+            if bR not in first:
+                first.append(bR)
+        assert list(offs_t) == first
+        n = 0
+        for bR, o in offs_t.items():
+            assert o == n
+            n += int(np.prod(O.blockdims(O.contract_inds(A.inds, la, B.inds, lb, lR), bR)))
+        Cs, _ = O.blocksparse_contract(A, la, B, lb, lR)
+        Ct, _ = O.blocksparse_contract(A, la, B, lb, lR, threaded=True)
+        ref = Cs.dense()
+        assert np.allclose(Ct.dense(), ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
+        kinds.add((len(A.offsets) > len(B.offsets), plan_t == plan_s))
+        done += 1
+    # both branches ran, and the T2-outer one changed the order somewhere
+    assert (True, True) in kinds and (False, False) in kinds
+
+
+def test_kat_threaded_contraction():
+    """test/threading/test_threading.jl:30-58 (A' * A with and without
+    threaded block sparse: equal tensors) and :60-78 (no matching pair: no
+    blocks either way), on the oracle."""
+    rng = np.random.default_rng(4)
+    i = qidx([(0, 50), (1, 50)], "i")
+    A = O.random_blocksparse(rng, O.QN(0), (i.prime(), i.dag()))
+    Ap = O.BlockSparse(A.data, A.offsets, tuple(x.prime() for x in A.inds))
+    la, lb = O.compute_contraction_labels(Ap.inds, A.inds)
+    R, plan = O.blocksparse_contract(Ap, la, A, lb)
+    Rt, plan_t = O.blocksparse_contract(Ap, la, A, lb, threaded=True)
+    assert sorted(plan) == sorted(plan_t)
+    assert np.allclose(R.dense(), Rt.dense(), rtol=0, atol=1e-12 * np.abs(R.dense()).max())
+    j = qidx([(0, 1), (1, 1)], "j")
+    offA, _ = O.blockoffsets([(0, 0)], (j.prime(), j.dag()))
+    offB, _ = O.blockoffsets([(1, 1)], (j.prime(), j.dag()))
+    A1 = O.BlockSparse(np.array([11.0]), offA, (j.prime(), j.dag()))
+    B1 = O.BlockSparse(np.array([22.0]), offB, (j.prime(), j.dag()))
+    A1p = O.BlockSparse(A1.data, A1.offsets, tuple(x.prime() for x in A1.inds))
+    la, lb = O.compute_contraction_labels(A1p.inds, B1.inds)
+    for threaded in (False, True):
+        C, plan = O.blocksparse_contract(A1p, la, B1, lb, threaded=threaded)
+        assert plan == [] and len(C.offsets) == 0 and C.data.size == 0
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 def test_blocksparse_values_vs_dense(cplx):
     rng = np.random.default_rng(3)
