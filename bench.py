@@ -50,7 +50,8 @@ def parse():
                          "library INT8 GEMM)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the other configs of the metric (dense Float64 chi=1024, "
-                         "block-sparse ComplexF64 chi=4096, dense ComplexF64 chi=4096)")
+                         "block-sparse ComplexF64 chi=4096, dense ComplexF64 chi=4096, the "
+                         "chi=1024 energy gradient, the 256-network batch)")
     ap.add_argument("--diag-only", action="store_true",
                     help="run only the DiagBlockSparse x BlockSparse measurement (chi=4096 psi "
                          "times a diagonal over one bond) and print its JSON")
@@ -448,6 +449,80 @@ def secondary_configs(args, torch, dev):
     return out
 
 
+def c5_configs(args, torch, dev):
+    """Differentiation and batching. (1) E = Re<psi|H psi>/<psi|psi> over
+    the dense Float64 two-site network at chi = 1024 and its gradient with
+    respect to psi by reverse mode through rrule(contract) (the forward
+    network, the inner product and the pullbacks of all five contractions,
+    each pullback forming both cotangents as the rrule does; eager, host
+    planning included), FP64 and split. (2) 256 independent dense networks
+    with bond dimensions uniform in [16, 96], one grouped launch per kernel
+    and step, replayed from one captured graph (FP64); CPU beside it: the
+    oracle's pairwise contractions of the first 32 networks."""
+    import time
+    import numpy as np
+    import itg
+    from itg import chainrules as CR
+    from itg.configs import heff_network, heff_flops
+    from itg.network import BatchedSequence
+    from oracle import ndtensors_oracle as O
+
+    out = {}
+    log(args, "[bench] energy gradient, dense chi = 1024")
+    L, psi, W1, W2, R, ix = heff_network(1024, qns=False, seed=4, device=dev)
+    r = {"gflop_forward_apply": float(sum(heff_flops(L, psi, W1, W2, R))) / 1e9}
+    grads = {}
+    for mode in ("fp64", "split_int8"):
+        E, g = CR.heff_energy_and_grad(L, psi, W1, W2, R, ix.out(), prec=mode)  # warm-up
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            E, g = CR.heff_energy_and_grad(L, psi, W1, W2, R, ix.out(), prec=mode)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        grads[mode] = g.tensor.data.detach().clone()
+        r[mode] = {"ms_per_eval": float(np.median(ts)), "E": float(E)}
+    gd, gs = grads["fp64"], grads["split_int8"]
+    r["grad_rel_diff_split_vs_fp64"] = float(torch.linalg.vector_norm(gs - gd) /
+                                             torch.linalg.vector_norm(gd))
+    out["energy_gradient_dense_chi1024_f64"] = r
+    log(args, f"[bench]   {json.dumps(r)}")
+    del L, psi, W1, W2, R, grads, gd, gs
+    torch.cuda.empty_cache()
+
+    log(args, "[bench] batch of 256 dense networks, chi in [16, 96]")
+    rng = np.random.default_rng(6)
+    chis = rng.integers(16, 97, size=256)
+    nets = [list(heff_network(int(chi), qns=False, seed=1 + n, device=dev)[:5])
+            for n, chi in enumerate(chis)]
+    flops = float(sum(sum(heff_flops(*net)) for net in nets))
+    batch = BatchedSequence(nets)
+    batch.run()
+    g = batch.capture()
+    ms = time_steps(torch, g.replay, 20, 3)
+    r = {"networks": 256, "chi_range": [16, 96], "gflop_per_batch": flops / 1e9,
+         "ms_per_batch": ms, "gflops": flops / (ms * 1e-3) / 1e9,
+         "networks_per_s": 256 / (ms * 1e-3)}
+    net = nets[3]
+    ref = _dense_oracle_rows(net, np.arange(net[0].inds[0].dim), O, itg)
+    shape = tuple(i.dim for i in batch.results[3].inds)
+    got = batch.results[3].tensor.data.detach().cpu().numpy().reshape(shape, order="F")
+    r["rel_err_vs_oracle_network3"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        for net in nets[:32]:
+            _dense_oracle_rows(net, np.arange(net[0].inds[0].dim), O, itg)
+        dt = time.perf_counter() - t0
+        r["cpu_baseline"] = {"value": 32 / dt, "unit": "networks/s", "kind": "port",
+                             "cores": _blas_threads(),
+                             "sample": "the oracle's pairwise contractions of the first 32 "
+                                       "networks of the batch (numpy, BLAS threads as set)"}
+    out["batch_256_dense_chi16_96_f64"] = r
+    log(args, f"[bench]   {json.dumps(r)}")
+    return out
+
+
 def diag_config(args, torch, dev):
     """DiagBlockSparse x BlockSparse at chi = 4096: psi(a, s1, s2, b) times
     a diagonal S(dag(b), b') (the singular values of a bond, random here),
@@ -796,6 +871,7 @@ def main():
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         out["secondary_configs"] = secondary_configs(args, torch, dev)
+        out["secondary_configs"].update(c5_configs(args, torch, dev))
         out["diag_blocksparse"] = diag_config(args, torch, dev)
         out["permute_blocksparse"] = permute_config(args, torch, dev)
     if not args.no_peaks:

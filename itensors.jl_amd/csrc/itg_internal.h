@@ -318,6 +318,20 @@ hipError_t launch_gemm_f64(hipStream_t s, int cfg, int cplx, int ta, int tb,
                            unsigned flags, double ga_re, double ga_im, double beta_re,
                            double beta_im);
 
+// Split-K form of the grouped FP64 GEMM: the GEMM kernel writes S partial
+// sums of each output block (one per slice of its contracted extent) into a
+// workspace, and this kernel adds them in slice order and writes C = sum +
+// beta * C (beta == 0 does not read C). One entry per output block: partials
+// at P[poff + s*M*N], s = 0..S-1, each an M x N column-major block; the
+// block's elements are e0 .. e0 + M*N - 1 of the launch.
+struct KsReduce {
+  int64_t poff, off_c, e0;
+  int32_t M, N, ldc, S;
+};
+hipError_t launch_ksplit_reduce(hipStream_t s, int cplx, const KsReduce* red, int nred,
+                                int64_t total, const void* P, void* C, double beta_re,
+                                double beta_im);
+
 // ------------------------------------------------ split-precision GEMM
 // FP64 operands are split into kSliceCount int8 digits per element (radix
 // kSliceRadix, the first digit scaled by kSliceLead / 2^E with one E per row:

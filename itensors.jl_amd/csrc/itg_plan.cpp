@@ -676,6 +676,109 @@ hipError_t build_permute_launch(PermuteLaunch* pl, int n, const int* perm, const
 }
 
 // ----------------------------------------------------------- exec plan
+// Split-K tables of a GEMM plan (FP64). A plan whose output blocks make few
+// 64 x 64 tiles gives each workgroup a whole tile's contracted extent -- an
+// inner product, or the cotangent of a small tensor such as an MPO factor,
+// is one tile over millions of K values, i.e. one workgroup. Such a plan
+// also gets a split form: the pairs of each output block are laid end to end
+// along K, that range is cut into S slices of at least kKsSlice values, and
+// every (tile, slice) is a workgroup writing a partial block; the reduce
+// kernel adds the partials in slice order. Nothing is built (ks_grid = 0)
+// for plans with enough tiles, or when no block would be split.
+std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
+  constexpr int64_t kKsTiles = 128;       // split only below this many tiles
+  constexpr int64_t kKsSlice = 2048;      // contracted values per slice, at least
+  constexpr int64_t kKsTarget = 2048;     // workgroups aimed for
+  constexpr int64_t kKsPartial = 1 << 21; // partial elements, at most (32 MiB complex)
+  const size_t ng = ep->h_groups.size();
+  if (ng == 0) return "";
+  std::vector<int64_t> tiles(ng), klen(ng), S(ng, 1);
+  int64_t ttot = 0;
+  for (size_t g = 0; g < ng; ++g) {
+    const GemmGroup& G = ep->h_groups[g];
+    tiles[g] = (int64_t)((G.M + 63) / 64) * ((G.N + 63) / 64);
+    klen[g] = 0;
+    for (int p = G.p0; p < G.p1; ++p) klen[g] += std::max(ep->h_pairs[p].K, 0);
+    ttot += tiles[g];
+  }
+  if (ttot == 0 || ttot >= kKsTiles) return "";
+  const int64_t smax = std::max<int64_t>(1, kKsTarget / ttot);
+  int64_t partial = 0;
+  for (size_t g = 0; g < ng; ++g) {
+    S[g] = std::max<int64_t>(1, std::min<int64_t>(klen[g] / kKsSlice, smax));
+    partial += S[g] * ep->h_groups[g].M * ep->h_groups[g].N;
+  }
+  while (partial > kKsPartial) {
+    partial = 0;
+    for (size_t g = 0; g < ng; ++g) {
+      S[g] = std::max<int64_t>(1, S[g] / 2);
+      partial += S[g] * ep->h_groups[g].M * ep->h_groups[g].N;
+    }
+  }
+  if (*std::max_element(S.begin(), S.end()) <= 1) return "";
+  // sub-pairs: a pair cut at slice boundaries keeps its leading dimensions;
+  // its operands start k0 values further along K
+  std::vector<GemmPair> sp;
+  std::vector<GemmGroup> sg;
+  std::vector<KsReduce> red(ng);
+  int64_t poff = 0, e0 = 0, maxk = 0;
+  for (size_t g = 0; g < ng; ++g) {
+    const GemmGroup& G = ep->h_groups[g];
+    const int64_t mn = (int64_t)G.M * G.N;
+    red[g] = KsReduce{poff, G.off_c, e0, G.M, G.N, G.ldc, (int32_t)S[g]};
+    maxk = std::max(maxk, klen[g]);
+    int p = G.p0;
+    int64_t kin = 0;  // values of pair p already given to earlier slices
+    for (int64_t q = 0; q < S[g]; ++q) {
+      const int64_t want = klen[g] * (q + 1) / S[g] - klen[g] * q / S[g];
+      GemmGroup h = G;
+      h.off_c = poff + q * mn;
+      h.ldc = std::max(G.M, 1);
+      h.p0 = (int32_t)sp.size();
+      int64_t got = 0;
+      while (got < want && p < G.p1) {
+        const GemmPair& pr = ep->h_pairs[p];
+        const int64_t left = std::max(pr.K, 0) - kin;
+        if (left <= 0) {
+          ++p;
+          kin = 0;
+          continue;
+        }
+        const int64_t take = std::min(left, want - got);
+        GemmPair x = pr;
+        x.K = (int32_t)take;
+        x.off_a += ep->ta ? kin : kin * pr.lda;
+        x.off_b += ep->tb ? kin * pr.ldb : kin;
+        sp.push_back(x);
+        got += take;
+        kin += take;
+      }
+      h.p1 = (int32_t)sp.size();
+      sg.push_back(h);
+    }
+    poff += S[g] * mn;
+    e0 += mn;
+  }
+  ExecPlan tab;
+  tab.h_groups = sg;
+  tab.h_pairs = sp;
+  std::vector<int2> tiles_tab = xcd_tile_table(tab, gemm_cfg(0));
+  hipError_t e;
+  if ((e = ep->ks_pairs.upload(sp.data(), std::max<size_t>(sp.size(), 1) * sizeof(GemmPair), s)) !=
+          hipSuccess ||
+      (e = ep->ks_groups.upload(sg.data(), sg.size() * sizeof(GemmGroup), s)) != hipSuccess ||
+      (e = ep->ks_tiles.upload(tiles_tab.data(), tiles_tab.size() * sizeof(int2), s)) !=
+          hipSuccess ||
+      (e = ep->ks_red.upload(red.data(), red.size() * sizeof(KsReduce), s)) != hipSuccess)
+    return std::string("split-K table upload failed: ") + hipGetErrorString(e);
+  ep->ks_grid = (int64_t)tiles_tab.size();
+  ep->ks_elems = poff;
+  ep->ks_total = e0;
+  ep->ks_maxk = maxk;
+  ep->ks_nred = (int)ng;
+  return "";
+}
+
 std::string build_exec_plan(const Structure& st, const BlockList& b1, const BlockList& b2,
                             const BlockList& bR, const std::vector<PairSpec>& pairs,
                             const double* pair_alpha, hipStream_t s, ExecPlan* ep,
@@ -1000,6 +1103,8 @@ std::string build_exec_plan(const Structure& st, const BlockList& b1, const Bloc
       (e = ep->g_groups.upload(ep->h_groups.data(), ep->h_groups.size() * sizeof(GemmGroup),
                                s)) != hipSuccess)
     return std::string("device table upload failed: ") + hipGetErrorString(e);
+  std::string kerr = build_ksplit(ep, s);
+  if (!kerr.empty()) return kerr;
   std::string serr = build_split_tables(ep, P, Q, Mp, Nq, ch.Xp, ch.Xq, pairs, swap, s);
   if (!serr.empty()) return serr;
   return "";
@@ -1281,10 +1386,21 @@ static bool split_cplx_scale(const ExecPlan& ep, int cplx, bool alpha_cplx) {
   return cplx && (alpha_cplx || ep.cplx_pair_alpha);
 }
 
+// Split precision hands a plan to the FP64 split-K form when one of its
+// output blocks sums over at least this many contracted values: the split
+// GEMM has no K split, and such a product is a reduction, not GEMM-shaped
+// work.
+constexpr int64_t kKsSplitPrecK = 1 << 17;
+bool ksplit_takes_split_prec(const ExecPlan& ep) {
+  return ep.ks_grid > 0 && ep.ks_maxk >= kKsSplitPrecK;
+}
+
 size_t exec_workspace_bytes(const ExecPlan& ep, int cplx, int prec, bool alpha_cplx) {
   if (ep.strategy != ExecPlan::GEMM) return 0;
   const size_t es = cplx ? 16 : 8;
   size_t b = 0;
+  if (prec != 0 && ksplit_takes_split_prec(ep)) prec = 0;
+  if (prec == 0 && ep.ks_grid > 0) b += align256((size_t)ep.ks_elems * es);
   if (prec != 0 && ep.split_ok) {
     const size_t ns = cplx ? 3 : 1;
     b += align256((size_t)ep.sA_plane_bytes * ns) + align256((size_t)ep.sB_plane_bytes * ns);
@@ -1545,7 +1661,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     if (e != hipSuccess) *err = std::string("fiber kernel launch: ") + hipGetErrorString(e);
     return e;
   }
-  if (prec != 0)
+  if (prec != 0 && !ksplit_takes_split_prec(ep))
     return exec_split(ep, cplx, prec, cfg, s, ws, T1, T2, R, flags, a_re, a_im, b_re, b_im, err,
                       kt, may_build);
   const size_t es = cplx ? 16 : 8;
@@ -1585,9 +1701,28 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
   double gbr = b_re, gbi = b_im;
   if (ep.permC) {
     C = w;
+    w += (size_t)ep.nnzC * es;
     gbr = 0.0;
     gbi = 0.0;
   }
+  if (ep.ks_grid > 0) {
+    // split-K: partial blocks into the workspace, then their sum into C
+    {
+      TimeScope ts(kt, "gemm_f64_kernel", s);
+      e = launch_gemm_f64(s, 0, cplx, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,
+                          (const GemmGroup*)ep.ks_groups.p, (const int2*)ep.ks_tiles.p,
+                          ep.ks_grid, A, B, w, gflags, a_re, a_im, 0.0, 0.0);
+    }
+    if (e == hipSuccess) {
+      TimeScope ts(kt, "ksplit_reduce_kernel", s);
+      e = launch_ksplit_reduce(s, cplx, (const KsReduce*)ep.ks_red.p, ep.ks_nred, ep.ks_total, w,
+                               C, gbr, gbi);
+    }
+    if (e != hipSuccess) {
+      *err = std::string("split-K gemm launch: ") + hipGetErrorString(e);
+      return e;
+    }
+  } else {
   if ((e = ensure_gemm_launch(ep, cfg, cplx, s, may_build)) != hipSuccess) {
     *err = e == hipErrorNotReady ? std::string(kTablesMissing)
                                  : std::string("gemm table upload: ") + hipGetErrorString(e);
@@ -1615,6 +1750,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     *err = std::string(what) + hipGetErrorString(e);
     return e;
   }
+  }  // split-K
   if (ep.permC) {
     TimeScope ts(kt, "permute_kernel", s);
     e = launch_permute(s, cplx, ep.pC.nm, ep.pC.perm, (const PermBlock*)ep.pC.blocks.p,

@@ -156,6 +156,12 @@ struct ExecPlan {
   std::vector<double> group_cost;  // flops per group (for LPT ordering)
   std::map<int, std::unique_ptr<GemmLaunch>> launches;  // per tile cfg
   int tuned[2][3] = {{-1, -1, -1}, {-1, -1, -1}};      // autotuned cfg per (complex, prec)
+  // split-K form (FP64): few output tiles over long contracted extents; the
+  // K range of each output block is cut into slices whose partial sums go
+  // to the workspace (ks_elems elements) and are added by the reduce kernel
+  int64_t ks_grid = 0, ks_elems = 0, ks_total = 0, ks_maxk = 0;
+  int ks_nred = 0;
+  DevBuf ks_pairs, ks_groups, ks_tiles, ks_red;
   double flops = 0.0;                                   // real flops (2*M*N*K summed)
   int64_t max_dim = 0;
   // split precision (real data): slice tables of the A-side operand P and

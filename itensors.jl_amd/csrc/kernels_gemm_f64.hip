@@ -30,6 +30,8 @@
 // Replaces BLAS dgemm/zgemm behind mul!! (NDTensors/src/abstractarray/mul.jl:1-12,
 // NDTensors/src/array/mul.jl:1-4) and the per-pair loop of
 // NDTensors/src/blocksparse/contract_generic.jl:78-129.
+#include <algorithm>
+
 #include "itg_internal.h"
 
 namespace itg {
@@ -251,6 +253,52 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// One thread per output element: the block is found by binary search over
+// the blocks' first elements (a split-K plan has few blocks), the partials
+// are added in slice order, and beta is applied as the GEMM's epilogue does.
+template <int CPLX>
+__global__ void __launch_bounds__(kThreads)
+    ksplit_reduce_kernel(const KsReduce* __restrict__ red, int nred, int64_t total,
+                         const double* __restrict__ P, double* __restrict__ C, double beta_re,
+                         double beta_im) {
+  const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kThreads) {
+    int lo = 0, hi = nred - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (red[mid].e0 <= e)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const KsReduce r = red[lo];
+    const int64_t loc = e - r.e0, mn = (int64_t)r.M * r.N;
+    const int64_t n = loc / r.M, m = loc - n * r.M;
+    const int64_t o = r.off_c + m + n * r.ldc;
+    if (CPLX) {
+      const double2* P2 = reinterpret_cast<const double2*>(P);
+      double2 v = P2[r.poff + loc];
+      for (int q = 1; q < r.S; ++q) {
+        const double2 w = P2[r.poff + q * mn + loc];
+        v.x += w.x;
+        v.y += w.y;
+      }
+      if (use_beta) {
+        const double2 c = reinterpret_cast<double2*>(C)[o];
+        v.x += beta_re * c.x - beta_im * c.y;
+        v.y += beta_re * c.y + beta_im * c.x;
+      }
+      reinterpret_cast<double2*>(C)[o] = v;
+    } else {
+      double v = P[r.poff + loc];
+      for (int q = 1; q < r.S; ++q) v += P[r.poff + q * mn + loc];
+      if (use_beta) v = fma(beta_re, C[o], v);
+      C[o] = v;
+    }
+  }
+}
+
 struct CfgEntry {
   int bm, bn;
 };
@@ -278,6 +326,21 @@ hipError_t launch_tt(hipStream_t s, int ta, int tb, dim3 grid, const GemmPair* p
 }
 
 }  // namespace
+
+hipError_t launch_ksplit_reduce(hipStream_t s, int cplx, const KsReduce* red, int nred,
+                                int64_t total, const void* P, void* C, double beta_re,
+                                double beta_im) {
+  if (total <= 0) return hipSuccess;
+  if (nred <= 0) return hipErrorInvalidValue;
+  const int64_t blocks = std::min<int64_t>((total + kThreads - 1) / kThreads, 4096);
+  if (cplx)
+    ITG_LAUNCH((ksplit_reduce_kernel<1>), dim3((unsigned)blocks), dim3(kThreads), 0, s, red,
+               nred, total, (const double*)P, (double*)C, beta_re, beta_im);
+  else
+    ITG_LAUNCH((ksplit_reduce_kernel<0>), dim3((unsigned)blocks), dim3(kThreads), 0, s, red,
+               nred, total, (const double*)P, (double*)C, beta_re, beta_im);
+  return hipGetLastError();
+}
 
 int gemm_num_cfgs() { return (int)(sizeof(kCfgs) / sizeof(kCfgs[0])); }
 GemmTileCfg gemm_cfg(int cfg) { return GemmTileCfg{kCfgs[cfg].bm, kCfgs[cfg].bn}; }
