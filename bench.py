@@ -451,7 +451,8 @@ def secondary_configs(args, torch, dev):
 
 def c5_configs(args, torch, dev):
     """Differentiation and batching. (1) E = Re<psi|H psi>/<psi|psi> over
-    the dense Float64 two-site network at chi = 1024 and its gradient with
+    the two-site network at chi = 1024 (dense, and the U(1) block structure
+    scaled to chi = 1024), Float64, and its gradient with
     respect to psi by reverse mode through rrule(contract) (the forward
     network, the inner product and the pullbacks of all five contractions,
     each pullback forming both cotangents as the rrule does; eager, host
@@ -468,28 +469,30 @@ def c5_configs(args, torch, dev):
     from oracle import ndtensors_oracle as O
 
     out = {}
-    log(args, "[bench] energy gradient, dense chi = 1024")
-    L, psi, W1, W2, R, ix = heff_network(1024, qns=False, seed=4, device=dev)
-    r = {"gflop_forward_apply": float(sum(heff_flops(L, psi, W1, W2, R))) / 1e9}
-    grads = {}
-    for mode in ("fp64", "split_int8"):
-        E, g = CR.heff_energy_and_grad(L, psi, W1, W2, R, ix.out(), prec=mode)  # warm-up
-        torch.cuda.synchronize()
-        ts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            E, g = CR.heff_energy_and_grad(L, psi, W1, W2, R, ix.out(), prec=mode)
+    for qns, name in ((False, "energy_gradient_dense_chi1024_f64"),
+                      (True, "energy_gradient_u1_blocksparse_chi1024_f64")):
+        log(args, f"[bench] {name}")
+        L, psi, W1, W2, R, ix = heff_network(1024, qns=qns, seed=4, device=dev)
+        r = {"gflop_forward_apply": float(sum(heff_flops(L, psi, W1, W2, R))) / 1e9}
+        grads = {}
+        for mode in ("fp64", "split_int8"):
+            E, g = CR.heff_energy_and_grad(L, psi, W1, W2, R, ix.out(), prec=mode)  # warm-up
             torch.cuda.synchronize()
-            ts.append((time.perf_counter() - t0) * 1e3)
-        grads[mode] = g.tensor.data.detach().clone()
-        r[mode] = {"ms_per_eval": float(np.median(ts)), "E": float(E)}
-    gd, gs = grads["fp64"], grads["split_int8"]
-    r["grad_rel_diff_split_vs_fp64"] = float(torch.linalg.vector_norm(gs - gd) /
-                                             torch.linalg.vector_norm(gd))
-    out["energy_gradient_dense_chi1024_f64"] = r
-    log(args, f"[bench]   {json.dumps(r)}")
-    del L, psi, W1, W2, R, grads, gd, gs
-    torch.cuda.empty_cache()
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                E, g = CR.heff_energy_and_grad(L, psi, W1, W2, R, ix.out(), prec=mode)
+                torch.cuda.synchronize()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            grads[mode] = g.tensor.data.detach().clone()
+            r[mode] = {"ms_per_eval": float(np.median(ts)), "E": float(E)}
+        gd, gs = grads["fp64"], grads["split_int8"]
+        r["grad_rel_diff_split_vs_fp64"] = float(torch.linalg.vector_norm(gs - gd) /
+                                                 torch.linalg.vector_norm(gd))
+        out[name] = r
+        log(args, f"[bench]   {json.dumps(r)}")
+        del L, psi, W1, W2, R, grads, gd, gs
+        torch.cuda.empty_cache()
 
     log(args, "[bench] batch of 256 dense networks, chi in [16, 96]")
     rng = np.random.default_rng(6)
