@@ -51,6 +51,8 @@ def test_header_constants_match_binding():
     assert val("ITG_PREC_FP64") == _lib.PREC["fp64"]
     assert val("ITG_PREC_SPLIT_BF16") == _lib.PREC["split_bf16"]
     assert val("ITG_PREC_SPLIT_INT8") == _lib.PREC["split_int8"]
+    assert val("ITG_BS_ORDER_SEQUENTIAL") == _lib.ITG_BS_ORDER_SEQUENTIAL
+    assert val("ITG_BS_ORDER_THREADED") == _lib.ITG_BS_ORDER_THREADED
 
 
 def test_version_and_init_without_device():
@@ -78,6 +80,8 @@ def test_null_handle_is_rejected():
     assert L.itg_timer_enable(None, 1) == _lib.ITG_ERR_INVALID
     assert L.itg_timer_count(None) == 0
     assert L.itg_set_split_tile(None, 1) == _lib.ITG_ERR_INVALID
+    assert L.itg_set_gemm_backend(None, 0) == _lib.ITG_ERR_INVALID
+    assert L.itg_set_blocksparse_order(None, _lib.ITG_BS_ORDER_THREADED) == _lib.ITG_ERR_INVALID
     assert L.itg_bs_copy(None, _lib.ITG_F64, 1, 1, None, None, None, None, None, None, 0,
                          None, 0) == _lib.ITG_ERR_INVALID
 

@@ -70,6 +70,9 @@ def test_switch_state(env):
     assert itg.disable_threaded_blocksparse()
     itg.enable_threaded_blocksparse(e0)
     assert itg.using_threaded_blocksparse() == e0
+    from itg import _lib
+    h = _lib.handle_for(0)
+    assert _lib.lib().itg_set_blocksparse_order(h.h, 7) == _lib.ITG_ERR_INVALID
 
 
 @pytest.mark.parametrize("cplx", [False, True], ids=["f64", "c128"])
