@@ -51,23 +51,42 @@ def replaceinds(x, inds):
     return ITensor(x.tensor, inds)
 
 
+_LAYOUT_MAPS = {}  # (src structure, dst structure, device) -> map, block-sparse only
+_LAYOUT_MAPS_MAX = 8
+
+
 def _layout_map(src, dst):
     """Element positions in dst's storage of every element of src's storage
     (same index set; src's blocks must be blocks of dst). Returns a device
-    int64 tensor, or None when the layouts are identical."""
+    int64 tensor, or None when the layouts are identical. Block-sparse maps
+    (one small index computation per block) are kept for the structures seen
+    last: an eigensolver's gradients repeat them."""
+    if tuple(src.inds) == tuple(dst.inds):
+        if not src.hasqns() or list(src.tensor.blockoffsets.items()) == list(dst.tensor.blockoffsets.items()):
+            return None
+    if not src.hasqns():
+        return _build_layout_map(src, dst)
+    key = (tuple(src.inds), tuple(dst.inds), src.tensor.data.device,
+           tuple(src.tensor.blockoffsets.items()), tuple(dst.tensor.blockoffsets.items()))
+    m = _LAYOUT_MAPS.get(key)
+    if m is None:
+        m = _build_layout_map(src, dst)
+        if len(_LAYOUT_MAPS) >= _LAYOUT_MAPS_MAX:
+            _LAYOUT_MAPS.pop(next(iter(_LAYOUT_MAPS)))
+        _LAYOUT_MAPS[key] = m
+    return m
+
+
+def _build_layout_map(src, dst):
     import math
 
     import torch
 
-    if tuple(src.inds) == tuple(dst.inds):
-        if not src.hasqns() or list(src.tensor.blockoffsets.items()) == list(dst.tensor.blockoffsets.items()):
-            return None
     perm = [src.inds.index(i) for i in dst.inds]  # dst mode k = src mode perm[k]
     chunks = []
     if not src.hasqns():
         dims = src.tensor.dims
-        idx = torch.arange(math.prod(dims), device=src.tensor.data.device).reshape(tuple(reversed(dims)))
-        # idx[c_{n-1},...,c_0] = linear src index; dst position of each src element
+        # dst position of each src element
         ddims = tuple(dims[p] for p in perm)
         dst_lin = torch.arange(math.prod(ddims), device=src.tensor.data.device).reshape(tuple(reversed(ddims)))
         # dst coordinate k = src coordinate perm[k]; permute dst_lin's axes to src order

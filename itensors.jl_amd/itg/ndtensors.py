@@ -304,8 +304,28 @@ class BlockSparseTensor:
         self.blockdims = tuple(tuple(int(x) for x in bd) for bd in blockdims)
         if structure_only:  # block structure for planning; holds no values
             return
+        self._validate()
+
+    @classmethod
+    def _checked(cls, data, blockoffsets, blockdims, end):
+        """A tensor whose block structure was already checked: the output
+        structure of a plan (built and checked by the library, `end` = its
+        nnz), or the structure of an existing tensor with new data of the
+        same length. Only the data length is checked."""
+        T = cls.__new__(cls)
+        T.data = data
+        T.blockoffsets = blockoffsets if isinstance(blockoffsets, OrderedDict) \
+            else OrderedDict(blockoffsets)
+        T.blockdims = tuple(tuple(bd) for bd in blockdims)
+        if data is None or data.dim() != 1 or data.numel() < end:
+            raise ValueError("BlockSparseTensor data must be a flat vector covering its blocks")
+        T._end = end
+        return T
+
+    def _validate(self):
         # every block must lie inside the data: the device kernels index the
         # flat vector through these offsets without further checks
+        data = self.data
         if data is None or data.dim() != 1:
             raise ValueError("BlockSparseTensor data must be a flat vector")
         end = 0
@@ -318,6 +338,7 @@ class BlockSparseTensor:
             end = max(end, o + math.prod(self.blockdims_of(b)))
         if data.numel() < end:
             raise ValueError(f"data holds {data.numel()} elements, the blocks reach {end}")
+        self._end = end
 
     @property
     def eltype(self):
@@ -515,7 +536,8 @@ def _contract_diag(T1, labels1, T2, labels2, labelsR, alpha=1.0):
         if not p.npairs:
             data = torch.zeros(math.prod(dims), dtype=B.eltype, device=B.data.device)
         return DenseTensor(data, dims)
-    return BlockSparseTensor(data, OrderedDict(zip(p.blocksR, p.offsR)), blockdims)
+    return BlockSparseTensor._checked(data, OrderedDict(zip(p.blocksR, p.offsR)), blockdims,
+                                      p.nnzR)
 
 
 def _promote(T1, T2):
@@ -547,6 +569,9 @@ def convert(data, dtype):
 def _with_data(T, data):
     if isinstance(T, DenseTensor):
         return DenseTensor(data, T.dims)
+    end = getattr(T, "_end", None)
+    if end is not None and type(T) is BlockSparseTensor:
+        return BlockSparseTensor._checked(data, T.blockoffsets, T.blockdims, end)
     return BlockSparseTensor(data, T.blockoffsets, T.blockdims)
 
 
@@ -806,7 +831,8 @@ def contract(T1, labels1, T2, labels2, labelsR=None, *, alpha=1.0, prec=None, co
             else:
                 blockdims.append(T2.blockdims[labels2.index(r)])
         data = torch.empty(p.nnzR, dtype=T1.eltype, device=T1.data.device)
-        R = BlockSparseTensor(data, OrderedDict(zip(p.blocksR, p.offsR)), blockdims)
+        R = BlockSparseTensor._checked(data, OrderedDict(zip(p.blocksR, p.offsR)), blockdims,
+                                       p.nnzR)
         if p.npairs:
             flags = (_lib.ITG_FLAG_CONJ_A if conj1 else 0) | (_lib.ITG_FLAG_CONJ_B if conj2 else 0)
             p.execute(data, T1.data, T2.data, alpha=alpha, prec=prec, flags=flags)

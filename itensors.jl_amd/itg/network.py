@@ -108,7 +108,7 @@ class ContractionSequence:
             bd = [A.tensor.blockdims[lA.index(r)] if r in lA else B.tensor.blockdims[lB.index(r)]
                   for r in lR]
             data = torch.empty(p.nnzR, dtype=A.eltype, device=A.tensor.data.device)
-            R = ND.BlockSparseTensor(data, dict(zip(p.blocksR, p.offsR)), bd)
+            R = ND.BlockSparseTensor._checked(data, dict(zip(p.blocksR, p.offsR)), bd, p.nnzR)
             return ("bs", p, A.tensor, B.tensor, R, (lA, lB, lR)), ITensor(R, indsR)
         dims = tuple(i.dim for i in indsR)
         R = ND.DenseTensor(torch.empty(math.prod(dims), dtype=A.eltype,
