@@ -328,9 +328,10 @@ struct KsReduce {
   int64_t poff, off_c, e0;
   int32_t M, N, ldc, S;
 };
+// max_slices: the largest S of the entries (many slices: a wave per element).
 hipError_t launch_ksplit_reduce(hipStream_t s, int cplx, const KsReduce* red, int nred,
-                                int64_t total, const void* P, void* C, double beta_re,
-                                double beta_im);
+                                int64_t total, int max_slices, const void* P, void* C,
+                                double beta_re, double beta_im);
 
 // ------------------------------------------------ split-precision GEMM
 // FP64 operands are split into kSliceCount int8 digits per element (radix

@@ -776,6 +776,7 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
   ep->ks_total = e0;
   ep->ks_maxk = maxk;
   ep->ks_nred = (int)ng;
+  ep->ks_maxs = (int)*std::max_element(S.begin(), S.end());
   return "";
 }
 
@@ -1715,8 +1716,8 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
     }
     if (e == hipSuccess) {
       TimeScope ts(kt, "ksplit_reduce_kernel", s);
-      e = launch_ksplit_reduce(s, cplx, (const KsReduce*)ep.ks_red.p, ep.ks_nred, ep.ks_total, w,
-                               C, gbr, gbi);
+      e = launch_ksplit_reduce(s, cplx, (const KsReduce*)ep.ks_red.p, ep.ks_nred, ep.ks_total,
+                               ep.ks_maxs, w, C, gbr, gbi);
     }
     if (e != hipSuccess) {
       *err = std::string("split-K gemm launch: ") + hipGetErrorString(e);

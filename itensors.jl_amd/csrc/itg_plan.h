@@ -160,7 +160,7 @@ struct ExecPlan {
   // K range of each output block is cut into slices whose partial sums go
   // to the workspace (ks_elems elements) and are added by the reduce kernel
   int64_t ks_grid = 0, ks_elems = 0, ks_total = 0, ks_maxk = 0;
-  int ks_nred = 0;
+  int ks_nred = 0, ks_maxs = 0;
   DevBuf ks_pairs, ks_groups, ks_tiles, ks_red;
   double flops = 0.0;                                   // real flops (2*M*N*K summed)
   int64_t max_dim = 0;

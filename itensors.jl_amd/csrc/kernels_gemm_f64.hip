@@ -253,37 +253,62 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// One thread per output element: the block is found by binary search over
-// the blocks' first elements (a split-K plan has few blocks), the partials
-// are added in slice order, and beta is applied as the GEMM's epilogue does.
-template <int CPLX>
+// The block of output element e, found by binary search over the blocks'
+// first elements (a split-K plan has few blocks).
+__device__ inline int ksplit_block(const KsReduce* red, int nred, int64_t e) {
+  int lo = 0, hi = nred - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (red[mid].e0 <= e)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// WIDE = false: one thread per output element, adding the partials in slice
+// order. WIDE = true (many slices, few elements: inner products, small
+// cotangents): one wave per output element, lane l adding slices l, l + 64,
+// ... in order, then a butterfly over the lanes -- a fixed order for a given
+// slice count, so the result is deterministic. Beta is applied as the
+// GEMM's epilogue does.
+template <int CPLX, bool WIDE>
 __global__ void __launch_bounds__(kThreads)
     ksplit_reduce_kernel(const KsReduce* __restrict__ red, int nred, int64_t total,
                          const double* __restrict__ P, double* __restrict__ C, double beta_re,
                          double beta_im) {
   const bool use_beta = (beta_re != 0.0 || beta_im != 0.0);
-  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kThreads) {
-    int lo = 0, hi = nred - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (red[mid].e0 <= e)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const KsReduce r = red[lo];
+  constexpr int kPer = WIDE ? 64 : 1;  // threads per element
+  const int lane = WIDE ? (int)(threadIdx.x & 63) : 0;
+  const int64_t stride = (int64_t)gridDim.x * (kThreads / kPer);
+  for (int64_t e = (int64_t)blockIdx.x * (kThreads / kPer) + threadIdx.x / kPer; e < total;
+       e += stride) {
+    const KsReduce r = red[ksplit_block(red, nred, e)];
     const int64_t loc = e - r.e0, mn = (int64_t)r.M * r.N;
+    double vr = 0.0, vi = 0.0;
+    for (int q = lane; q < r.S; q += kPer) {
+      if (CPLX) {
+        const double2 w = reinterpret_cast<const double2*>(P)[r.poff + q * mn + loc];
+        vr = q == lane ? w.x : vr + w.x;
+        vi = q == lane ? w.y : vi + w.y;
+      } else {
+        const double w = P[r.poff + q * mn + loc];
+        vr = q == lane ? w : vr + w;
+      }
+    }
+    if (WIDE) {
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        vr += __shfl_xor(vr, m);
+        if (CPLX) vi += __shfl_xor(vi, m);
+      }
+      if (lane != 0) continue;
+    }
     const int64_t n = loc / r.M, m = loc - n * r.M;
     const int64_t o = r.off_c + m + n * r.ldc;
     if (CPLX) {
-      const double2* P2 = reinterpret_cast<const double2*>(P);
-      double2 v = P2[r.poff + loc];
-      for (int q = 1; q < r.S; ++q) {
-        const double2 w = P2[r.poff + q * mn + loc];
-        v.x += w.x;
-        v.y += w.y;
-      }
+      double2 v = make_double2(vr, vi);
       if (use_beta) {
         const double2 c = reinterpret_cast<double2*>(C)[o];
         v.x += beta_re * c.x - beta_im * c.y;
@@ -291,8 +316,7 @@ __global__ void __launch_bounds__(kThreads)
       }
       reinterpret_cast<double2*>(C)[o] = v;
     } else {
-      double v = P[r.poff + loc];
-      for (int q = 1; q < r.S; ++q) v += P[r.poff + q * mn + loc];
+      double v = vr;
       if (use_beta) v = fma(beta_re, C[o], v);
       C[o] = v;
     }
@@ -328,17 +352,25 @@ hipError_t launch_tt(hipStream_t s, int ta, int tb, dim3 grid, const GemmPair* p
 }  // namespace
 
 hipError_t launch_ksplit_reduce(hipStream_t s, int cplx, const KsReduce* red, int nred,
-                                int64_t total, const void* P, void* C, double beta_re,
-                                double beta_im) {
+                                int64_t total, int max_slices, const void* P, void* C,
+                                double beta_re, double beta_im) {
   if (total <= 0) return hipSuccess;
   if (nred <= 0) return hipErrorInvalidValue;
-  const int64_t blocks = std::min<int64_t>((total + kThreads - 1) / kThreads, 4096);
-  if (cplx)
-    ITG_LAUNCH((ksplit_reduce_kernel<1>), dim3((unsigned)blocks), dim3(kThreads), 0, s, red,
-               nred, total, (const double*)P, (double*)C, beta_re, beta_im);
+  const bool wide = max_slices >= 64;
+  const int64_t per_block = wide ? kThreads / 64 : kThreads;
+  const int64_t blocks = std::min<int64_t>((total + per_block - 1) / per_block, 4096);
+#define ITG_KS_LAUNCH(c, w)                                                                  \
+  ITG_LAUNCH((ksplit_reduce_kernel<c, w>), dim3((unsigned)blocks), dim3(kThreads), 0, s, red, \
+             nred, total, (const double*)P, (double*)C, beta_re, beta_im)
+  if (cplx && wide)
+    ITG_KS_LAUNCH(1, true);
+  else if (cplx)
+    ITG_KS_LAUNCH(1, false);
+  else if (wide)
+    ITG_KS_LAUNCH(0, true);
   else
-    ITG_LAUNCH((ksplit_reduce_kernel<0>), dim3((unsigned)blocks), dim3(kThreads), 0, s, red,
-               nred, total, (const double*)P, (double*)C, beta_re, beta_im);
+    ITG_KS_LAUNCH(0, false);
+#undef ITG_KS_LAUNCH
   return hipGetLastError();
 }
 
