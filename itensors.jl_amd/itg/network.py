@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import math
 
+from . import _lib
 from . import ndtensors as ND
 from ._lib import ItgUnsupported
 from .itensors import ITensor, compute_contraction_labels
@@ -138,8 +139,6 @@ class ContractionSequence:
         self.chains[i].execute(Z.data, X.data, B1.data, B2.data)
 
     def _flags(self, i, prec):
-        from . import _lib
-
         ops = self._fixed_bits.get(i, 0)
         if not ops:
             return 0
@@ -204,8 +203,6 @@ class BatchPlan:
 
     def __init__(self, A_list, lA, B_list, lB, R_list, lR):
         import ctypes
-        from . import _lib
-
         n = len(A_list)
         dev = R_list[0].data.device
         self.h = ND._handle(dev)
@@ -236,15 +233,12 @@ class BatchPlan:
         self.tensors = (list(A_list), list(B_list), list(R_list))
 
     def execute(self, prec=None, alpha=1.0):
-        from . import _lib
-
         h = ND._handle(self.dev)
         a, _k = _lib.scalar2(alpha)
         h.check(_lib.lib().itg_batch_execute(h.h, self.plan, a, _lib.PREC[prec or ND.precision()]))
 
     def __del__(self):
-        from . import _lib
-
+        # no import here: at interpreter exit the import system is gone
         try:
             if getattr(self, "plan", None) and _lib._lib is not None:
                 _lib._lib.itg_batch_plan_destroy(self.plan)
@@ -257,9 +251,21 @@ class BatchedSequence:
     """Many independent dense networks with the same index pattern (each with
     its own dimensions) contracted left to right, one grouped launch per
     kernel of each step for the whole batch. The intermediates of a step
-    share one allocation."""
+    share one allocation.
 
-    def __init__(self, networks, *, prec=None):
+    reorder: an intermediate made by a step with a short contracted extent
+    (the strided or fiber kernels, which write any mode order at the same
+    cost) holds the indices the next step contracts last, in the order the
+    next operand holds them, so that a GEMM-shaped next step reads it in
+    place instead of permuting it (the Z of the two-site network before its
+    product with R). The result's index order is the fold's either way."""
+
+    # contracted extent below which the planner's automatic choice is not the
+    # grouped GEMM (the library's kBackendAuto rule, flop-weighted mean
+    # contracted extent >= 24)
+    _GEMM_MIN_K = 24
+
+    def __init__(self, networks, *, prec=None, reorder=True):
         import torch
 
         self.prec = prec
@@ -271,6 +277,14 @@ class BatchedSequence:
             Bs = [net[s + 1] for net in networks]
             lA, lB = compute_contraction_labels(cur[0].inds, Bs[0].inds)
             lR = ND.contract_labels(lA, lB)
+            if reorder and s + 1 < nsteps and ND.gemm_backend() == "Auto":
+                kext = math.prod(cur[0].inds[i].dim for i, x in enumerate(lA) if x < 0)
+                if kext < self._GEMM_MIN_K:
+                    out0 = tuple(cur[0].inds[lA.index(x)] if x in lA else Bs[0].inds[lB.index(x)]
+                                 for x in lR)
+                    order = ContractionSequence._order_for_next(out0, networks[0][s + 2].inds)
+                    lab = dict(zip(out0, lR))
+                    lR = tuple(lab[i] for i in order)
             outs, sizes, dims_all, inds_all = [], [], [], []
             for A, B in zip(cur, Bs):
                 la, lb = compute_contraction_labels(A.inds, B.inds)
