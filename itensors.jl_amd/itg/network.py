@@ -36,9 +36,13 @@ class ContractionSequence:
     reorder: let an intermediate produced by a fiber or strided step hold the
     indices its consumer contracts last (in the consumer's order) when the
     consumer is a grouped GEMM that would otherwise permute it first (the Z
-    of the two-site network before its product with R). Only the order of
+    of the two-site network before its product with R). Block-sparse steps
+    ask their plans (itg_bs_plan_info); a dense step with a short contracted
+    extent is reordered whenever a next step exists. Only the order of
     intermediates changes: the result's index order and values are the ones
-    of the left-to-right fold of tensor_algebra.jl:121-130."""
+    of the left-to-right fold of tensor_algebra.jl:121-130 (dense values to
+    rounding: a GEMM that reads Z in place may sum in another order than
+    one that reads a permuted copy)."""
 
     def __init__(self, tensors, *, prec=None, fuse=True, fixed=(), reorder=True):
         self.tensors = list(tensors)
@@ -76,6 +80,16 @@ class ContractionSequence:
                     if probe2 is not None and not probe2["perm1"] and \
                             step2[1].info()["strategy"] == step[1].info()["strategy"]:
                         step, cur = step2, cur2
+            elif reorder and C is not None and step[0] == "dense" and \
+                    ND.gemm_backend() == "Auto" and \
+                    _contracted_extent(a_it, step[5][0]) < BatchedSequence._GEMM_MIN_K:
+                # dense steps have no plan to ask: a step with a short
+                # contracted extent runs on the strided or fiber kernels
+                # (any output order costs the same), so its output takes the
+                # order a GEMM-shaped next step reads in place
+                order = self._order_for_next(cur.inds, C.inds)
+                if order != tuple(cur.inds):
+                    step, cur = self._build_step(a_it, B, out_inds=order)
             self.steps.append(step)
         self.result = cur
         i = 0
@@ -197,6 +211,12 @@ def _capture(run, prec):
     return g
 
 
+def _contracted_extent(A, lA):
+    """Product of the dimensions of A's modes that a step contracts (labels
+    lA of A's modes, contracted ones negative)."""
+    return math.prod(A.inds[i].dim for i, x in enumerate(lA) if x < 0)
+
+
 class BatchPlan:
     """libitg batch plan: nbatch dense contractions with one label pattern,
     bound to their device buffers, executed as grouped launches."""
@@ -278,8 +298,7 @@ class BatchedSequence:
             lA, lB = compute_contraction_labels(cur[0].inds, Bs[0].inds)
             lR = ND.contract_labels(lA, lB)
             if reorder and s + 1 < nsteps and ND.gemm_backend() == "Auto":
-                kext = math.prod(cur[0].inds[i].dim for i, x in enumerate(lA) if x < 0)
-                if kext < self._GEMM_MIN_K:
+                if _contracted_extent(cur[0], lA) < self._GEMM_MIN_K:
                     out0 = tuple(cur[0].inds[lA.index(x)] if x in lA else Bs[0].inds[lB.index(x)]
                                  for x in lR)
                     order = ContractionSequence._order_for_next(out0, networks[0][s + 2].inds)

@@ -47,3 +47,34 @@ def test_reordered_intermediate_same_result(cplx):
             assert "permute_kernel" in t0.results, t0.results
             assert "permute_kernel" not in t.results, t.results
         assert torch.equal(_bits(got), _bits(want)), prec
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["f64", "c128"])
+def test_dense_reordered_intermediate(cplx):
+    """The dense network: step 3 (K = 10, strided / fiber kernels) writes Z
+    with R's contracted modes last, so step 4's GEMM reads it in place; the
+    result matches the plain fold's to rounding (the GEMM may sum in another
+    order) and the oracle's restriction to sampled rows."""
+    import torch
+    from itg import ndtensors as ND
+    from itg.configs import heff_network
+    from itg.network import ContractionSequence
+    torch.cuda.init()
+    dt = np.complex128 if cplx else np.float64
+    net = heff_network(96, qns=False, dtype=dt, seed=6, device="cuda")[:5]
+    plain = ContractionSequence(net, reorder=False)
+    seq = ContractionSequence(net)
+    assert seq.result.inds == plain.result.inds
+    assert seq.steps[2][4].dims != plain.steps[2][4].dims  # Z's modes moved
+    for prec in ("fp64", "split_int8"):
+        with ND.timer() as t0:
+            want = plain.run(prec).tensor.data.clone()
+            torch.cuda.synchronize()
+        with ND.timer() as t:
+            got = seq.run(prec).tensor.data.clone()
+            torch.cuda.synchronize()
+        if prec == "fp64":
+            assert "permute_kernel" in t0.results, t0.results
+            assert "permute_kernel" not in t.results, t.results
+        err = float(torch.linalg.vector_norm(got - want) / torch.linalg.vector_norm(want))
+        assert err < 1e-13, (prec, err)
