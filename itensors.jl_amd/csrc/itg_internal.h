@@ -328,6 +328,15 @@ struct KsReduce {
   int64_t poff, off_c, e0;
   int32_t M, N, ldc, S;
 };
+// Dot-like split-K plans (every output block has at most kKsDotMN elements):
+// the partial blocks come from a streaming vector-ALU kernel, one workgroup
+// per entry of `groups` (the split tables), instead of the MFMA GEMM.
+constexpr int kKsDotMN = 16;
+// one: every block has a single element (inner products).
+hipError_t launch_ksplit_dot(hipStream_t s, int cplx, int one, int ta, int tb,
+                             const GemmPair* pairs, const GemmGroup* groups, int64_t ngroups,
+                             const void* A, const void* B, void* P, unsigned flags,
+                             double ga_re, double ga_im);
 // max_slices: the largest S of the entries (many slices: a wave per element).
 hipError_t launch_ksplit_reduce(hipStream_t s, int cplx, const KsReduce* red, int nred,
                                 int64_t total, int max_slices, const void* P, void* C,

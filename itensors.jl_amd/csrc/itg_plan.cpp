@@ -777,6 +777,12 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
   ep->ks_maxk = maxk;
   ep->ks_nred = (int)ng;
   ep->ks_maxs = (int)*std::max_element(S.begin(), S.end());
+  ep->ks_ngroups = (int64_t)sg.size();
+  ep->ks_dot = ep->ks_dot1 = true;
+  for (const GemmGroup& G : ep->h_groups) {
+    if ((int64_t)G.M * G.N > kKsDotMN) ep->ks_dot = false;
+    if ((int64_t)G.M * G.N != 1) ep->ks_dot1 = false;
+  }
   return "";
 }
 
@@ -1708,7 +1714,12 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
   }
   if (ep.ks_grid > 0) {
     // split-K: partial blocks into the workspace, then their sum into C
-    {
+    if (ep.ks_dot) {
+      TimeScope ts(kt, "ksplit_dot_kernel", s);
+      e = launch_ksplit_dot(s, cplx, ep.ks_dot1, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,
+                            (const GemmGroup*)ep.ks_groups.p, ep.ks_ngroups, A, B, w, gflags,
+                            a_re, a_im);
+    } else {
       TimeScope ts(kt, "gemm_f64_kernel", s);
       e = launch_gemm_f64(s, 0, cplx, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,
                           (const GemmGroup*)ep.ks_groups.p, (const int2*)ep.ks_tiles.p,

@@ -161,6 +161,9 @@ struct ExecPlan {
   // to the workspace (ks_elems elements) and are added by the reduce kernel
   int64_t ks_grid = 0, ks_elems = 0, ks_total = 0, ks_maxk = 0;
   int ks_nred = 0, ks_maxs = 0;
+  bool ks_dot = false;      // every output block has <= kKsDotMN elements
+  bool ks_dot1 = false;     // ... and exactly one
+  int64_t ks_ngroups = 0;   // (block, slice) entries of the split tables
   DevBuf ks_pairs, ks_groups, ks_tiles, ks_red;
   double flops = 0.0;                                   // real flops (2*M*N*K summed)
   int64_t max_dim = 0;

@@ -323,6 +323,106 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Split-K partial blocks of dot-like plans (every output block has at most
+// kKsDotMN elements: inner products, norms): one workgroup per (block,
+// slice) entry of the split tables streams its sub-pairs' operands once on
+// the vector ALUs (a 64 x 64 MFMA tile would compute 4096 products per
+// useful one), each thread summing every kThreads-th contracted value into
+// M x N accumulators (MNC = 1: one output, four values in flight per thread;
+// MNC = kKsDotMN: up to that many outputs); the workgroup then adds its
+// threads' sums (a fixed tree) and writes the partial block. Per pair, the
+// sum is scaled by that pair's factor times the global one, as the GEMM
+// scales its A operand.
+template <int CPLX, int MNC>
+__global__ void __launch_bounds__(kThreads)
+    ksplit_dot_kernel(const GemmPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
+                      const double* __restrict__ Ag, const double* __restrict__ Bg,
+                      double* __restrict__ P, int ta, int tb, unsigned flags, double ga_re,
+                      double ga_im) {
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int U = MNC == 1 ? 4 : 1;  // contracted values per thread in flight
+  __shared__ double red[kThreads / 64][MNC * NP];
+  const GemmGroup grp = groups[blockIdx.x];
+  const int M = grp.M, MN = MNC == 1 ? 1 : grp.M * grp.N;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double sa = (flags & 1u) ? -1.0 : 1.0;  // conj(A): negate Im(A)
+  const double sb = (flags & 2u) ? -1.0 : 1.0;
+  double tot[MNC][NP];
+#pragma unroll
+  for (int i = 0; i < MNC; ++i)
+#pragma unroll
+    for (int c = 0; c < NP; ++c) tot[i][c] = 0.0;
+  for (int p = grp.p0; p < grp.p1; ++p) {
+    const GemmPair pr = pairs[p];
+    double acc[MNC][NP];
+#pragma unroll
+    for (int i = 0; i < MNC; ++i)
+#pragma unroll
+      for (int c = 0; c < NP; ++c) acc[i][c] = 0.0;
+    for (int k0 = tid; k0 < pr.K; k0 += U * kThreads) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * kThreads;
+        if (U > 1 && k >= pr.K) break;
+#pragma unroll
+        for (int i = 0; i < MNC; ++i) {
+          if (MNC > 1 && i >= MN) continue;
+          const int m = i % M, n = i / M;
+          const int64_t oa =
+              pr.off_a + (ta ? (int64_t)m * pr.lda + k : (int64_t)k * pr.lda + m);
+          const int64_t ob =
+              pr.off_b + (tb ? (int64_t)k * pr.ldb + n : (int64_t)n * pr.ldb + k);
+          if (CPLX) {
+            const double2 a = reinterpret_cast<const double2*>(Ag)[oa];
+            const double2 b = reinterpret_cast<const double2*>(Bg)[ob];
+            const double ai = sa * a.y, bi = sb * b.y;
+            acc[i][0] = fma(a.x, b.x, fma(-ai, bi, acc[i][0]));
+            acc[i][NP - 1] = fma(a.x, bi, fma(ai, b.x, acc[i][NP - 1]));
+          } else {
+            acc[i][0] = fma(Ag[oa], Bg[ob], acc[i][0]);
+          }
+        }
+      }
+    }
+    const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
+    const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
+#pragma unroll
+    for (int i = 0; i < MNC; ++i) {
+      if (CPLX) {
+        tot[i][0] += al_re * acc[i][0] - al_im * acc[i][NP - 1];
+        tot[i][NP - 1] += al_re * acc[i][NP - 1] + al_im * acc[i][0];
+      } else {
+        tot[i][0] += al_re * acc[i][0];
+      }
+    }
+  }
+  // threads -> waves (butterfly), waves -> one thread per output (wave order)
+#pragma unroll
+  for (int i = 0; i < MNC; ++i)
+#pragma unroll
+    for (int c = 0; c < NP; ++c) {
+      double v = tot[i][c];
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+      if (lane == 0) red[wave][i * NP + c] = v;
+    }
+  __syncthreads();
+  if (tid < MN) {
+    double v[NP];
+#pragma unroll
+    for (int c = 0; c < NP; ++c) {
+      v[c] = red[0][tid * NP + c];
+      for (int w = 1; w < kThreads / 64; ++w) v[c] += red[w][tid * NP + c];
+    }
+    const int m = tid % M, n = tid / M;
+    const int64_t o = grp.off_c + m + (int64_t)n * grp.ldc;
+    if (CPLX)
+      reinterpret_cast<double2*>(P)[o] = make_double2(v[0], v[NP - 1]);
+    else
+      P[o] = v[0];
+  }
+}
+
 struct CfgEntry {
   int bm, bn;
 };
@@ -371,6 +471,27 @@ hipError_t launch_ksplit_reduce(hipStream_t s, int cplx, const KsReduce* red, in
   else
     ITG_KS_LAUNCH(0, false);
 #undef ITG_KS_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_ksplit_dot(hipStream_t s, int cplx, int one, int ta, int tb,
+                             const GemmPair* pairs, const GemmGroup* groups, int64_t ngroups,
+                             const void* A, const void* B, void* P, unsigned flags,
+                             double ga_re, double ga_im) {
+  if (ngroups <= 0) return hipSuccess;
+#define ITG_DOT_LAUNCH(c, mn)                                                                  \
+  ITG_LAUNCH((ksplit_dot_kernel<c, mn>), dim3((unsigned)ngroups), dim3(kThreads), 0, s, pairs, \
+             groups, (const double*)A, (const double*)B, (double*)P, ta, tb, flags, ga_re,     \
+             ga_im)
+  if (cplx && one)
+    ITG_DOT_LAUNCH(1, 1);
+  else if (cplx)
+    ITG_DOT_LAUNCH(1, kKsDotMN);
+  else if (one)
+    ITG_DOT_LAUNCH(0, 1);
+  else
+    ITG_DOT_LAUNCH(0, kKsDotMN);
+#undef ITG_DOT_LAUNCH
   return hipGetLastError();
 }
 

@@ -50,9 +50,12 @@ def test_inner_product_and_beta(env):
     rng = np.random.default_rng(12)
     A = rng.standard_normal((64, 3, 2048))
     B = rng.standard_normal((64, 3, 2048))
-    got, _ = run_dense(env, A, (-1, -2, -3), B, (-1, -2, -3))
+    with ND.timer() as t:
+        got, _ = run_dense(env, A, (-1, -2, -3), B, (-1, -2, -3))
     ref = float(np.sum(A * B))
     assert abs(got.reshape(-1)[0] - ref) < 1e-12 * np.sqrt(np.sum(A * A) * np.sum(B * B))
+    if "ksplit_reduce_kernel" in t.results:  # a split plan: the streaming form
+        assert "ksplit_dot_kernel" in t.results and "gemm_f64_kernel" not in t.results
     # beta != 0 adds beta * R; beta == 0 overwrites a NaN-filled R
     A2 = rng.standard_normal((20, 30000))
     B2 = rng.standard_normal((30000, 9))
@@ -126,3 +129,26 @@ def test_graph_replay(env):
         g.launch()
         torch.cuda.synchronize()
         assert torch.equal(R.data, eager), rep
+
+
+@pytest.mark.parametrize("qns", [False, True], ids=["dense", "u1"])
+def test_norm_squared_complex(env, qns):
+    """A * dag(A) = ||A||^2 (test/base/test_qnitensor.jl:937-947 at scale):
+    conj inside the streaming dot kernel, one output element."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(16)
+    if qns:
+        QN, Index = itg.QN, itg.Index
+        i = Index([(QN(q), 700) for q in range(-2, 3)], "i")
+        j = Index([(QN(q), 300) for q in range(-2, 3)], "j")
+        A = itg.random_itensor(QN(0), i, j.dag(), dtype=np.complex128, rng=rng)
+    else:
+        A = itg.random_itensor(itg.Index(900, "i"), itg.Index(700, "j"), dtype=np.complex128,
+                               rng=rng)
+    with ND.timer() as t:
+        C = A * itg.dag(A)
+        torch.cuda.synchronize()
+    assert "ksplit_dot_kernel" in t.results, t.results
+    got = complex(C.tensor.data.reshape(-1)[0].item())
+    ref = float(torch.sum(torch.abs(A.tensor.data) ** 2).item())
+    assert abs(got - ref) < 1e-12 * ref and abs(got.imag) < 1e-12 * ref
