@@ -1274,15 +1274,15 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
 }
 
 int default_gemm_cfg(const ExecPlan& ep, int cplx) {
-  // 128x128 tiles once the typical block is large enough to fill them;
-  // complex keeps to the tiles that fit its doubled accumulators.
+  // Real data: 64x64 tiles whatever the block size. Timed on MI355X against
+  // the other shapes (128x128, 128x64, 64x128) on the two-site network's
+  // GEMMs at chi = 1024 (one 5120 x 4096 x 1024 block), the chi = 4096 U(1)
+  // blocks (about 410 on a side) and a 4096^3 product, 64x64 is the fastest
+  // or within 3 % of it, and 128x128 the slowest by 10-30 %: four waves per
+  // SIMD hide the fill latency that the larger register tile cannot.
+  // Complex data keeps to the tiles that fit its doubled accumulators.
   if (cplx) return ep.max_dim >= 128 ? 2 : 0;
-  double big = 0, tot = 0;
-  for (size_t g = 0; g < ep.h_groups.size(); ++g) {
-    tot += ep.group_cost[g];
-    if (ep.h_groups[g].M >= 128 && ep.h_groups[g].N >= 128) big += ep.group_cost[g];
-  }
-  return (tot > 0 && big >= 0.5 * tot) ? 1 : 0;
+  return 0;
 }
 
 int default_split_cfg(const ExecPlan& ep) {
