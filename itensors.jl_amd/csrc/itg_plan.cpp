@@ -1273,15 +1273,15 @@ std::string build_split_tables(ExecPlan* ep, const BlockList& P, const BlockList
   return "";
 }
 
-int default_gemm_cfg(const ExecPlan& ep, int cplx) {
-  // Real data: 64x64 tiles whatever the block size. Timed on MI355X against
-  // the other shapes (128x128, 128x64, 64x128) on the two-site network's
-  // GEMMs at chi = 1024 (one 5120 x 4096 x 1024 block), the chi = 4096 U(1)
-  // blocks (about 410 on a side) and a 4096^3 product, 64x64 is the fastest
-  // or within 3 % of it, and 128x128 the slowest by 10-30 %: four waves per
-  // SIMD hide the fill latency that the larger register tile cannot.
-  // Complex data keeps to the tiles that fit its doubled accumulators.
-  if (cplx) return ep.max_dim >= 128 ? 2 : 0;
+int default_gemm_cfg(const ExecPlan&, int) {
+  // 64x64 tiles whatever the block size. Timed on MI355X against the other
+  // shapes (128x128, 128x64, 64x128 real; 128x64 complex) on the two-site
+  // network's GEMMs at chi = 1024 (one 5120 x 4096 x 1024 block), the
+  // chi = 4096 U(1) blocks (about 410 on a side) and 4096^3 / 2048^3
+  // products, 64x64 is the fastest or within 3 % of it, and the large tiles
+  // are slower by up to 30 % (128x128 real) and 14 % (128x64 complex): four
+  // waves per SIMD hide the fill latency that the larger register tiles
+  // cannot.
   return 0;
 }
 
