@@ -14,7 +14,6 @@ the relative error of the device results.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
