@@ -287,6 +287,8 @@ __global__ void __launch_bounds__(kThreads)
     const KsReduce r = red[ksplit_block(red, nred, e)];
     const int64_t loc = e - r.e0, mn = (int64_t)r.M * r.N;
     double vr = 0.0, vi = 0.0;
+    // unrolled so that the loads of several slices are in flight at once
+#pragma unroll 8
     for (int q = lane; q < r.S; q += kPer) {
       if (CPLX) {
         const double2 w = reinterpret_cast<const double2*>(P)[r.poff + q * mn + loc];
