@@ -165,3 +165,29 @@ def test_kat_no_blocks_threaded(env, threaded):
     B = itg.blocksparse_itensor(np.array([22.0]), [(2, 2)], (i.prime(), i.dag()))
     C = A * B
     assert C.nnzblocks() == 0 and C.tensor.data.numel() == 0
+
+
+def test_network_threaded_fused_and_oracle(env, threaded):
+    """The two-site network under the threaded order: the fused MPO pair is
+    bit-identical to the two separate steps, and the result (block order,
+    offsets, values) matches the oracle's threaded fold."""
+    torch, itg, ND, O = env
+    from itg.configs import heff_network
+    from itg.network import ContractionSequence
+    net = heff_network(256, qns=True, seed=9, device="cuda")[:5]
+    fused = ContractionSequence(net)
+    plain = ContractionSequence(net, fuse=False)
+    assert list(fused.chains) == [1]
+    a = fused.run("fp64").tensor.data.clone()
+    b = plain.run("fp64").tensor.data.clone()
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    cur = to_oracle(O, net[0])
+    for T in net[1:]:
+        OT = to_oracle(O, T)
+        lA, lB = O.compute_contraction_labels(cur.inds, OT.inds)
+        cur, _ = O.blocksparse_contract(cur, lA, OT, lB, O.contract_labels(lA, lB), threaded=True)
+    res = fused.result
+    # the result's index order is the fold's; compare in the oracle's order
+    assert tuple(i.id for i in res.inds) == tuple(i.id for i in cur.inds)
+    _check(O, res, cur)
