@@ -5,7 +5,8 @@ itg_bs_execute on the same plan and inputs, through the C ABI.
 Two calls in a row on one GPU, the second with a smaller contraction, in
 FP64 MFMA and split precision, real and complex: every output must be bit
 for bit the one itg_bs_execute gives (same kernels, same tables; the scratch
-contents left by the first call must not leak into the second). Also: a
+contents left by the first call must not leak into the second), including
+an inner product (split-K partial sums in the scratch array). Also: a
 plan whose launch tables do not exist yet, and a contraction whose
 workspace exceeds the scratch array, are refused with ITG_ERR_INVALID.
 """
@@ -111,6 +112,33 @@ def test_noalloc_pair_factors(env, prec):
     # a set of factors without tables is refused, not built
     st, msg = _noalloc(env, p, got, A, B, prec, pair_alpha=-fac)
     assert st == ITG_ERR_INVALID and "itg_bs_execute first" in msg, msg
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["f64", "c128"])
+def test_noalloc_inner_product_two_calls(env, cplx):
+    """A(i, dag j) * B(dag i, j) -> a number: the split-K form with the
+    streaming dot kernel, its partial sums in the scratch array; two calls,
+    the second smaller, bit for bit as itg_bs_execute."""
+    torch, itg, _lib, ND = env
+    QN, Index = itg.QN, itg.Index
+    dt = np.complex128 if cplx else np.float64
+    for call, n in enumerate((400, 150)):
+        rng = np.random.default_rng(10 + call)
+        i = Index([(QN(("Sz", q)), n) for q in range(-2, 3)], "i", dir=itg.Out)
+        j = Index([(QN(("Sz", q)), n + 7) for q in range(-2, 3)], "j", dir=itg.Out)
+        A = itg.random_itensor(QN(("Sz", 0)), i, j.dag(), dtype=dt, rng=rng)
+        B = itg.random_itensor(QN(("Sz", 0)), i.dag(), j, dtype=dt, rng=rng)
+        p = _plan(itg, ND, A, B)
+        want = torch.full((p.nnzR,), float("nan"), dtype=A.tensor.data.dtype, device="cuda")
+        with ND.timer() as t:
+            p.execute(want, A.tensor.data, B.tensor.data, prec="fp64")
+            torch.cuda.synchronize()
+        assert "ksplit_dot_kernel" in t.results, t.results
+        got = torch.full_like(want, float("nan"))
+        print(f"[noalloc] inner call {call}: n={n} pairs={p.npairs}", flush=True)
+        st, msg = _noalloc(env, p, got, A, B, "fp64")
+        assert st == 0, msg
+        assert torch.equal(_bits(got), _bits(want))
 
 
 def test_noalloc_refuses_unbuilt_tables(env):
