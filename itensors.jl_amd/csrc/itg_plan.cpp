@@ -702,10 +702,23 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
     ttot += tiles[g];
   }
   if (ttot == 0 || ttot >= kKsTiles) return "";
-  const int64_t smax = std::max<int64_t>(1, kKsTarget / ttot);
+  // dot-like: every block <= kKsDotMN elements; skinny: every block has a
+  // side <= kKsSkinny (matrix times vectors). The skinny kernel gives each
+  // workgroup 256 rows of the long side, so its slices are cut finer
+  // (>= 64 values) to reach the same number of workgroups.
+  bool dot = true, skinny = true;
+  int64_t chunks = 0;
+  for (const GemmGroup& G : ep->h_groups) {
+    if ((int64_t)G.M * G.N > kKsDotMN) dot = false;
+    if (std::min(G.M, G.N) > kKsSkinny) skinny = false;
+    chunks += (std::max(G.M, G.N) + 255) / 256;
+  }
+  skinny = skinny && !dot;
+  const int64_t smax = std::max<int64_t>(1, kKsTarget / (skinny ? chunks : ttot));
+  const int64_t kslice = skinny ? 64 : kKsSlice;
   int64_t partial = 0;
   for (size_t g = 0; g < ng; ++g) {
-    S[g] = std::max<int64_t>(1, std::min<int64_t>(klen[g] / kKsSlice, smax));
+    S[g] = std::max<int64_t>(1, std::min<int64_t>(klen[g] / kslice, smax));
     partial += S[g] * ep->h_groups[g].M * ep->h_groups[g].N;
   }
   while (partial > kKsPartial) {
@@ -763,13 +776,22 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
   tab.h_groups = sg;
   tab.h_pairs = sp;
   std::vector<int2> tiles_tab = xcd_tile_table(tab, gemm_cfg(0));
+  // skinny: (entry, first row) per workgroup, 256 rows of the long side
+  std::vector<int2> rows;
+  if (skinny)
+    for (size_t j = 0; j < sg.size(); ++j) {
+      const int R = sg[j].N <= kKsSkinny ? sg[j].M : sg[j].N;
+      for (int r0 = 0; r0 < R; r0 += 256) rows.push_back(int2{(int)j, r0});
+    }
   hipError_t e;
   if ((e = ep->ks_pairs.upload(sp.data(), std::max<size_t>(sp.size(), 1) * sizeof(GemmPair), s)) !=
           hipSuccess ||
       (e = ep->ks_groups.upload(sg.data(), sg.size() * sizeof(GemmGroup), s)) != hipSuccess ||
       (e = ep->ks_tiles.upload(tiles_tab.data(), tiles_tab.size() * sizeof(int2), s)) !=
           hipSuccess ||
-      (e = ep->ks_red.upload(red.data(), red.size() * sizeof(KsReduce), s)) != hipSuccess)
+      (e = ep->ks_red.upload(red.data(), red.size() * sizeof(KsReduce), s)) != hipSuccess ||
+      (skinny && (e = ep->ks_rows.upload(rows.data(), rows.size() * sizeof(int2), s)) !=
+                     hipSuccess))
     return std::string("split-K table upload failed: ") + hipGetErrorString(e);
   ep->ks_grid = (int64_t)tiles_tab.size();
   ep->ks_elems = poff;
@@ -778,11 +800,12 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
   ep->ks_nred = (int)ng;
   ep->ks_maxs = (int)*std::max_element(S.begin(), S.end());
   ep->ks_ngroups = (int64_t)sg.size();
-  ep->ks_dot = ep->ks_dot1 = true;
-  for (const GemmGroup& G : ep->h_groups) {
-    if ((int64_t)G.M * G.N > kKsDotMN) ep->ks_dot = false;
+  ep->ks_dot = dot;
+  ep->ks_dot1 = dot;
+  for (const GemmGroup& G : ep->h_groups)
     if ((int64_t)G.M * G.N != 1) ep->ks_dot1 = false;
-  }
+  ep->ks_skinny = skinny;
+  ep->ks_nrows = (int64_t)rows.size();
   return "";
 }
 
@@ -1719,6 +1742,11 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
       e = launch_ksplit_dot(s, cplx, ep.ks_dot1, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,
                             (const GemmGroup*)ep.ks_groups.p, ep.ks_ngroups, A, B, w, gflags,
                             a_re, a_im);
+    } else if (ep.ks_skinny) {
+      TimeScope ts(kt, "ksplit_skinny_kernel", s);
+      e = launch_ksplit_skinny(s, cplx, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,
+                               (const GemmGroup*)ep.ks_groups.p, (const int2*)ep.ks_rows.p,
+                               ep.ks_nrows, A, B, w, gflags, a_re, a_im);
     } else {
       TimeScope ts(kt, "gemm_f64_kernel", s);
       e = launch_gemm_f64(s, 0, cplx, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,

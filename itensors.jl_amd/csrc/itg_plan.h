@@ -163,6 +163,9 @@ struct ExecPlan {
   int ks_nred = 0, ks_maxs = 0;
   bool ks_dot = false;      // every output block has <= kKsDotMN elements
   bool ks_dot1 = false;     // ... and exactly one
+  bool ks_skinny = false;   // every output block has a side <= kKsSkinny
+  int64_t ks_nrows = 0;     // skinny: workgroups (entry, first row)
+  DevBuf ks_rows;
   int64_t ks_ngroups = 0;   // (block, slice) entries of the split tables
   DevBuf ks_pairs, ks_groups, ks_tiles, ks_red;
   double flops = 0.0;                                   // real flops (2*M*N*K summed)

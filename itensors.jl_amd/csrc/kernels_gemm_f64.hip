@@ -425,6 +425,102 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Split-K partial blocks of skinny plans (every output block has one side of
+// at most kKsSkinny: matrix times vector or a few vectors). Workgroup w
+// takes 256 consecutive rows of the long side of entry rows[w].x of the
+// split tables, starting at rows[w].y; a thread owns one row and walks the
+// entry's contracted values in order (four in flight), multiplying by the
+// short side's values (the same for the whole workgroup, so they come from
+// the caches), and writes its row of the partial block. Per pair the sums
+// are scaled by that pair's factor times the global one.
+template <int CPLX>
+__global__ void __launch_bounds__(kThreads)
+    ksplit_skinny_kernel(const GemmPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
+                         const int2* __restrict__ rows, const double* __restrict__ Ag,
+                         const double* __restrict__ Bg, double* __restrict__ P, int ta, int tb,
+                         unsigned flags, double ga_re, double ga_im) {
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int CS = kKsSkinny;
+  const int2 t = rows[blockIdx.x];
+  const GemmGroup grp = groups[t.x];
+  const bool mrows = grp.N <= CS;  // the long side is M (A's rows), else N
+  const int R = mrows ? grp.M : grp.N, C = mrows ? grp.N : grp.M;
+  const int r = t.y + (int)threadIdx.x;
+  if (r >= R) return;
+  const double sa = (flags & 1u) ? -1.0 : 1.0;  // conj(A): negate Im(A)
+  const double sb = (flags & 2u) ? -1.0 : 1.0;
+  double tot[CS][NP];
+#pragma unroll
+  for (int c = 0; c < CS; ++c)
+#pragma unroll
+    for (int q = 0; q < NP; ++q) tot[c][q] = 0.0;
+  for (int p = grp.p0; p < grp.p1; ++p) {
+    const GemmPair pr = pairs[p];
+    // element (m, k) of A and (n, k) of B, as the GEMM kernel indexes them
+    auto aoff = [&](int m, int k) {
+      return pr.off_a + (ta ? (int64_t)m * pr.lda + k : (int64_t)k * pr.lda + m);
+    };
+    auto boff = [&](int n, int k) {
+      return pr.off_b + (tb ? (int64_t)k * pr.ldb + n : (int64_t)n * pr.ldb + k);
+    };
+    double acc[CS][NP];
+#pragma unroll
+    for (int c = 0; c < CS; ++c)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) acc[c][q] = 0.0;
+    for (int k0 = 0; k0 < pr.K; k0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + u;
+        if (k >= pr.K) break;
+        const int64_t ox = mrows ? aoff(r, k) : boff(r, k);
+        if (CPLX) {
+          double2 x = reinterpret_cast<const double2*>(mrows ? Ag : Bg)[ox];
+          x.y *= mrows ? sa : sb;
+#pragma unroll
+          for (int c = 0; c < CS; ++c) {
+            if (c >= C) break;
+            double2 y = reinterpret_cast<const double2*>(mrows ? Bg : Ag)[mrows ? boff(c, k)
+                                                                                : aoff(c, k)];
+            y.y *= mrows ? sb : sa;
+            acc[c][0] = fma(x.x, y.x, fma(-x.y, y.y, acc[c][0]));
+            acc[c][NP - 1] = fma(x.x, y.y, fma(x.y, y.x, acc[c][NP - 1]));
+          }
+        } else {
+          const double x = (mrows ? Ag : Bg)[ox];
+#pragma unroll
+          for (int c = 0; c < CS; ++c) {
+            if (c >= C) break;
+            const double y = (mrows ? Bg : Ag)[mrows ? boff(c, k) : aoff(c, k)];
+            acc[c][0] = fma(x, y, acc[c][0]);
+          }
+        }
+      }
+    }
+    const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
+    const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+      if (CPLX) {
+        tot[c][0] += al_re * acc[c][0] - al_im * acc[c][NP - 1];
+        tot[c][NP - 1] += al_re * acc[c][NP - 1] + al_im * acc[c][0];
+      } else {
+        tot[c][0] += al_re * acc[c][0];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CS; ++c) {
+    if (c >= C) break;
+    const int m = mrows ? r : c, n = mrows ? c : r;
+    const int64_t o = grp.off_c + m + (int64_t)n * grp.ldc;
+    if (CPLX)
+      reinterpret_cast<double2*>(P)[o] = make_double2(tot[c][0], tot[c][NP - 1]);
+    else
+      P[o] = tot[c][0];
+  }
+}
+
 struct CfgEntry {
   int bm, bn;
 };
@@ -494,6 +590,22 @@ hipError_t launch_ksplit_dot(hipStream_t s, int cplx, int one, int ta, int tb,
   else
     ITG_DOT_LAUNCH(0, kKsDotMN);
 #undef ITG_DOT_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_ksplit_skinny(hipStream_t s, int cplx, int ta, int tb, const GemmPair* pairs,
+                                const GemmGroup* groups, const int2* rows, int64_t nrows,
+                                const void* A, const void* B, void* P, unsigned flags,
+                                double ga_re, double ga_im) {
+  if (nrows <= 0) return hipSuccess;
+  if (cplx)
+    ITG_LAUNCH((ksplit_skinny_kernel<1>), dim3((unsigned)nrows), dim3(kThreads), 0, s, pairs,
+               groups, rows, (const double*)A, (const double*)B, (double*)P, ta, tb, flags,
+               ga_re, ga_im);
+  else
+    ITG_LAUNCH((ksplit_skinny_kernel<0>), dim3((unsigned)nrows), dim3(kThreads), 0, s, pairs,
+               groups, rows, (const double*)A, (const double*)B, (double*)P, ta, tb, flags,
+               ga_re, ga_im);
   return hipGetLastError();
 }
 

@@ -152,3 +152,49 @@ def test_norm_squared_complex(env, qns):
     got = complex(C.tensor.data.reshape(-1)[0].item())
     ref = float(torch.sum(torch.abs(A.tensor.data) ** 2).item())
     assert abs(got - ref) < 1e-12 * ref and abs(got.imag) < 1e-12 * ref
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["f64", "c128"])
+def test_skinny_matrix_times_vectors(env, cplx):
+    """Matrix times one to four vectors over a long K (every output block has
+    a side of at most 4): the skinny split-K kernel, every orientation, the
+    long side on either operand, alpha and beta."""
+    torch, itg, ND, O = env
+    rng = np.random.default_rng(17)
+    K = 20000
+    for M, N in ((3000, 1), (3, 2500)):
+        for ta, tb, tc in itertools.product([0, 1], repeat=3):
+            A = rand(rng, (K, M) if ta else (M, K), cplx)
+            B = rand(rng, (N, K) if tb else (K, N), cplx)
+            lA = (-1, 1) if ta else (1, -1)
+            lB = (2, -1) if tb else (-1, 2)
+            lR = (2, 1) if tc else (1, 2)
+            R0 = rand(rng, (N, M) if tc else (M, N), cplx)
+            with ND.timer() as t:
+                got, _ = run_dense(env, A, lA, B, lB, lR, alpha=0.5, beta=2.0, R0=R0)
+            assert "ksplit_skinny_kernel" in t.results, (M, N, t.results)
+            ref = O.dense_contract(R0, lR, A, lA, B, lB, 0.5, 2.0)
+            assert relerr(got, ref) < FP64_RTOL, (M, N, ta, tb, tc)
+
+
+def test_skinny_blocksparse_and_conj(env):
+    """Block-sparse A(i, j, k) * dag(B)(dag i, dag j, l) with a short l:
+    several output blocks of one or two columns, conj(B) inside the kernel,
+    against the oracle."""
+    torch, itg, ND, O = env
+    from test_gpu_blocksparse import check_against_oracle
+    QN, Index = itg.QN, itg.Index
+    rng = np.random.default_rng(18)
+    i = Index([(QN(q), 120) for q in range(-2, 3)], "i")
+    j = Index([(QN(q), 90) for q in range(-2, 3)], "j")
+    k = Index([(QN(0), 300), (QN(1), 200)], "k")
+    l = Index([(QN(0), 1), (QN(1), 2)], "l")
+    A = itg.random_itensor(QN(0), i, j.dag(), k, dtype=np.complex128, rng=rng)
+    B = itg.random_itensor(QN(0), i, j.dag(), l, dtype=np.complex128, rng=rng)
+    with ND.timer() as t:
+        C = A * itg.dag(B)
+        torch.cuda.synchronize()
+    assert "ksplit_skinny_kernel" in t.results, t.results
+    Bd = itg.ITensor(ND._with_data(B.tensor, B.tensor.data.conj().resolve_conj()),
+                     tuple(x.dag() for x in B.inds))
+    check_against_oracle(env, A, Bd, C)
