@@ -107,19 +107,25 @@ def test_blocksparse_small_output(env):
         check_against_oracle(env, A, B, C)
 
 
-def test_graph_replay(env):
-    """The split-K launches replay from a graph captured through the C ABI
-    with the eager result (no memset in the path)."""
+@pytest.mark.parametrize("shape", [(8, 6), (2, 3), (3000, 2)], ids=["mfma", "dot", "skinny"])
+def test_graph_replay(env, shape):
+    """Each split-K form (MFMA partial tiles, the streaming dot kernel, the
+    skinny kernel) replays from a graph captured through the C ABI with the
+    eager result (no memset in the path)."""
     torch, itg, ND, O = env
+    M, N = shape
+    kname = {(8, 6): "gemm_f64_kernel", (2, 3): "ksplit_dot_kernel",
+             (3000, 2): "ksplit_skinny_kernel"}[shape]
     rng = np.random.default_rng(15)
-    TA = ND.DenseTensor.from_numpy(rng.standard_normal((8, 100000)))
-    TB = ND.DenseTensor.from_numpy(rng.standard_normal((100000, 6)))
-    R = ND.DenseTensor(torch.empty(48, dtype=torch.float64, device="cuda"), (8, 6))
+    K = 100000 if M * N < 1000 else 20000
+    TA = ND.DenseTensor.from_numpy(rng.standard_normal((M, K)))
+    TB = ND.DenseTensor.from_numpy(rng.standard_normal((K, N)))
+    R = ND.DenseTensor(torch.empty(M * N, dtype=torch.float64, device="cuda"), (M, N))
     run = lambda: ND.contract_(R, (1, 2), TA, (1, -1), TB, (-1, 2))  # noqa: E731
     with ND.timer() as t:
         run()
         torch.cuda.synchronize()
-    assert "ksplit_reduce_kernel" in t.results
+    assert "ksplit_reduce_kernel" in t.results and kname in t.results, t.results
     eager = R.data.clone()
     g = ND.Graph.capture(run)
     for rep in range(3):
