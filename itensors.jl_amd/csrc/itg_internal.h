@@ -339,12 +339,14 @@ hipError_t launch_ksplit_dot(hipStream_t s, int cplx, int one, int ta, int tb,
                              double ga_re, double ga_im);
 // Skinny split-K plans (every output block has a side of at most kKsSkinny:
 // matrix times a few vectors): rows = (entry of `groups`, first row of the
-// long side) per workgroup, 256 rows each.
+// long side) per workgroup, kKsSkinnyRows[kc] rows each. kc: the long side's
+// rows are contiguous along the contracted values.
 constexpr int kKsSkinny = 4;
-hipError_t launch_ksplit_skinny(hipStream_t s, int cplx, int ta, int tb, const GemmPair* pairs,
-                                const GemmGroup* groups, const int2* rows, int64_t nrows,
-                                const void* A, const void* B, void* P, unsigned flags,
-                                double ga_re, double ga_im);
+constexpr int kKsSkinnyRows[2] = {256, 16};
+hipError_t launch_ksplit_skinny(hipStream_t s, int cplx, int kc, int ta, int tb,
+                                const GemmPair* pairs, const GemmGroup* groups, const int2* rows,
+                                int64_t nrows, const void* A, const void* B, void* P,
+                                unsigned flags, double ga_re, double ga_im);
 // max_slices: the largest S of the entries (many slices: a wave per element).
 hipError_t launch_ksplit_reduce(hipStream_t s, int cplx, const KsReduce* red, int nred,
                                 int64_t total, int max_slices, const void* P, void* C,

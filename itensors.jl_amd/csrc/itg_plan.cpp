@@ -707,15 +707,23 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
   // workgroup 256 rows of the long side, so its slices are cut finer
   // (>= 64 values) to reach the same number of workgroups.
   bool dot = true, skinny = true;
-  int64_t chunks = 0;
   for (const GemmGroup& G : ep->h_groups) {
     if ((int64_t)G.M * G.N > kKsDotMN) dot = false;
     if (std::min(G.M, G.N) > kKsSkinny) skinny = false;
-    chunks += (std::max(G.M, G.N) + 255) / 256;
   }
   skinny = skinny && !dot;
+  // skinny: are the long side's rows contiguous along K? A(m, k) is when
+  // ta, B(n, k) when !tb; the first block decides which operand holds the
+  // long side (a plan whose blocks disagree is read correctly either way,
+  // only less coalesced). K-contiguous rows go 16 to a workgroup with lanes
+  // along K and slices of >= 512 values; the others 256 to a workgroup, one
+  // row per thread, slices of >= 64.
+  const bool kc = skinny && (ep->h_groups[0].N <= kKsSkinny ? ep->ta != 0 : ep->tb == 0);
+  const int rows_wg = kKsSkinnyRows[kc ? 1 : 0];
+  int64_t chunks = 0;
+  for (const GemmGroup& G : ep->h_groups) chunks += (std::max(G.M, G.N) + rows_wg - 1) / rows_wg;
   const int64_t smax = std::max<int64_t>(1, kKsTarget / (skinny ? chunks : ttot));
-  const int64_t kslice = skinny ? 64 : kKsSlice;
+  const int64_t kslice = skinny ? (kc ? 512 : 64) : kKsSlice;
   int64_t partial = 0;
   for (size_t g = 0; g < ng; ++g) {
     S[g] = std::max<int64_t>(1, std::min<int64_t>(klen[g] / kslice, smax));
@@ -781,7 +789,7 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
   if (skinny)
     for (size_t j = 0; j < sg.size(); ++j) {
       const int R = sg[j].N <= kKsSkinny ? sg[j].M : sg[j].N;
-      for (int r0 = 0; r0 < R; r0 += 256) rows.push_back(int2{(int)j, r0});
+      for (int r0 = 0; r0 < R; r0 += rows_wg) rows.push_back(int2{(int)j, r0});
     }
   hipError_t e;
   if ((e = ep->ks_pairs.upload(sp.data(), std::max<size_t>(sp.size(), 1) * sizeof(GemmPair), s)) !=
@@ -805,6 +813,7 @@ std::string build_ksplit(ExecPlan* ep, hipStream_t s) {
   for (const GemmGroup& G : ep->h_groups)
     if ((int64_t)G.M * G.N != 1) ep->ks_dot1 = false;
   ep->ks_skinny = skinny;
+  ep->ks_kc = kc;
   ep->ks_nrows = (int64_t)rows.size();
   return "";
 }
@@ -1744,7 +1753,7 @@ hipError_t exec_run(ExecPlan& ep, int cplx, int prec, int cfg, hipStream_t s, vo
                             a_re, a_im);
     } else if (ep.ks_skinny) {
       TimeScope ts(kt, "ksplit_skinny_kernel", s);
-      e = launch_ksplit_skinny(s, cplx, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,
+      e = launch_ksplit_skinny(s, cplx, ep.ks_kc, ep.ta, ep.tb, (const GemmPair*)ep.ks_pairs.p,
                                (const GemmGroup*)ep.ks_groups.p, (const int2*)ep.ks_rows.p,
                                ep.ks_nrows, A, B, w, gflags, a_re, a_im);
     } else {

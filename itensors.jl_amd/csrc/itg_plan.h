@@ -164,6 +164,7 @@ struct ExecPlan {
   bool ks_dot = false;      // every output block has <= kKsDotMN elements
   bool ks_dot1 = false;     // ... and exactly one
   bool ks_skinny = false;   // every output block has a side <= kKsSkinny
+  bool ks_kc = false;       // skinny, long side's rows contiguous along K
   int64_t ks_nrows = 0;     // skinny: workgroups (entry, first row)
   DevBuf ks_rows;
   int64_t ks_ngroups = 0;   // (block, slice) entries of the split tables

@@ -427,13 +427,17 @@ __global__ void __launch_bounds__(kThreads)
 
 // Split-K partial blocks of skinny plans (every output block has one side of
 // at most kKsSkinny: matrix times vector or a few vectors). Workgroup w
-// takes 256 consecutive rows of the long side of entry rows[w].x of the
-// split tables, starting at rows[w].y; a thread owns one row and walks the
-// entry's contracted values in order (four in flight), multiplying by the
-// short side's values (the same for the whole workgroup, so they come from
-// the caches), and writes its row of the partial block. Per pair the sums
+// takes consecutive rows of the long side of entry rows[w].x of the split
+// tables, starting at rows[w].y, and writes those rows of the partial
+// block. KC = 0 (the long side's rows run along the long side in memory):
+// 256 rows, a thread owns one and walks the entry's contracted values in
+// order, four in flight, so a wave's loads are 64 consecutive rows. KC = 1
+// (each row is contiguous along the contracted values): 16 rows, a wave
+// takes four of them with its lanes striding along the contracted values,
+// then adds each row's lane sums with a butterfly. The short side's values
+// are the same for every row (they come from the caches). Per pair the sums
 // are scaled by that pair's factor times the global one.
-template <int CPLX>
+template <int CPLX, int KC>
 __global__ void __launch_bounds__(kThreads)
     ksplit_skinny_kernel(const GemmPair* __restrict__ pairs, const GemmGroup* __restrict__ groups,
                          const int2* __restrict__ rows, const double* __restrict__ Ag,
@@ -441,19 +445,23 @@ __global__ void __launch_bounds__(kThreads)
                          unsigned flags, double ga_re, double ga_im) {
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int CS = kKsSkinny;
+  constexpr int NR = KC ? 4 : 1;  // rows per thread (KC = 1: per wave)
   const int2 t = rows[blockIdx.x];
   const GemmGroup grp = groups[t.x];
   const bool mrows = grp.N <= CS;  // the long side is M (A's rows), else N
   const int R = mrows ? grp.M : grp.N, C = mrows ? grp.N : grp.M;
-  const int r = t.y + (int)threadIdx.x;
-  if (r >= R) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r0 = KC ? t.y + wave * NR : t.y + (int)threadIdx.x;
+  if (r0 >= R) return;  // KC = 1: the whole wave (r0 is the same on every lane)
   const double sa = (flags & 1u) ? -1.0 : 1.0;  // conj(A): negate Im(A)
   const double sb = (flags & 2u) ? -1.0 : 1.0;
-  double tot[CS][NP];
+  double tot[NR][CS][NP];
 #pragma unroll
-  for (int c = 0; c < CS; ++c)
+  for (int j = 0; j < NR; ++j)
 #pragma unroll
-    for (int q = 0; q < NP; ++q) tot[c][q] = 0.0;
+    for (int c = 0; c < CS; ++c)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) tot[j][c][q] = 0.0;
   for (int p = grp.p0; p < grp.p1; ++p) {
     const GemmPair pr = pairs[p];
     // element (m, k) of A and (n, k) of B, as the GEMM kernel indexes them
@@ -463,36 +471,44 @@ __global__ void __launch_bounds__(kThreads)
     auto boff = [&](int n, int k) {
       return pr.off_b + (tb ? (int64_t)k * pr.ldb + n : (int64_t)n * pr.ldb + k);
     };
-    double acc[CS][NP];
+    double acc[NR][CS][NP];
 #pragma unroll
-    for (int c = 0; c < CS; ++c)
+    for (int j = 0; j < NR; ++j)
 #pragma unroll
-      for (int q = 0; q < NP; ++q) acc[c][q] = 0.0;
-    for (int k0 = 0; k0 < pr.K; k0 += 4) {
+      for (int c = 0; c < CS; ++c)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+        for (int q = 0; q < NP; ++q) acc[j][c][q] = 0.0;
+    // KC = 0: k = 0, 1, ... four at a time; KC = 1: k = lane, lane + 64, ...
+    constexpr int U = KC ? 1 : 4;
+    for (int k0 = KC ? lane : 0; k0 < pr.K; k0 += KC ? 64 : U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
         const int k = k0 + u;
-        if (k >= pr.K) break;
-        const int64_t ox = mrows ? aoff(r, k) : boff(r, k);
-        if (CPLX) {
-          double2 x = reinterpret_cast<const double2*>(mrows ? Ag : Bg)[ox];
-          x.y *= mrows ? sa : sb;
+        if (!KC && k >= pr.K) break;
 #pragma unroll
-          for (int c = 0; c < CS; ++c) {
-            if (c >= C) break;
-            double2 y = reinterpret_cast<const double2*>(mrows ? Bg : Ag)[mrows ? boff(c, k)
-                                                                                : aoff(c, k)];
-            y.y *= mrows ? sb : sa;
-            acc[c][0] = fma(x.x, y.x, fma(-x.y, y.y, acc[c][0]));
-            acc[c][NP - 1] = fma(x.x, y.y, fma(x.y, y.x, acc[c][NP - 1]));
-          }
-        } else {
-          const double x = (mrows ? Ag : Bg)[ox];
+        for (int j = 0; j < NR; ++j) {
+          const int rr = min(r0 + j, R - 1);  // rows past R: computed, not stored
+          const int64_t ox = mrows ? aoff(rr, k) : boff(rr, k);
+          if (CPLX) {
+            double2 x = reinterpret_cast<const double2*>(mrows ? Ag : Bg)[ox];
+            x.y *= mrows ? sa : sb;
 #pragma unroll
-          for (int c = 0; c < CS; ++c) {
-            if (c >= C) break;
-            const double y = (mrows ? Bg : Ag)[mrows ? boff(c, k) : aoff(c, k)];
-            acc[c][0] = fma(x, y, acc[c][0]);
+            for (int c = 0; c < CS; ++c) {
+              if (c >= C) break;
+              double2 y = reinterpret_cast<const double2*>(mrows ? Bg : Ag)[mrows ? boff(c, k)
+                                                                                  : aoff(c, k)];
+              y.y *= mrows ? sb : sa;
+              acc[j][c][0] = fma(x.x, y.x, fma(-x.y, y.y, acc[j][c][0]));
+              acc[j][c][NP - 1] = fma(x.x, y.y, fma(x.y, y.x, acc[j][c][NP - 1]));
+            }
+          } else {
+            const double x = (mrows ? Ag : Bg)[ox];
+#pragma unroll
+            for (int c = 0; c < CS; ++c) {
+              if (c >= C) break;
+              const double y = (mrows ? Bg : Ag)[mrows ? boff(c, k) : aoff(c, k)];
+              acc[j][c][0] = fma(x, y, acc[j][c][0]);
+            }
           }
         }
       }
@@ -500,24 +516,39 @@ __global__ void __launch_bounds__(kThreads)
     const double al_re = ga_re * pr.alpha_re - ga_im * pr.alpha_im;
     const double al_im = ga_re * pr.alpha_im + ga_im * pr.alpha_re;
 #pragma unroll
-    for (int c = 0; c < CS; ++c) {
-      if (CPLX) {
-        tot[c][0] += al_re * acc[c][0] - al_im * acc[c][NP - 1];
-        tot[c][NP - 1] += al_re * acc[c][NP - 1] + al_im * acc[c][0];
-      } else {
-        tot[c][0] += al_re * acc[c][0];
-      }
-    }
-  }
+    for (int j = 0; j < NR; ++j)
 #pragma unroll
-  for (int c = 0; c < CS; ++c) {
-    if (c >= C) break;
-    const int m = mrows ? r : c, n = mrows ? c : r;
-    const int64_t o = grp.off_c + m + (int64_t)n * grp.ldc;
-    if (CPLX)
-      reinterpret_cast<double2*>(P)[o] = make_double2(tot[c][0], tot[c][NP - 1]);
-    else
-      P[o] = tot[c][0];
+      for (int c = 0; c < CS; ++c) {
+        double ar = acc[j][c][0], ai = acc[j][c][NP - 1];
+        if (KC) {  // the row's sum over the wave's lanes (a fixed tree)
+#pragma unroll
+          for (int m = 32; m >= 1; m >>= 1) {
+            ar += __shfl_xor(ar, m);
+            if (CPLX) ai += __shfl_xor(ai, m);
+          }
+        }
+        if (CPLX) {
+          tot[j][c][0] += al_re * ar - al_im * ai;
+          tot[j][c][NP - 1] += al_re * ai + al_im * ar;
+        } else {
+          tot[j][c][0] += al_re * ar;
+        }
+      }
+  }
+  if (KC && lane != 0) return;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    if (r0 + j >= R) break;
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+      if (c >= C) break;
+      const int m = mrows ? r0 + j : c, n = mrows ? c : r0 + j;
+      const int64_t o = grp.off_c + m + (int64_t)n * grp.ldc;
+      if (CPLX)
+        reinterpret_cast<double2*>(P)[o] = make_double2(tot[j][c][0], tot[j][c][NP - 1]);
+      else
+        P[o] = tot[j][c][0];
+    }
   }
 }
 
@@ -593,19 +624,24 @@ hipError_t launch_ksplit_dot(hipStream_t s, int cplx, int one, int ta, int tb,
   return hipGetLastError();
 }
 
-hipError_t launch_ksplit_skinny(hipStream_t s, int cplx, int ta, int tb, const GemmPair* pairs,
-                                const GemmGroup* groups, const int2* rows, int64_t nrows,
-                                const void* A, const void* B, void* P, unsigned flags,
-                                double ga_re, double ga_im) {
+hipError_t launch_ksplit_skinny(hipStream_t s, int cplx, int kc, int ta, int tb,
+                                const GemmPair* pairs, const GemmGroup* groups, const int2* rows,
+                                int64_t nrows, const void* A, const void* B, void* P,
+                                unsigned flags, double ga_re, double ga_im) {
   if (nrows <= 0) return hipSuccess;
-  if (cplx)
-    ITG_LAUNCH((ksplit_skinny_kernel<1>), dim3((unsigned)nrows), dim3(kThreads), 0, s, pairs,
-               groups, rows, (const double*)A, (const double*)B, (double*)P, ta, tb, flags,
-               ga_re, ga_im);
+#define ITG_SK_LAUNCH(c, k)                                                                    \
+  ITG_LAUNCH((ksplit_skinny_kernel<c, k>), dim3((unsigned)nrows), dim3(kThreads), 0, s, pairs, \
+             groups, rows, (const double*)A, (const double*)B, (double*)P, ta, tb, flags,      \
+             ga_re, ga_im)
+  if (cplx && kc)
+    ITG_SK_LAUNCH(1, 1);
+  else if (cplx)
+    ITG_SK_LAUNCH(1, 0);
+  else if (kc)
+    ITG_SK_LAUNCH(0, 1);
   else
-    ITG_LAUNCH((ksplit_skinny_kernel<0>), dim3((unsigned)nrows), dim3(kThreads), 0, s, pairs,
-               groups, rows, (const double*)A, (const double*)B, (double*)P, ta, tb, flags,
-               ga_re, ga_im);
+    ITG_SK_LAUNCH(0, 0);
+#undef ITG_SK_LAUNCH
   return hipGetLastError();
 }
 
