@@ -58,6 +58,10 @@ def parse():
                     help="run only the block-sparse permutedims measurement (the chi=4096 "
                          "intermediate Z into the order the last contraction reads) and print "
                          "its JSON")
+    ap.add_argument("--ops-only", action="store_true",
+                    help="run only the common-contraction measurements (inner products, "
+                         "matrix times vector in both storage orders, an outer product, one "
+                         "block-sparse GEMM step; eager, default tiles) and print their JSON")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="(internal) run step 1 in every mode a few times plus a 256 MiB copy "
                          "and exit; the workload of the counter passes")
@@ -575,6 +579,53 @@ def diag_config(args, torch, dev):
     return r
 
 
+def ops_config(args, torch, dev):
+    """Common contractions outside the two-site apply, eager (A * B with the
+    default tiles, no autotuning), FP64, kernel time from the library's
+    timer (median of 5 after 2 warm-ups), with the algorithmic bytes or
+    flops behind each rate: <psi|psi> of the chi = 4096 U(1) psi, a 4096^2
+    dense inner product, a 4096^2 matrix times a vector in both storage
+    orders, the outer product of two 4096-vectors and the apply's L * psi."""
+    import numpy as np
+    import itg
+    from itg import ndtensors as ND
+    from itg.configs import heff_network
+
+    L, psi, _W1, _W2, _R, _ix = heff_network(4096, qns=True, seed=0, device=dev)
+    rng = np.random.default_rng(2)
+    x, y, z = itg.Index(4096, "x"), itg.Index(4096, "y"), itg.Index(4096, "z")
+    A = itg.random_itensor(x, y, rng=rng)
+    vy, vx = itg.random_itensor(y, rng=rng), itg.random_itensor(x, rng=rng)
+    u = itg.random_itensor(z, rng=rng)
+    nA = 4096 * 4096 * 8
+    cases = [
+        ("inner_psi_dag_psi_u1_chi4096", lambda: psi * itg.dag(psi), "bytes",
+         2 * psi.tensor.data.numel() * 8),
+        ("inner_dense_4096sq", lambda: A * A, "bytes", nA),
+        ("matvec_rows_along_long_side", lambda: A * vy, "bytes", nA),
+        ("matvec_rows_along_contracted", lambda: A * vx, "bytes", nA),
+        ("outer_4096_x_4096", lambda: u * vy, "bytes", nA),
+        ("L_psi_u1_chi4096", lambda: L * psi, "flops", None),
+    ]
+    out = {}
+    for name, fn, kind, amount in cases:
+        ts, names = [], None
+        for it in range(7):
+            with ND.timer(dev) as t:
+                fn()
+                torch.cuda.synchronize()
+            if it >= 2:
+                ts.append(sum(v[1] for v in t.results.values()))
+                names = sorted(t.results)
+        ms = float(np.median(ts))
+        r = {"ms": ms, "kernels": names}
+        if kind == "bytes":
+            r["GBps"] = amount / (ms * 1e-3) / 1e9
+        out[name] = r
+        log(args, f"[bench] {name}: {json.dumps(r)}")
+    return out
+
+
 def permute_config(args, torch, dev):
     """Block-sparse permutedims at chi = 4096: the intermediate Z(a', b, s1',
     s2', w'') of the plain left-to-right fold permuted into the order the
@@ -696,11 +747,12 @@ def main():
     if args.traffic_probe:
         traffic_probe(args)
         return
-    if args.diag_only or args.permute_only:
+    if args.diag_only or args.permute_only or args.ops_only:
         import torch
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
-        r = (diag_config if args.diag_only else permute_config)(args, torch, dev)
+        r = (diag_config if args.diag_only else
+             permute_config if args.permute_only else ops_config)(args, torch, dev)
         if args.calib_copy_mib > 0:
             _calib_copy(torch, dev, args.calib_copy_mib)
         print(json.dumps(r))
